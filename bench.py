@@ -1,0 +1,251 @@
+#!/usr/bin/env python3
+"""Benchmark of the reweight-and-fill + LLH hot path on one MI355X.
+
+Workload (default): 5M events in 4 samples (2M/1.5M/1M/0.5M) of 30x50 bins,
+300 parameters, 12 dials per event from the seven-type mix (uniform, general,
+compact splines, graphs, norms, tabulated, kriged), Barlow-Beeston OA2021
+likelihood against the Asimov histogram -- seeded synthetic data from
+gundam_amd/csrc/synth.cpp (workload 2).
+
+A step is one LLH evaluation as a fitter issues it: upload the parameter
+point and its lookup tables, mirror/clamp, evaluate every dial, form the
+event weights, fill the histograms, evaluate the LLH and fetch it to the host.
+The device tables are resident in HBM before the timed region starts.
+
+Prints one JSON line (rank 0): the plain FP64 path as "value", plus the
+batched path (K points per launch), the fill kernel's roofline position and
+the CPU oracle baseline on a bounded sample.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def read_traffic(path):
+    """HBM bytes per fill launch measured by rocprofv3 --pmc (FETCH_SIZE +
+    WRITE_SIZE, scaled by the copy-kernel calibration), from a committed
+    profile summary; None when absent."""
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
+def cpu_baseline(workload, n_sample, n_full, seed, threads, evals=20):
+    from gundam_amd import Workload
+    from oracle import pyoracle
+
+    wl = Workload(workload, n_events=n_sample, seed=seed)
+    t = wl.tables
+    t.bin[:] = pyoracle.bin_events(wl.vars, wl.edges)
+    orc = pyoracle.Oracle(t, n_threads=threads)
+    pts = wl.random_points(evals + 3, 0.5, seed=seed + 1)
+    tabs = wl.table_values(pts) if t.table_len else None
+    _, data, _ = orc.propagate(wl.nominal(), want_weights=False)
+    times = []
+    for i, th in enumerate(pts):
+        t0 = time.perf_counter()
+        if tabs is not None:
+            orc.set_table_values(tabs[i])
+        _, s, e = orc.propagate(th, want_weights=False)
+        pyoracle.llh("barlow", data, s, e, th, wl.prior_mean, wl.prior_sigma)
+        dt = time.perf_counter() - t0
+        if i >= 3:
+            times.append(dt)
+    med = float(np.median(times))
+    binned = int((t.bin >= 0).sum())
+    return {
+        "value": (1.0 / med) * (n_sample / n_full),
+        "unit": "LLH evals/s (scaled to the full event count)",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"workload {workload} restricted to {n_sample} events "
+                   f"({binned} binned), median of {len(times)} evaluations "
+                   f"after 3 warm-ups = {med*1e3:.2f} ms, scaled linearly by "
+                   f"{n_sample}/{n_full} events"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", type=int, default=2)
+    ap.add_argument("--events", type=int, default=0, help="0: full size")
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--batch", type=int, default=64,
+                    help="points per launch for the batched figure (0: skip)")
+    ap.add_argument("--cpu-sample", type=int, default=100000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--tile-words", type=int, default=0)
+    ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--seed", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; "
+                          "--gpus must be 1 (multi-GPU sharding is a design "
+                          "note, not built)"}))
+        sys.exit(2)
+
+    from gundam_amd import CacheManager, Workload
+
+    t0 = time.time()
+    wl = Workload(args.workload, n_events=args.events, seed=args.seed)
+    t = wl.tables
+    log(f"generated workload {args.workload}: {t.n_events} events, "
+        f"{t.n_dials} dials, {t.n_bins} bins in {time.time()-t0:.1f}s")
+    t0 = time.time()
+    wl.bin_events_on_device(0)
+    log(f"binned on device in {time.time()-t0:.1f}s, "
+        f"{int((t.bin < 0).sum())} unbinned")
+
+    word = 8 if args.precision == "fp64" else 4
+    alg_bytes = t.algorithmic_bytes(word)
+    fill_bytes = alg_bytes - 16 * t.n_bins  # the finalize kernel's share
+
+    K = max(1, args.batch)
+    t0 = time.time()
+    cm = CacheManager(0, precision=args.precision, block_threads=args.block,
+                      tile_words=args.tile_words,
+                      blocks_per_cu=args.blocks_per_cu)
+    cm.build(t, max_points=K)
+    counts = cm.counts()
+    log(f"built in {time.time()-t0:.1f}s: {counts}, resident "
+        f"{cm.resident_memory()/2**30:.2f} GiB, launch {cm.launch_config()}")
+
+    nominal = wl.nominal()
+    cm.propagate_parameters(nominal, wl.table_values(nominal))
+    data = cm.hist(0)[0].copy()  # Asimov data at nominal
+    cm.set_likelihood("barlow", data, wl.prior_mean, wl.prior_sigma)
+
+    n_pts = args.warmup + args.steps
+    pts = wl.random_points(n_pts, 0.5, seed=args.seed + 17)
+    tabs = wl.table_values(pts) if t.table_len else None
+
+    def one(i):
+        cm.set_parameters(pts[i])
+        if tabs is not None:
+            cm.set_table_values(tabs[i])
+        cm.fill()
+        return cm.llh()[0][0]
+
+    for i in range(args.warmup):
+        one(i)
+    cm.enable_timing(True)
+    t0 = time.perf_counter()
+    llhs = [one(args.warmup + i) for i in range(args.steps)]
+    elapsed = time.perf_counter() - t0
+    tm = cm.timing()
+    cm.enable_timing(False)
+    if not np.all(np.isfinite(llhs)):
+        raise RuntimeError("non-finite LLH in the timed region")
+    ms_per_step = elapsed / args.steps * 1e3
+    evals_per_s = args.steps / elapsed
+    fill_ms = tm["fill_ms"] / max(1, tm["count"])
+    fin_ms = tm["finalize_ms"] / max(1, tm["count"])
+    achieved = fill_bytes / (fill_ms * 1e-3) / 1e9
+    log(f"plain: {ms_per_step:.3f} ms/eval, fill kernel {fill_ms:.3f} ms "
+        f"({achieved:.0f} GB/s), finalize {fin_ms:.3f} ms")
+
+    batched = None
+    if args.batch > 1:
+        bp = wl.random_points(K, 0.5, seed=args.seed + 99)
+        btab = wl.table_values(bp) if t.table_len else None
+        cm.set_parameters(bp)
+        if btab is not None:
+            cm.set_table_values(btab)
+        cm.fill()
+        cm.llh()
+        reps = 3
+        cm.enable_timing(True)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            cm.set_parameters(bp)
+            if btab is not None:
+                cm.set_table_values(btab)
+            cm.fill()
+            cm.llh()
+        el = (time.perf_counter() - t0) / reps
+        tmb = cm.timing()
+        cm.enable_timing(False)
+        batched = {"points_per_launch": K, "evals_per_s": K / el,
+                   "latency_ms_per_batch": el * 1e3,
+                   "fill_kernel_ms": tmb["fill_ms"] / max(1, tmb["count"])}
+        log(f"batched K={K}: {K/el:.1f} evals/s, {el*1e3:.2f} ms/batch")
+
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        n_full = t.n_events
+        sample = min(args.cpu_sample, n_full)
+        log(f"cpu baseline: oracle on {sample} events, {threads} threads")
+        cpu = cpu_baseline(args.workload, sample, n_full, args.seed, threads)
+
+    traffic = read_traffic(os.path.join(
+        ROOT, "profiles", f"fill_kernel_workload{args.workload}_{args.precision}_traffic.txt"))
+    out = {
+        "metric": "LLH evals/s, 5M events x 300 dials: plain vs batched vs surrogate; % HBM peak",
+        "value": evals_per_s,
+        "unit": "LLH evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if args.precision == "fp64" else "f32 dials, f64 sums",
+        "data": "synthetic (seeded generator, gundam_amd/csrc/synth.cpp)",
+        "config": {
+            "workload": ("plain reweight+fill+BB LLH: 5M events in 4 samples "
+                         "(2M/1.5M/1M/0.5M) x 1500 bins, 300 parameters, 12 "
+                         "dials/event (uniform/general/compact splines, graphs, "
+                         "norms, tabulated, kriged)") if args.workload == 2 and not args.events
+                        else f"workload {args.workload}, {t.n_events} events",
+            "events": int(t.n_events),
+            "binned_events": int(counts["events"]),
+            "dials": int(t.n_dials),
+            "bins": int(t.n_bins),
+            "parameters": int(t.n_parameters),
+            "precision": args.precision,
+            "launch": cm.launch_config(),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "kernel": "fill_kernel",
+            "algorithmic_bytes_per_launch": fill_bytes,
+            "avg_kernel_ms": fill_ms,
+        },
+        "batched": batched,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

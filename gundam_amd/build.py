@@ -1,0 +1,55 @@
+"""Build the native libraries in-tree (no JIT cache): the HIP library for
+gfx950 with hipcc and the host workload generator with g++."""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+# -ffp-contract=off: every FP64 operation rounds as the reference's x86-64
+# build does (no fused multiply-add), which the parity tests rely on.
+# packed-fp32-ops is disabled for all HIP code.
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+             "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang",
+             "-packed-fp32-ops", "-Wall", "-Wno-unused-function"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT,
+                       text=True)
+    out = "\n".join(l for l in r.stdout.splitlines()
+                    if "packed-fp32-ops" not in l and "is not a recognized feature" not in l)
+    if r.returncode != 0:
+        sys.stderr.write(out + "\n")
+        raise RuntimeError("build failed: " + " ".join(cmd))
+    return out
+
+
+def build_gcm() -> str:
+    out = os.path.join(HERE, "libgcm.so")
+    tmp = out + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-o", tmp, os.path.join(CSRC, "gcm.hip")])
+    os.replace(tmp, out)
+    return out
+
+
+def build_synth() -> str:
+    out = os.path.join(HERE, "libgcm_synth.so")
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-o", tmp,
+          os.path.join(CSRC, "synth.cpp"), "-lpthread"])
+    os.replace(tmp, out)
+    return out
+
+
+def build_all() -> None:
+    build_synth()
+    build_gcm()
+
+
+if __name__ == "__main__":
+    build_all()

@@ -1,0 +1,195 @@
+"""Python mirror of the reference's ``Cache::Manager`` contract
+(src/CacheManager/include/CacheManager.h:56-410) over the C ABI.
+
+    Build(tables)                       -> CacheManager.build
+    Fill() -> deferred future           -> CacheManager.fill (returns Future)
+    PropagateParameters()               -> CacheManager.propagate_parameters
+    CopyHistogramContents()             -> CacheManager.copy_histogram_contents
+    CopyEventWeights()                  -> CacheManager.copy_event_weights
+    GetResidentMemory()                 -> CacheManager.resident_memory
+    SetIsEventWeightCopyEnabled(bool)   -> CacheManager.set_event_weight_copy
+
+The likelihood that the reference evaluates on the host after the copy-back
+(JointProbabilityBase.h:46-54, LikelihoodInterface.cpp:165-215) is evaluated
+on the device and fetched with ``llh()``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _native as N
+from .tables import Tables
+
+
+class Future:
+    """A deferred result, like the std::launch::deferred future that
+    Cache::Manager::Fill returns (CacheManager.cpp:804-809)."""
+
+    def __init__(self, mgr: "CacheManager"):
+        self._mgr = mgr
+
+    def valid(self) -> bool:
+        return self._mgr is not None
+
+    def get(self) -> bool:
+        m, self._mgr = self._mgr, None
+        if m is None:
+            raise RuntimeError("future already consumed")
+        m.sync()
+        return True
+
+
+class CacheManager:
+    def __init__(self, device: int = 0, precision: str = "fp64",
+                 block_threads: int = 0, tile_words: int = 0,
+                 blocks_per_cu: int = 0):
+        self._lib = N.lib()
+        h = C.c_void_p()
+        N.check(self._lib.gcm_create(int(device), C.byref(h)))
+        self._h = h
+        self.device = device
+        prec = {"fp64": N.PRECISION_FP64, "fp32": N.PRECISION_FP32}[precision]
+        N.check(self._lib.gcm_set_precision(self._h, prec))
+        self.precision = precision
+        if block_threads or tile_words or blocks_per_cu:
+            self.set_launch(block_threads, tile_words, blocks_per_cu)
+        self.n_bins = 0
+        self.n_parameters = 0
+        self.n_events = 0
+        self.table_len = 0
+        self.k_points = 0
+        self.max_points = 1
+
+    # -- configuration ----------------------------------------------------
+    def set_launch(self, block_threads=0, tile_words=0, blocks_per_cu=0):
+        cfg = N.GcmLaunchConfig(int(block_threads), int(tile_words),
+                                int(blocks_per_cu), 0)
+        N.check(self._lib.gcm_set_launch_config(self._h, C.byref(cfg)))
+
+    def launch_config(self) -> dict:
+        cfg = N.GcmLaunchConfig()
+        N.check(self._lib.gcm_get_launch_config(self._h, C.byref(cfg)))
+        return {"block_threads": cfg.block_threads, "tile_words": cfg.tile_words,
+                "blocks_per_cu": cfg.blocks_per_cu}
+
+    # -- Build --------------------------------------------------------------
+    def build(self, tables: Tables, max_points: int = 1) -> None:
+        t = tables.as_c()
+        N.check(self._lib.gcm_build(self._h, C.byref(t), int(max_points)))
+        self.n_bins = tables.n_bins
+        self.n_parameters = tables.n_parameters
+        self.n_events = tables.n_events
+        self.table_len = tables.table_len
+        self.max_points = max_points
+
+    def counts(self) -> dict:
+        e, d, t = C.c_int64(), C.c_int64(), C.c_int64()
+        N.check(self._lib.gcm_counts(self._h, C.byref(e), C.byref(d), C.byref(t)))
+        return {"events": e.value, "dials": d.value, "tiles": t.value}
+
+    def resident_memory(self) -> int:
+        b = C.c_int64()
+        N.check(self._lib.gcm_resident_bytes(self._h, C.byref(b)))
+        return b.value
+
+    def set_event_weight_copy(self, enable: bool) -> None:
+        N.check(self._lib.gcm_enable_weights(self._h, int(bool(enable))))
+
+    def set_likelihood(self, kind: str, data: np.ndarray,
+                       prior_mean: Optional[np.ndarray] = None,
+                       prior_sigma: Optional[np.ndarray] = None,
+                       use_poisson_likelihood: bool = False,
+                       fractional_error_limit: float = 1.0e150,
+                       expected_value_minimum: float = -1.0) -> None:
+        cfg = N.GcmLlhConfig(
+            {"poisson": N.LLH_POISSON, "barlow": N.LLH_BARLOW_OA2021}[kind],
+            int(use_poisson_likelihood), float(fractional_error_limit),
+            float(expected_value_minimum))
+        data = np.ascontiguousarray(data, dtype=np.float64)
+        if data.shape != (self.n_bins,):
+            raise ValueError("data histogram has the wrong size")
+        pm = ps = None
+        if prior_mean is not None:
+            pm = np.ascontiguousarray(prior_mean, dtype=np.float64)
+            ps = np.ascontiguousarray(prior_sigma, dtype=np.float64)
+        self._keep_llh = (data, pm, ps)
+        N.check(self._lib.gcm_set_likelihood(
+            self._h, C.byref(cfg), data.ctypes.data,
+            None if pm is None else pm.ctypes.data,
+            None if ps is None else ps.ctypes.data))
+
+    # -- per evaluation -------------------------------------------------------
+    def set_parameters(self, theta: np.ndarray) -> None:
+        th = np.ascontiguousarray(np.atleast_2d(theta), dtype=np.float64)
+        N.check(self._lib.gcm_set_parameters(self._h, th.ctypes.data,
+                                             th.shape[1], th.shape[0]))
+        self.k_points = th.shape[0]
+
+    def set_table_values(self, values: np.ndarray) -> None:
+        v = np.ascontiguousarray(np.atleast_2d(values), dtype=np.float64)
+        N.check(self._lib.gcm_set_table_values(self._h, v.ctypes.data,
+                                               v.shape[1], v.shape[0]))
+
+    def fill(self, stream: Optional[int] = None) -> Future:
+        N.check(self._lib.gcm_fill(self._h, stream))
+        return Future(self)
+
+    def sync(self) -> None:
+        N.check(self._lib.gcm_sync(self._h))
+
+    def propagate_parameters(self, theta, table_values=None) -> bool:
+        self.set_parameters(theta)
+        if table_values is not None and self.table_len:
+            self.set_table_values(table_values)
+        return self.fill().get()
+
+    # -- fetch ----------------------------------------------------------------
+    def hist(self, k: int = 0):
+        s = np.empty(self.n_bins)
+        s2 = np.empty(self.n_bins)
+        N.check(self._lib.gcm_get_hist(self._h, int(k), s.ctypes.data,
+                                       s2.ctypes.data))
+        return s, s2
+
+    def copy_histogram_contents(self, k: int = 0):
+        """(sumWeights, sqrtSumSqWeights) per bin, as
+        CacheSampleHistFiller::copyHistogram fills them
+        (CacheSampleHistFiller.cpp:9-19)."""
+        s, s2 = self.hist(k)
+        return s, np.sqrt(s2)
+
+    def copy_event_weights(self, out: Optional[np.ndarray] = None):
+        if out is None:
+            out = np.full(self.n_events, np.nan)
+        N.check(self._lib.gcm_get_weights(self._h, out.ctypes.data))
+        return out
+
+    def llh(self):
+        k = max(1, self.k_points)
+        tot, st, pen = np.empty(k), np.empty(k), np.empty(k)
+        N.check(self._lib.gcm_get_llh(self._h, tot.ctypes.data, st.ctypes.data,
+                                      pen.ctypes.data))
+        return tot, st, pen
+
+    def enable_timing(self, enable: bool = True) -> None:
+        N.check(self._lib.gcm_enable_timing(self._h, int(bool(enable))))
+
+    def timing(self) -> dict:
+        f, z, n = C.c_double(), C.c_double(), C.c_int64()
+        N.check(self._lib.gcm_get_timing(self._h, C.byref(f), C.byref(z),
+                                         C.byref(n)))
+        return {"fill_ms": f.value, "finalize_ms": z.value, "count": n.value}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.gcm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,1257 @@
+// gcm.hip -- MI355X (gfx950) event-reweighting cache: device layout, HIP
+// kernels and the C ABI declared in include/gcm.h.
+//
+// Device layout (built once by gcm_build, resident in HBM):
+//   * Binned events are stably sorted by global bin, so each bin's events are
+//     contiguous and keep the caller's order inside the bin.
+//   * The sorted events are cut into tiles of whole events (<= 64 events,
+//     <= max_dials dials, records <= tile_bytes).  A tile's records are one
+//     contiguous, 16-byte aligned run of words, ordered type-major so the
+//     lanes of a wave evaluate the same dial type; every dial carries a 64-bit
+//     descriptor {input, type, knots, event-major slot, record offset}.
+//   * Per tile, the distinct bins of its events are "segments"; each segment
+//     owns one partial-sum slot.  Slots are numbered tile by tile, so the
+//     slots of one bin are contiguous (bin_pbeg).
+//
+// One evaluation (gcm_fill) is three launches on one stream:
+//   prologue  : mirror each dial input (DialInputBuffer.cpp:56-70) and the
+//               prior penalty, per parameter point.
+//   fill      : persistent blocks stream tiles HBM -> LDS with 16-byte loads,
+//               evaluate every dial from LDS, clamp (DialResponseSupervisor),
+//               form each event's product in dial-list order, cap, times the
+//               base weight (EventDialCache.cpp:156-171), then a wave-64
+//               segmented scan by bin writes one (sum w, sum w^2) partial per
+//               segment.  No atomics on the data path.
+//   finalize  : per bin, sum the partial slots in slot order (deterministic),
+//               evaluate the bin's LLH, reduce per block, and the last block
+//               of each point sums the block results in order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/gcm.h"
+#include "dial_eval.hpp"
+
+namespace gcm {
+
+constexpr int kWave = 64;
+constexpr int kMaxTileEvents = 64;
+constexpr int kDefaultBlock = 256;
+constexpr int kDefaultTileWords = 4096;  // 32 KiB of records per tile
+constexpr int kDefaultBlocksPerCU = 3;
+constexpr int kFinalizeBlock = 256;
+
+struct TileHdr {
+  uint32_t first_event;
+  uint32_t first_dial;
+  uint64_t rec_begin;  // in record words of the precision's type
+  uint32_t rec_words;
+  uint16_t n_events;
+  uint16_t n_dials;
+  uint32_t first_partial;
+  uint32_t max_dials_unused;
+};
+static_assert(sizeof(TileHdr) == 32, "tile header is 32 bytes");
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string& msg) {
+  g_error = msg;
+  return code;
+}
+
+#define GCM_HIP(call)                                                        \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return fail(GCM_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// Per point k (one block each): mirrored input values and the prior penalty.
+__global__ void prologue_kernel(const double* __restrict__ theta, int n_par,
+                                const int32_t* __restrict__ in_par,
+                                const double* __restrict__ in_mlo,
+                                const double* __restrict__ in_mrange,
+                                int n_inputs, double* __restrict__ xin,
+                                const double* __restrict__ prior_mean,
+                                const double* __restrict__ prior_sigma,
+                                double* __restrict__ penalty) {
+  const int k = blockIdx.x;
+  const double* th = theta + (size_t)k * n_par;
+  for (int i = threadIdx.x; i < n_inputs; i += blockDim.x) {
+    double v = th[in_par[i]];
+    const double lo = in_mlo[i];
+    if (!isnan(lo)) {  // DialInputBuffer.cpp:56-70
+      const double range = in_mrange[i];
+      v = fabs(fmod(v - lo, 2 * range));
+      if (v > range) {
+        v -= 2 * range;
+        v = -v;
+      }
+      v += lo;
+    }
+    xin[(size_t)k * n_inputs + i] = v;
+  }
+  if (threadIdx.x == 0) {
+    // LikelihoodInterface.cpp:186-206 with a diagonal covariance, summed in
+    // parameter order.
+    double pen = 0;
+    if (prior_sigma) {
+      for (int p = 0; p < n_par; ++p) {
+        const double s = prior_sigma[p];
+        if (!(s > 0)) continue;
+        const double z = (th[p] - prior_mean[p]) / s;
+        pen += z * z;
+      }
+    }
+    penalty[k] = pen;
+  }
+}
+
+// Knuth two-sum: s + e == a + b exactly.
+__device__ __forceinline__ void two_sum(double a, double b, double& s,
+                                        double& e) {
+  s = a + b;
+  const double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+struct FillArgs {
+  const TileHdr* tiles;
+  int64_t n_tiles;
+  const uint64_t* desc;
+  const void* rec;
+  const double* ev_base;
+  const int32_t* ev_bin;
+  const uint32_t* ev_dial;
+  const int32_t* ev_result;
+  const double* xin;
+  int n_inputs;
+  const int32_t* in_flags;
+  const double* in_min;
+  const double* in_max;
+  const double* tab;
+  int64_t table_len;
+  double cap;
+  int k_points;
+  double* psum;
+  double* psum2;
+  double* pcomp;
+  double* pcomp2;
+  int64_t n_partials;
+  double* weights_out;
+  int* flags;
+  int tile_words;  // record words per tile (capacity, type R)
+  int max_dials;
+};
+
+// R: record/evaluation type.  kComp: compensated (two-sum) bin partials.
+template <typename R, bool kComp>
+__global__ void __launch_bounds__(512) fill_kernel(FillArgs a) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  R* s_rec = reinterpret_cast<R*>(lds);
+  uint64_t* s_desc =
+      reinterpret_cast<uint64_t*>(lds + (size_t)a.tile_words * sizeof(R));
+  double* s_resp = reinterpret_cast<double*>(s_desc + a.max_dials);
+  const R* rec = reinterpret_cast<const R*>(a.rec);
+
+  for (int64_t t = blockIdx.x; t < a.n_tiles; t += gridDim.x) {
+    const TileHdr h = a.tiles[t];
+    // Stage the tile's records and descriptors: 16-byte coalesced loads.
+    {
+      const int n16 = (int)((h.rec_words * sizeof(R) + 15) / 16);
+      const uint4* src = reinterpret_cast<const uint4*>(rec + h.rec_begin);
+      uint4* dst = reinterpret_cast<uint4*>(s_rec);
+      for (int i = threadIdx.x; i < n16; i += blockDim.x) dst[i] = src[i];
+      const int nd2 = (h.n_dials + 1) / 2;
+      const uint4* dsrc = reinterpret_cast<const uint4*>(a.desc + h.first_dial);
+      uint4* ddst = reinterpret_cast<uint4*>(s_desc);
+      for (int i = threadIdx.x; i < nd2; i += blockDim.x) ddst[i] = dsrc[i];
+    }
+    __syncthreads();
+    for (int k = 0; k < a.k_points; ++k) {
+      const double* xk = a.xin + (size_t)k * a.n_inputs;
+      const double* tk = a.tab + (size_t)k * a.table_len;
+      for (int i = threadIdx.x; i < h.n_dials; i += blockDim.x) {
+        const uint64_t d = s_desc[i];
+        const int in = (int)(d & 0xffff);
+        const int type = (int)((d >> 16) & 0xf);
+        const int n = (int)((d >> 20) & 0xff);
+        const int dest = (int)((d >> 32) & 0xffff);
+        const int off = (int)(d >> 48);
+        double r = eval_dial<R>(type, n, s_rec + off, xk[in], a.in_flags[in], tk);
+        // DialResponseSupervisor::process; unset bounds are stored as -+inf.
+        const double lo = a.in_min[in], hi = a.in_max[in];
+        if (r < lo) r = lo;
+        else if (r > hi) r = hi;
+        s_resp[dest] = r;
+      }
+      __syncthreads();
+      if (threadIdx.x < kWave) {
+        const int lane = threadIdx.x;
+        double w = 0.0, w2 = 0.0, c = 0.0, c2 = 0.0;
+        int bin = INT_MAX;
+        if (lane < h.n_events) {
+          const int64_t e = (int64_t)h.first_event + lane;
+          const uint32_t ed = a.ev_dial[e];
+          const int b0 = (int)(ed & 0xffff), cnt = (int)(ed >> 16);
+          double temp = 1;
+          for (int j = 0; j < cnt; ++j) temp *= s_resp[b0 + j];
+          if (temp > a.cap) temp = a.cap;
+          w = a.ev_base[e];
+          w *= temp;
+          bin = a.ev_bin[e];
+          if (k == 0 && a.weights_out) a.weights_out[a.ev_result[e]] = w;
+          if (isnan(w)) atomicOr(a.flags, 1);
+          w2 = w * w;
+        }
+        // Inclusive segmented scan over the wave; bins are sorted, so equal
+        // bins are one contiguous segment.
+#pragma unroll
+        for (int dlt = 1; dlt < kWave; dlt <<= 1) {
+          const double ws = __shfl_up(w, dlt);
+          const double w2s = __shfl_up(w2, dlt);
+          const int bs = __shfl_up(bin, dlt);
+          if (kComp) {
+            const double cs = __shfl_up(c, dlt);
+            const double c2s = __shfl_up(c2, dlt);
+            if (lane >= dlt && bs == bin) {
+              double s, e;
+              two_sum(ws, w, s, e);
+              w = s;
+              c = c + cs + e;
+              two_sum(w2s, w2, s, e);
+              w2 = s;
+              c2 = c2 + c2s + e;
+            }
+          } else {
+            if (lane >= dlt && bs == bin) {
+              w = ws + w;
+              w2 = w2s + w2;
+            }
+          }
+        }
+        const int prev = __shfl_up(bin, 1);
+        const int next = __shfl_down(bin, 1);
+        const bool valid = lane < h.n_events;
+        const bool head = valid && (lane == 0 || prev != bin);
+        const bool tail = valid && (lane == h.n_events - 1 || next != bin);
+        const unsigned long long heads = __ballot(head);
+        const int seg = __popcll(heads & ((2ull << lane) - 1ull)) - 1;
+        if (tail) {
+          const size_t slot = (size_t)k * a.n_partials + h.first_partial + seg;
+          a.psum[slot] = w;
+          a.psum2[slot] = w2;
+          if (kComp) {
+            a.pcomp[slot] = c;
+            a.pcomp2[slot] = c2;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+struct LlhParams {
+  int32_t type;
+  int32_t use_poisson;
+  double frac_limit;
+  double expected_min;
+};
+
+// PoissonLogLikelihood.h:19-53 and BarlowBeestonBanff2022.h:150-294
+// (BBNoUpdateWeights false; err = sqrt(sum w^2), squared as at :125-126).
+__device__ double llh_bin(const LlhParams& c, double data, double pred,
+                          double err) {
+  const double inf = __builtin_huge_val();
+  if (c.type == GCM_LLH_POISSON) {
+    if (pred <= 0) return inf;
+    if (data <= 0) return 2.0 * pred;
+    return 2.0 * (pred - data + data * log(data / pred));
+  }
+  double mcuncert = err;
+  mcuncert *= mcuncert;
+  if (!isfinite(mcuncert) || mcuncert < 0.0) return inf;
+  if (pred <= c.expected_min) {
+    if (pred <= 0.0 && data > 0.0) return inf;
+    return 0.0;
+  }
+  const double dmin = 2.2250738585072014e-308;  // numeric_limits<double>::min
+  const double deps = 2.220446049250313e-16;    // numeric_limits<double>::epsilon
+  double newmc = pred;
+  double penalty = 0.0;
+  if (mcuncert > 0.0) {
+    const double fractional = fmin(sqrt(mcuncert) / pred, c.frac_limit);
+    const double temp = pred * fractional * fractional - 1;
+    const double temp2 = temp * temp + 4 * data * fractional * fractional;
+    if (temp2 < 0) return inf;
+    const double beta = (-1 * temp + sqrt(temp2)) / 2.;
+    newmc = fmax(pred * beta, dmin);
+    penalty = (beta - 1) * (beta - 1) / (2 * fractional * fractional);
+  }
+  double stat = 0;
+  if (data <= deps) {
+    stat = newmc;
+  } else if (newmc > dmin) {
+    stat = newmc - data + data * (log(data) - log(newmc));
+  } else {
+    stat = dmin - data + data * (log(data) - log(dmin));
+  }
+  double chisq = 2.0 * stat;
+  if (!c.use_poisson) chisq += 2.0 * penalty;
+  return chisq;
+}
+
+template <bool kComp>
+__global__ void __launch_bounds__(kFinalizeBlock)
+    finalize_kernel(const double* __restrict__ psum,
+                    const double* __restrict__ psum2,
+                    const double* __restrict__ pcomp,
+                    const double* __restrict__ pcomp2, int64_t n_partials,
+                    const uint32_t* __restrict__ bin_pbeg, int n_bins,
+                    const double* __restrict__ data, LlhParams cfg,
+                    double* __restrict__ hist, double* __restrict__ hist2,
+                    double* llh_part, unsigned* tickets,
+                    const double* __restrict__ penalty, double* stat_out,
+                    double* pen_out, double* total_out) {
+  __shared__ double s_red[kFinalizeBlock];
+  __shared__ int s_last;
+  const int k = blockIdx.y;
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t kp = (size_t)k * n_partials;
+  double term = 0.0;
+  if (b < n_bins) {
+    double s = 0.0, s2 = 0.0, c = 0.0, c2 = 0.0;
+    const uint32_t p0 = bin_pbeg[b], p1 = bin_pbeg[b + 1];
+    for (uint32_t p = p0; p < p1; ++p) {
+      if (kComp) {
+        double t, e;
+        two_sum(s, psum[kp + p], t, e);
+        s = t;
+        c += e + pcomp[kp + p];
+        two_sum(s2, psum2[kp + p], t, e);
+        s2 = t;
+        c2 += e + pcomp2[kp + p];
+      } else {
+        s += psum[kp + p];
+        s2 += psum2[kp + p];
+      }
+    }
+    if (kComp) {
+      s = s + c;
+      s2 = s2 + c2;
+    }
+    hist[(size_t)k * n_bins + b] = s;
+    hist2[(size_t)k * n_bins + b] = s2;
+    term = data ? llh_bin(cfg, data[b], s, sqrt(s2)) : 0.0;
+  }
+  s_red[threadIdx.x] = term;
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) s_red[threadIdx.x] += s_red[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    llh_part[(size_t)k * gridDim.x + blockIdx.x] = s_red[0];
+    __threadfence();
+    const unsigned ticket = atomicAdd(&tickets[k], 1u);
+    s_last = (ticket == gridDim.x - 1);
+  }
+  __syncthreads();
+  if (s_last && threadIdx.x == 0) {
+    __threadfence();
+    const volatile double* parts = llh_part + (size_t)k * gridDim.x;
+    double stat = 0.0;
+    for (unsigned j = 0; j < gridDim.x; ++j) stat += parts[j];
+    stat_out[k] = stat;
+    pen_out[k] = penalty[k];
+    total_out[k] = stat + penalty[k];
+    tickets[k] = 0;  // ready for the next launch, no memset needed
+  }
+}
+
+// First-match binning: LoaderUtils.cpp:32-40 / Bin.cpp:167-176.
+constexpr int kMaxBinVars = 8;
+__global__ void bin_kernel(int64_t n, int nv, const double* __restrict__ vals,
+                           int nb, const double* __restrict__ edges,
+                           int32_t* __restrict__ out) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += stride) {
+    double v[kMaxBinVars];
+    for (int j = 0; j < nv; ++j) v[j] = vals[e * nv + j];
+    int found = -1;
+    for (int b = 0; b < nb; ++b) {
+      bool in = true;
+      for (int j = 0; j < nv; ++j) {
+        const double lo = edges[((int64_t)b * nv + j) * 2];
+        const double hi = edges[((int64_t)b * nv + j) * 2 + 1];
+        bool ok;
+        if (lo == hi) ok = (lo == v[j]);
+        else ok = !(lo > v[j]) && !(hi <= v[j]);
+        in = in && ok;
+      }
+      if (in) {
+        found = b;
+        break;
+      }
+    }
+    out[e] = found;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+
+// Words of device record per dial (per precision).
+int record_words(int type, int n, bool fp32) {
+  switch (type) {
+    case GCM_DIAL_NORM: return 0;
+    case GCM_DIAL_SHIFT: return 1;
+    case GCM_DIAL_MONOTONIC:
+    case GCM_DIAL_COMPACT: return n + 2;
+    case GCM_DIAL_UNIFORM: return 2 * n + 2;
+    case GCM_DIAL_GENERAL: return 3 * n + 2;
+    case GCM_DIAL_GRAPH: return 2 * n;
+    case GCM_DIAL_TABULATED: return fp32 ? 2 : 1;
+    case GCM_DIAL_KRIGED: return fp32 ? 2 * n : n;
+    default: return -1;
+  }
+}
+
+template <typename T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+};
+
+}  // namespace gcm
+
+using namespace gcm;
+
+struct gcm_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int precision = GCM_PRECISION_FP64;
+  gcm_launch_config launch{kDefaultBlock, kDefaultTileWords, kDefaultBlocksPerCU, 0};
+  int n_cu = 256;
+  bool built = false;
+  bool weights_enabled = false;
+  bool llh_set = false;
+  int k_points = 0;
+  int max_points = 1;
+  int64_t n_events_all = 0, n_events = 0, n_dials = 0, n_tiles = 0;
+  int64_t n_partials = 0;
+  int n_inputs = 0, n_par = 0, n_bins = 0, n_tables = 0;
+  int64_t table_len = 0;
+  int max_dials = 1024;
+  int fin_blocks = 1;
+  double cap = __builtin_huge_val();
+  LlhParams llh{GCM_LLH_POISSON, 0, 1e150, -1.0};
+  std::vector<void*> allocations;
+  int64_t resident = 0;
+  // device
+  TileHdr* d_tiles = nullptr;
+  uint64_t* d_desc = nullptr;
+  void* d_rec = nullptr;
+  double* d_ev_base = nullptr;
+  int32_t* d_ev_bin = nullptr;
+  uint32_t* d_ev_dial = nullptr;
+  int32_t* d_ev_result = nullptr;
+  uint32_t* d_bin_pbeg = nullptr;
+  double *d_psum = nullptr, *d_psum2 = nullptr, *d_pcomp = nullptr,
+         *d_pcomp2 = nullptr;
+  int32_t *d_in_par = nullptr, *d_in_flags = nullptr;
+  double *d_in_mlo = nullptr, *d_in_mrange = nullptr, *d_in_min = nullptr,
+         *d_in_max = nullptr;
+  double *d_theta = nullptr, *d_xin = nullptr, *d_tab = nullptr;
+  double *d_data = nullptr, *d_prior_mean = nullptr, *d_prior_sigma = nullptr;
+  double *d_hist = nullptr, *d_hist2 = nullptr;
+  double *d_llh_part = nullptr, *d_stat = nullptr, *d_pen = nullptr,
+         *d_total = nullptr, *d_penalty = nullptr;
+  unsigned* d_tickets = nullptr;
+  int* d_flags = nullptr;
+  double* d_weights = nullptr;
+  // pinned staging
+  double* h_theta = nullptr;
+  double* h_tab = nullptr;
+  bool tab_dirty = false, theta_dirty = false;
+  hipEvent_t ev_staged = nullptr;  // staging buffers consumed by the device
+  hipStream_t last_stream = nullptr;
+  // kernel timing: a ring of event triples {before fill, after fill, after
+  // finalize}, harvested at every synchronisation
+  static constexpr int kTimingRing = 64;
+  bool timing = false;
+  hipEvent_t tev[kTimingRing][3] = {};
+  int64_t t_issued = 0, t_done = 0;
+  double t_fill_ms = 0, t_fin_ms = 0;
+
+  template <typename T>
+  int alloc(T** p, size_t count) {
+    *p = nullptr;
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e != hipSuccess)
+      return fail(GCM_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    allocations.push_back(*p);
+    resident += (int64_t)(count * sizeof(T));
+    return GCM_OK;
+  }
+  void free_all() {
+    for (void* p : allocations) (void)hipFree(p);
+    allocations.clear();
+    resident = 0;
+  }
+};
+
+#define GCM_TRY(expr)         \
+  do {                        \
+    int rc_ = (expr);         \
+    if (rc_ != GCM_OK) return rc_; \
+  } while (0)
+
+namespace {
+
+template <typename T>
+int upload(gcm_context* c, T** dst, const T* src, size_t count) {
+  GCM_TRY(c->alloc(dst, count));
+  if (count && src) GCM_HIP(hipMemcpy(*dst, src, count * sizeof(T), hipMemcpyHostToDevice));
+  return GCM_OK;
+}
+
+int validate_dial(const gcm_tables* t, int64_t d, int* n_out) {
+  const int type = t->dial_type[d];
+  const int64_t len = t->data_begin[d + 1] - t->data_begin[d];
+  if (len < 0) return fail(GCM_ERR_ARG, "dial data_begin not monotonic");
+  int n = 0;
+  switch (type) {
+    case GCM_DIAL_NORM: if (len != 0) return fail(GCM_ERR_ARG, "norm dial with payload"); break;
+    case GCM_DIAL_SHIFT: if (len != 1) return fail(GCM_ERR_ARG, "shift dial needs 1 value"); n = 1; break;
+    case GCM_DIAL_MONOTONIC:
+    case GCM_DIAL_COMPACT: n = (int)(len - 2); if (n < 2) return fail(GCM_ERR_ARG, "spline needs >= 2 knots"); break;
+    case GCM_DIAL_UNIFORM: n = (int)((len - 2) / 2); if ((len - 2) % 2 || len < 8) return fail(GCM_ERR_ARG, "uniform spline needs 2n+2 >= 8 values"); break;
+    case GCM_DIAL_GENERAL: n = (int)((len - 2) / 3); if ((len - 2) % 3 || n < 2) return fail(GCM_ERR_ARG, "general spline needs 3n+2 values, n >= 2"); break;
+    case GCM_DIAL_GRAPH: n = (int)(len / 2); if (len % 2 || n < 1) return fail(GCM_ERR_ARG, "graph needs 2n values"); break;
+    case GCM_DIAL_TABULATED: n = 1; if (len != 3) return fail(GCM_ERR_ARG, "tabulated dial needs table, index, fraction"); break;
+    case GCM_DIAL_KRIGED: n = (int)((len - 1) / 2); if (len < 1 || (len - 1) % 2) return fail(GCM_ERR_ARG, "kriged dial needs table + pairs"); break;
+    default: return fail(GCM_ERR_UNSUPPORTED, "unsupported dial type " + std::to_string(type));
+  }
+  if (n > 255) return fail(GCM_ERR_UNSUPPORTED, "more than 255 knots/entries in one dial");
+  const int in = t->dial_input[d];
+  if (in < 0 || in >= t->n_inputs) return fail(GCM_ERR_ARG, "dial input out of range");
+  if (type == GCM_DIAL_TABULATED || type == GCM_DIAL_KRIGED) {
+    const double* p = t->dial_data + t->data_begin[d];
+    const int tb = (int)p[0];
+    if (tb < 0 || tb >= t->n_tables) return fail(GCM_ERR_ARG, "dial table out of range");
+    const int64_t tsize = t->table_begin[tb + 1] - t->table_begin[tb];
+    if (type == GCM_DIAL_TABULATED) {
+      const int64_t i = (int64_t)p[1];
+      if (i < 0 || i + 1 >= tsize) return fail(GCM_ERR_ARG, "tabulated index out of table");
+    } else {
+      for (int j = 0; j < n; ++j) {
+        const int64_t i = (int64_t)p[1 + 2 * j];
+        if (i < 0 || i >= tsize) return fail(GCM_ERR_ARG, "kriged index out of table");
+      }
+    }
+  }
+  *n_out = n;
+  return GCM_OK;
+}
+
+// Lay out the tables for the device.  Templated on the record word type.
+template <typename R>
+int build_layout(gcm_context* c, const gcm_tables* t) {
+  const bool fp32 = sizeof(R) == 4;
+  const int64_t N = t->n_events;
+  // validate dials, record knot counts
+  std::vector<uint8_t> knots(t->n_dials);
+  for (int64_t d = 0; d < t->n_dials; ++d) {
+    int n;
+    GCM_TRY(validate_dial(t, d, &n));
+    knots[d] = (uint8_t)n;
+  }
+  // stable counting sort of binned events by bin
+  std::vector<int64_t> count(t->n_bins + 1, 0);
+  for (int64_t e = 0; e < N; ++e) {
+    const int b = t->bin[e];
+    if (b < -1 || b >= t->n_bins) return fail(GCM_ERR_ARG, "event bin out of range");
+    if (t->dial_begin[e + 1] < t->dial_begin[e]) return fail(GCM_ERR_ARG, "dial_begin not monotonic");
+    if (b >= 0) count[b + 1]++;
+  }
+  for (int b = 0; b < t->n_bins; ++b) count[b + 1] += count[b];
+  const int64_t NB = count[t->n_bins];
+  std::vector<int64_t> order(NB);
+  {
+    std::vector<int64_t> pos(count.begin(), count.end() - 1);
+    for (int64_t e = 0; e < N; ++e)
+      if (t->bin[e] >= 0) order[pos[t->bin[e]]++] = e;
+  }
+  // tiles: greedy over sorted events
+  const int words_per16 = 16 / (int)sizeof(R);
+  const int64_t cap_words = (int64_t)c->launch.tile_words * 8 / (int64_t)sizeof(R);
+  std::vector<TileHdr> tiles;
+  {
+    TileHdr cur{};
+    int64_t cur_words = 0, cur_dials = 0, cur_events = 0;
+    int64_t rec_pos = 0, dial_pos = 0;
+    auto close = [&](int64_t ev_end) {
+      (void)ev_end;
+      cur.rec_words = (uint32_t)cur_words;
+      cur.n_dials = (uint16_t)cur_dials;
+      cur.n_events = (uint16_t)cur_events;
+      tiles.push_back(cur);
+      rec_pos += (cur_words + words_per16 - 1) / words_per16 * words_per16;
+      dial_pos += (cur_dials + 1) / 2 * 2;
+    };
+    for (int64_t i = 0; i < NB; ++i) {
+      const int64_t e = order[i];
+      int64_t w = 0;
+      const int64_t nd = t->dial_begin[e + 1] - t->dial_begin[e];
+      for (int64_t d = t->dial_begin[e]; d < t->dial_begin[e + 1]; ++d)
+        w += record_words(t->dial_type[d], knots[d], fp32);
+      if (w > cap_words || nd > c->max_dials)
+        return fail(GCM_ERR_UNSUPPORTED, "one event's dials exceed the tile capacity; raise tile_words");
+      if (cur_events > 0 && (cur_events + 1 > kMaxTileEvents ||
+                             cur_dials + nd > c->max_dials ||
+                             cur_words + w > cap_words)) {
+        close(i);
+        cur_words = cur_dials = cur_events = 0;
+      }
+      if (cur_events == 0) {
+        cur = TileHdr{};
+        cur.first_event = (uint32_t)i;
+        cur.first_dial = (uint32_t)dial_pos;
+        cur.rec_begin = (uint64_t)rec_pos;
+      }
+      cur_words += w;
+      cur_dials += nd;
+      cur_events += 1;
+    }
+    if (cur_events > 0) close(NB);
+    if (dial_pos > UINT32_MAX || NB > UINT32_MAX)
+      return fail(GCM_ERR_UNSUPPORTED, "more than 2^32 dials or events");
+  }
+  const int64_t T = (int64_t)tiles.size();
+  const int64_t total_words = T ? (int64_t)tiles.back().rec_begin +
+                                      (tiles.back().rec_words + words_per16 - 1) / words_per16 * words_per16
+                                : 0;
+  const int64_t total_desc = T ? (int64_t)tiles.back().first_dial + (tiles.back().n_dials + 1) / 2 * 2 : 0;
+  // partial slots: one per distinct bin per tile
+  std::vector<uint32_t> bin_np(t->n_bins + 1, 0);
+  int64_t np = 0;
+  for (int64_t ti = 0; ti < T; ++ti) {
+    tiles[ti].first_partial = (uint32_t)np;
+    int prev = -1;
+    for (int j = 0; j < tiles[ti].n_events; ++j) {
+      const int b = t->bin[order[tiles[ti].first_event + j]];
+      if (j == 0 || b != prev) {
+        np++;
+        bin_np[b + 1]++;
+      }
+      prev = b;
+    }
+  }
+  if (np > UINT32_MAX) return fail(GCM_ERR_UNSUPPORTED, "too many partial slots");
+  for (int b = 0; b < t->n_bins; ++b) bin_np[b + 1] += bin_np[b];
+
+  // fill records, descriptors and event arrays, tiles in parallel
+  std::vector<R> rec((size_t)std::max<int64_t>(total_words, 1), R(0));
+  std::vector<uint64_t> desc((size_t)std::max<int64_t>(total_desc, 1), 0);
+  std::vector<double> ev_base(NB);
+  std::vector<int32_t> ev_bin(NB), ev_result(NB);
+  std::vector<uint32_t> ev_dial(NB);
+  int nthreads = (int)std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
+  std::vector<std::thread> pool;
+  std::vector<std::string> errs(nthreads);
+  for (int th = 0; th < nthreads; ++th) {
+    pool.emplace_back([&, th] {
+      std::vector<int64_t> dial_ids;
+      std::vector<int> slot;
+      for (int64_t ti = th; ti < T; ti += nthreads) {
+        const TileHdr& h = tiles[ti];
+        dial_ids.clear();
+        slot.clear();
+        int local = 0;
+        for (int j = 0; j < h.n_events; ++j) {
+          const int64_t se = h.first_event + j;
+          const int64_t e = order[se];
+          const int64_t d0 = t->dial_begin[e], d1 = t->dial_begin[e + 1];
+          ev_base[se] = t->base_weight[e];
+          ev_bin[se] = t->bin[e];
+          ev_result[se] = (int32_t)e;
+          ev_dial[se] = (uint32_t)local | ((uint32_t)(d1 - d0) << 16);
+          for (int64_t d = d0; d < d1; ++d) {
+            dial_ids.push_back(d);
+            slot.push_back(local++);
+          }
+        }
+        // type-major order, stable inside a type
+        std::vector<int> idx(dial_ids.size());
+        for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
+        std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) {
+          return t->dial_type[dial_ids[x]] < t->dial_type[dial_ids[y]];
+        });
+        int64_t off = 0;
+        for (size_t q = 0; q < idx.size(); ++q) {
+          const int64_t d = dial_ids[idx[q]];
+          const int type = t->dial_type[d];
+          const int n = knots[d];
+          const double* p = t->dial_data + t->data_begin[d];
+          R* out = rec.data() + h.rec_begin + off;
+          switch (type) {
+            case GCM_DIAL_NORM: break;
+            case GCM_DIAL_TABULATED:
+            case GCM_DIAL_KRIGED: {
+              const int tb = (int)p[0];
+              const int64_t base = t->table_begin[tb];
+              for (int j = 0; j < n; ++j) {
+                const uint32_t sl = (uint32_t)(base + (int64_t)p[1 + 2 * j]);
+                const float w = (float)p[2 + 2 * j];
+                if (fp32) {
+                  uint32_t wb;
+                  std::memcpy(&wb, &w, 4);
+                  std::memcpy(out + 2 * j, &sl, 4);
+                  std::memcpy(out + 2 * j + 1, &wb, 4);
+                } else {
+                  uint32_t wb;
+                  std::memcpy(&wb, &w, 4);
+                  const uint64_t bits = (uint64_t)sl | ((uint64_t)wb << 32);
+                  std::memcpy(out + j, &bits, 8);
+                }
+              }
+              break;
+            }
+            default: {
+              const int64_t len = t->data_begin[d + 1] - t->data_begin[d];
+              for (int64_t j = 0; j < len; ++j) out[j] = (R)p[j];
+            }
+          }
+          const uint64_t in = (uint64_t)t->dial_input[d];
+          desc[h.first_dial + q] = in | ((uint64_t)type << 16) | ((uint64_t)n << 20) |
+                                   ((uint64_t)slot[idx[q]] << 32) | ((uint64_t)off << 48);
+          off += record_words(type, n, fp32);
+        }
+      }
+    });
+  }
+  for (auto& th : pool) th.join();
+
+  c->n_events = NB;
+  c->n_dials = t->n_dials;
+  c->n_tiles = T;
+  c->n_partials = np;
+  GCM_TRY(upload(c, &c->d_tiles, tiles.data(), tiles.size()));
+  GCM_TRY(upload(c, &c->d_desc, desc.data(), desc.size()));
+  {
+    R* p = nullptr;
+    GCM_TRY(upload(c, &p, rec.data(), rec.size()));
+    c->d_rec = p;
+  }
+  GCM_TRY(upload(c, &c->d_ev_base, ev_base.data(), ev_base.size()));
+  GCM_TRY(upload(c, &c->d_ev_bin, ev_bin.data(), ev_bin.size()));
+  GCM_TRY(upload(c, &c->d_ev_dial, ev_dial.data(), ev_dial.size()));
+  GCM_TRY(upload(c, &c->d_ev_result, ev_result.data(), ev_result.size()));
+  GCM_TRY(upload(c, &c->d_bin_pbeg, bin_np.data(), bin_np.size()));
+  return GCM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* gcm_last_error(void) { return g_error.c_str(); }
+
+int gcm_version(void) { return 10000; }
+
+int gcm_device_count(int* count) {
+  if (!count) return fail(GCM_ERR_ARG, "count is NULL");
+  int n = 0;
+  hipError_t e = hipGetDeviceCount(&n);
+  if (e != hipSuccess) n = 0;
+  *count = n;
+  return GCM_OK;
+}
+
+int gcm_create(int device, gcm_context** out) {
+  if (!out) return fail(GCM_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+    return fail(GCM_ERR_HIP, "no HIP device visible");
+  if (device < 0 || device >= n) return fail(GCM_ERR_ARG, "device out of range");
+  GCM_HIP(hipSetDevice(device));
+  auto* c = new gcm_context();
+  c->device = device;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->n_cu = prop.multiProcessorCount;
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->ev_staged, hipEventDisableTiming) != hipSuccess) {
+    delete c;
+    return fail(GCM_ERR_HIP, "stream/event creation failed");
+  }
+  *out = c;
+  return GCM_OK;
+}
+
+int gcm_destroy(gcm_context* c) {
+  if (!c) return GCM_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  c->free_all();
+  for (auto& tr : c->tev)
+    for (auto& e : tr)
+      if (e) (void)hipEventDestroy(e);
+  if (c->h_theta) (void)hipHostFree(c->h_theta);
+  if (c->h_tab) (void)hipHostFree(c->h_tab);
+  if (c->ev_staged) (void)hipEventDestroy(c->ev_staged);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return GCM_OK;
+}
+
+int gcm_set_precision(gcm_context* c, int precision) {
+  if (!c) return fail(GCM_ERR_ARG, "ctx is NULL");
+  if (c->built) return fail(GCM_ERR_STATE, "precision must be set before gcm_build");
+  if (precision != GCM_PRECISION_FP64 && precision != GCM_PRECISION_FP32)
+    return fail(GCM_ERR_ARG, "unknown precision");
+  c->precision = precision;
+  return GCM_OK;
+}
+
+int gcm_set_launch_config(gcm_context* c, const gcm_launch_config* cfg) {
+  if (!c || !cfg) return fail(GCM_ERR_ARG, "NULL argument");
+  gcm_launch_config l = c->launch;
+  if (cfg->block_threads) l.block_threads = cfg->block_threads;
+  if (cfg->tile_words) {
+    if (c->built && cfg->tile_words != c->launch.tile_words)
+      return fail(GCM_ERR_STATE, "tile_words is fixed at gcm_build");
+    l.tile_words = cfg->tile_words;
+  }
+  if (cfg->blocks_per_cu) l.blocks_per_cu = cfg->blocks_per_cu;
+  if (l.block_threads < 64 || l.block_threads > 512 || l.block_threads % 64)
+    return fail(GCM_ERR_ARG, "block_threads must be a multiple of 64 in [64, 512]");
+  if (l.tile_words < 256 || l.tile_words > 16384 || l.tile_words % 64)
+    return fail(GCM_ERR_ARG, "tile_words must be a multiple of 64 in [256, 16384]");
+  if (l.blocks_per_cu < 1 || l.blocks_per_cu > 32)
+    return fail(GCM_ERR_ARG, "blocks_per_cu must be in [1, 32]");
+  c->launch = l;
+  return GCM_OK;
+}
+
+int gcm_get_launch_config(gcm_context* c, gcm_launch_config* cfg) {
+  if (!c || !cfg) return fail(GCM_ERR_ARG, "NULL argument");
+  *cfg = c->launch;
+  return GCM_OK;
+}
+
+int gcm_build(gcm_context* c, const gcm_tables* t, int32_t max_points) {
+  if (!c || !t) return fail(GCM_ERR_ARG, "NULL argument");
+  if (c->built) return fail(GCM_ERR_STATE, "context already built");
+  if (max_points < 1) return fail(GCM_ERR_ARG, "max_points must be >= 1");
+  if (t->n_events < 0 || t->n_dials < 0 || t->n_inputs < 0 || t->n_bins < 1 ||
+      t->n_parameters < 1 || t->n_tables < 0)
+    return fail(GCM_ERR_ARG, "invalid table sizes");
+  if (t->n_inputs > 65536) return fail(GCM_ERR_UNSUPPORTED, "more than 65536 dial inputs");
+  if (t->n_events && (!t->base_weight || !t->bin || !t->dial_begin))
+    return fail(GCM_ERR_ARG, "event arrays missing");
+  if (t->dial_begin && t->n_events && (t->dial_begin[0] != 0 || t->dial_begin[t->n_events] != t->n_dials))
+    return fail(GCM_ERR_ARG, "dial_begin must span [0, n_dials]");
+  if (t->n_dials && (!t->dial_type || !t->dial_input || !t->data_begin || !t->dial_data))
+    return fail(GCM_ERR_ARG, "dial arrays missing");
+  for (int i = 0; i < t->n_inputs; ++i)
+    if (t->inputs[i].parameter < 0 || t->inputs[i].parameter >= t->n_parameters)
+      return fail(GCM_ERR_ARG, "input parameter out of range");
+  GCM_HIP(hipSetDevice(c->device));
+  c->n_events_all = t->n_events;
+  c->n_inputs = t->n_inputs;
+  c->n_par = t->n_parameters;
+  c->n_bins = t->n_bins;
+  c->n_tables = t->n_tables;
+  c->table_len = t->n_tables ? t->table_begin[t->n_tables] : 0;
+  c->max_points = max_points;
+  c->cap = std::isfinite(t->reweight_cap) ? t->reweight_cap : __builtin_huge_val();
+  int rc = (c->precision == GCM_PRECISION_FP32) ? build_layout<float>(c, t)
+                                                 : build_layout<double>(c, t);
+  if (rc != GCM_OK) {
+    c->free_all();
+    return rc;
+  }
+  // inputs
+  std::vector<int32_t> par(t->n_inputs), flg(t->n_inputs);
+  std::vector<double> mlo(t->n_inputs), mr(t->n_inputs), mn(t->n_inputs), mx(t->n_inputs);
+  const double inf = __builtin_huge_val();
+  for (int i = 0; i < t->n_inputs; ++i) {
+    const gcm_input& in = t->inputs[i];
+    par[i] = in.parameter;
+    flg[i] = in.flags;
+    mlo[i] = in.mirror_lower;
+    mr[i] = in.mirror_range;
+    mn[i] = std::isnan(in.min_response) ? -inf : in.min_response;
+    mx[i] = std::isnan(in.max_response) ? inf : in.max_response;
+  }
+  const size_t K = (size_t)max_points;
+  c->fin_blocks = (c->n_bins + kFinalizeBlock - 1) / kFinalizeBlock;
+  rc = GCM_OK;
+  auto step = [&](int r) { if (rc == GCM_OK) rc = r; };
+  step(upload(c, &c->d_in_par, par.data(), par.size()));
+  step(upload(c, &c->d_in_flags, flg.data(), flg.size()));
+  step(upload(c, &c->d_in_mlo, mlo.data(), mlo.size()));
+  step(upload(c, &c->d_in_mrange, mr.data(), mr.size()));
+  step(upload(c, &c->d_in_min, mn.data(), mn.size()));
+  step(upload(c, &c->d_in_max, mx.data(), mx.size()));
+  step(c->alloc(&c->d_theta, K * c->n_par));
+  step(c->alloc(&c->d_xin, K * std::max(1, c->n_inputs)));
+  step(c->alloc(&c->d_tab, K * std::max<int64_t>(1, c->table_len)));
+  step(c->alloc(&c->d_psum, K * c->n_partials));
+  step(c->alloc(&c->d_psum2, K * c->n_partials));
+  if (c->precision == GCM_PRECISION_FP32) {
+    step(c->alloc(&c->d_pcomp, K * c->n_partials));
+    step(c->alloc(&c->d_pcomp2, K * c->n_partials));
+  }
+  step(c->alloc(&c->d_hist, K * c->n_bins));
+  step(c->alloc(&c->d_hist2, K * c->n_bins));
+  step(c->alloc(&c->d_llh_part, K * c->fin_blocks));
+  step(c->alloc(&c->d_stat, K));
+  step(c->alloc(&c->d_pen, K));
+  step(c->alloc(&c->d_total, K));
+  step(c->alloc(&c->d_penalty, K));
+  step(c->alloc(&c->d_tickets, K));
+  step(c->alloc(&c->d_flags, 1));
+  step(c->alloc(&c->d_data, c->n_bins));
+  step(c->alloc(&c->d_prior_mean, c->n_par));
+  step(c->alloc(&c->d_prior_sigma, c->n_par));
+  if (rc != GCM_OK) {
+    c->free_all();
+    return rc;
+  }
+  GCM_HIP(hipMemset(c->d_tickets, 0, K * sizeof(unsigned)));
+  GCM_HIP(hipMemset(c->d_flags, 0, sizeof(int)));
+  GCM_HIP(hipMemset(c->d_tab, 0, K * std::max<int64_t>(1, c->table_len) * sizeof(double)));
+  GCM_HIP(hipMemset(c->d_theta, 0, K * c->n_par * sizeof(double)));
+  if (hipHostMalloc((void**)&c->h_theta, K * c->n_par * sizeof(double)) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_tab, K * std::max<int64_t>(1, c->table_len) * sizeof(double)) != hipSuccess)
+    return fail(GCM_ERR_NOMEM, "pinned staging allocation failed");
+  // default likelihood: no data histogram -> stat term 0 until set
+  c->built = true;
+  return GCM_OK;
+}
+
+int gcm_resident_bytes(gcm_context* c, int64_t* bytes) {
+  if (!c || !bytes) return fail(GCM_ERR_ARG, "NULL argument");
+  *bytes = c->resident;
+  return GCM_OK;
+}
+
+int gcm_counts(gcm_context* c, int64_t* events, int64_t* dials, int64_t* tiles) {
+  if (!c) return fail(GCM_ERR_ARG, "ctx is NULL");
+  if (events) *events = c->n_events;
+  if (dials) *dials = c->n_dials;
+  if (tiles) *tiles = c->n_tiles;
+  return GCM_OK;
+}
+
+// Accumulate the elapsed times of every timed fill that has been issued.
+static int harvest_timing(gcm_context* c) {
+  while (c->t_done < c->t_issued) {
+    hipEvent_t* e = c->tev[c->t_done % gcm_context::kTimingRing];
+    GCM_HIP(hipEventSynchronize(e[2]));
+    float a = 0, b = 0;
+    GCM_HIP(hipEventElapsedTime(&a, e[0], e[1]));
+    GCM_HIP(hipEventElapsedTime(&b, e[1], e[2]));
+    c->t_fill_ms += a;
+    c->t_fin_ms += b;
+    c->t_done++;
+  }
+  return GCM_OK;
+}
+
+static int wait_staging(gcm_context* c) {
+  GCM_HIP(hipEventSynchronize(c->ev_staged));
+  return GCM_OK;
+}
+
+int gcm_set_parameters(gcm_context* c, const double* theta, int32_t n_parameters,
+                       int32_t k_points) {
+  if (!c || !theta) return fail(GCM_ERR_ARG, "NULL argument");
+  if (!c->built) return fail(GCM_ERR_STATE, "gcm_build first");
+  if (n_parameters != c->n_par) return fail(GCM_ERR_ARG, "parameter count mismatch");
+  if (k_points < 1 || k_points > c->max_points) return fail(GCM_ERR_ARG, "k_points out of [1, max_points]");
+  GCM_TRY(wait_staging(c));
+  std::memcpy(c->h_theta, theta, (size_t)k_points * n_parameters * sizeof(double));
+  c->k_points = k_points;
+  c->theta_dirty = true;
+  return GCM_OK;
+}
+
+int gcm_set_table_values(gcm_context* c, const double* values, int64_t table_len,
+                         int32_t k_points) {
+  if (!c || (!values && table_len)) return fail(GCM_ERR_ARG, "NULL argument");
+  if (!c->built) return fail(GCM_ERR_STATE, "gcm_build first");
+  if (table_len != c->table_len) return fail(GCM_ERR_ARG, "table length mismatch");
+  if (k_points < 1 || k_points > c->max_points) return fail(GCM_ERR_ARG, "k_points out of [1, max_points]");
+  if (table_len == 0) return GCM_OK;
+  GCM_TRY(wait_staging(c));
+  std::memcpy(c->h_tab, values, (size_t)k_points * table_len * sizeof(double));
+  c->tab_dirty = true;
+  return GCM_OK;
+}
+
+int gcm_set_likelihood(gcm_context* c, const gcm_llh_config* cfg,
+                       const double* data_hist, const double* prior_mean,
+                       const double* prior_sigma) {
+  if (!c || !cfg || !data_hist) return fail(GCM_ERR_ARG, "NULL argument");
+  if (!c->built) return fail(GCM_ERR_STATE, "gcm_build first");
+  if (cfg->type != GCM_LLH_POISSON && cfg->type != GCM_LLH_BARLOW_OA2021)
+    return fail(GCM_ERR_ARG, "unknown likelihood type");
+  GCM_HIP(hipSetDevice(c->device));
+  GCM_HIP(hipStreamSynchronize(c->stream));
+  c->llh.type = cfg->type;
+  c->llh.use_poisson = cfg->use_poisson_likelihood;
+  // BarlowBeestonBanff2022.h:85-88: hard limit on the fractional error
+  c->llh.frac_limit = std::min(cfg->fractional_error_limit, 1.0E+152);
+  c->llh.expected_min = cfg->expected_value_minimum;
+  GCM_HIP(hipMemcpy(c->d_data, data_hist, c->n_bins * sizeof(double), hipMemcpyHostToDevice));
+  std::vector<double> mean(c->n_par, 0.0), sig(c->n_par, 0.0);
+  if (prior_mean && prior_sigma) {
+    std::copy(prior_mean, prior_mean + c->n_par, mean.begin());
+    std::copy(prior_sigma, prior_sigma + c->n_par, sig.begin());
+  }
+  GCM_HIP(hipMemcpy(c->d_prior_mean, mean.data(), c->n_par * sizeof(double), hipMemcpyHostToDevice));
+  GCM_HIP(hipMemcpy(c->d_prior_sigma, sig.data(), c->n_par * sizeof(double), hipMemcpyHostToDevice));
+  c->llh_set = true;
+  return GCM_OK;
+}
+
+int gcm_enable_weights(gcm_context* c, int enable) {
+  if (!c) return fail(GCM_ERR_ARG, "ctx is NULL");
+  if (!c->built) return fail(GCM_ERR_STATE, "gcm_build first");
+  if (enable && !c->d_weights) {
+    GCM_TRY(c->alloc(&c->d_weights, (size_t)std::max<int64_t>(1, c->n_events_all)));
+    GCM_HIP(hipMemset(c->d_weights, 0, (size_t)std::max<int64_t>(1, c->n_events_all) * sizeof(double)));
+  }
+  c->weights_enabled = enable != 0;
+  return GCM_OK;
+}
+
+int gcm_fill(gcm_context* c, void* stream_ptr) {
+  if (!c) return fail(GCM_ERR_ARG, "ctx is NULL");
+  if (!c->built) return fail(GCM_ERR_STATE, "gcm_build first");
+  if (c->k_points < 1) return fail(GCM_ERR_STATE, "gcm_set_parameters first");
+  hipStream_t s = stream_ptr ? (hipStream_t)stream_ptr : c->stream;
+  GCM_HIP(hipSetDevice(c->device));
+  const int K = c->k_points;
+  if (c->theta_dirty) {
+    GCM_HIP(hipMemcpyAsync(c->d_theta, c->h_theta, (size_t)K * c->n_par * sizeof(double),
+                           hipMemcpyHostToDevice, s));
+    c->theta_dirty = false;
+  }
+  if (c->tab_dirty) {
+    GCM_HIP(hipMemcpyAsync(c->d_tab, c->h_tab, (size_t)K * c->table_len * sizeof(double),
+                           hipMemcpyHostToDevice, s));
+    c->tab_dirty = false;
+  }
+  GCM_HIP(hipEventRecord(c->ev_staged, s));
+  prologue_kernel<<<K, 256, 0, s>>>(c->d_theta, c->n_par, c->d_in_par, c->d_in_mlo,
+                                    c->d_in_mrange, c->n_inputs, c->d_xin,
+                                    c->llh_set ? c->d_prior_mean : nullptr,
+                                    c->llh_set ? c->d_prior_sigma : nullptr, c->d_penalty);
+  GCM_HIP(hipGetLastError());
+  const bool fp32 = c->precision == GCM_PRECISION_FP32;
+  FillArgs a;
+  a.tiles = c->d_tiles;
+  a.n_tiles = c->n_tiles;
+  a.desc = c->d_desc;
+  a.rec = c->d_rec;
+  a.ev_base = c->d_ev_base;
+  a.ev_bin = c->d_ev_bin;
+  a.ev_dial = c->d_ev_dial;
+  a.ev_result = c->d_ev_result;
+  a.xin = c->d_xin;
+  a.n_inputs = c->n_inputs;
+  a.in_flags = c->d_in_flags;
+  a.in_min = c->d_in_min;
+  a.in_max = c->d_in_max;
+  a.tab = c->d_tab;
+  a.table_len = c->table_len;
+  a.cap = c->cap;
+  a.k_points = K;
+  a.psum = c->d_psum;
+  a.psum2 = c->d_psum2;
+  a.pcomp = c->d_pcomp;
+  a.pcomp2 = c->d_pcomp2;
+  a.n_partials = c->n_partials;
+  a.weights_out = c->weights_enabled ? c->d_weights : nullptr;
+  a.flags = c->d_flags;
+  const int rec_size = fp32 ? 4 : 8;
+  a.tile_words = c->launch.tile_words * 8 / rec_size;
+  a.max_dials = c->max_dials;
+  const size_t lds = (size_t)c->launch.tile_words * 8 + (size_t)c->max_dials * 16;
+  hipEvent_t* tev = nullptr;
+  if (c->timing) {
+    if (c->t_issued - c->t_done >= gcm_context::kTimingRing) GCM_TRY(harvest_timing(c));
+    tev = c->tev[c->t_issued % gcm_context::kTimingRing];
+    GCM_HIP(hipEventRecord(tev[0], s));
+  }
+  if (c->n_tiles > 0) {
+    const int64_t want = (int64_t)c->launch.blocks_per_cu * c->n_cu;
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(c->n_tiles, want));
+    if (lds > 65536) {
+      if (fp32)
+        GCM_HIP(hipFuncSetAttribute((const void*)fill_kernel<float, true>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+      else
+        GCM_HIP(hipFuncSetAttribute((const void*)fill_kernel<double, false>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    }
+    if (fp32)
+      fill_kernel<float, true><<<grid, c->launch.block_threads, lds, s>>>(a);
+    else
+      fill_kernel<double, false><<<grid, c->launch.block_threads, lds, s>>>(a);
+    GCM_HIP(hipGetLastError());
+  }
+  if (tev) GCM_HIP(hipEventRecord(tev[1], s));
+  dim3 fg(c->fin_blocks, K);
+  if (fp32)
+    finalize_kernel<true><<<fg, kFinalizeBlock, 0, s>>>(
+        c->d_psum, c->d_psum2, c->d_pcomp, c->d_pcomp2, c->n_partials, c->d_bin_pbeg,
+        c->n_bins, c->llh_set ? c->d_data : nullptr, c->llh, c->d_hist, c->d_hist2,
+        c->d_llh_part, c->d_tickets, c->d_penalty, c->d_stat, c->d_pen, c->d_total);
+  else
+    finalize_kernel<false><<<fg, kFinalizeBlock, 0, s>>>(
+        c->d_psum, c->d_psum2, c->d_pcomp, c->d_pcomp2, c->n_partials, c->d_bin_pbeg,
+        c->n_bins, c->llh_set ? c->d_data : nullptr, c->llh, c->d_hist, c->d_hist2,
+        c->d_llh_part, c->d_tickets, c->d_penalty, c->d_stat, c->d_pen, c->d_total);
+  GCM_HIP(hipGetLastError());
+  if (tev) {
+    GCM_HIP(hipEventRecord(tev[2], s));
+    c->t_issued++;
+  }
+  c->last_stream = s;
+  return GCM_OK;
+}
+
+int gcm_sync(gcm_context* c) {
+  if (!c) return fail(GCM_ERR_ARG, "ctx is NULL");
+  GCM_HIP(hipSetDevice(c->device));
+  GCM_HIP(hipStreamSynchronize(c->last_stream ? c->last_stream : c->stream));
+  if (c->timing) GCM_TRY(harvest_timing(c));
+  if (c->built) {
+    int flag = 0;
+    GCM_HIP(hipMemcpy(&flag, c->d_flags, sizeof(int), hipMemcpyDeviceToHost));
+    if (flag) {
+      GCM_HIP(hipMemset(c->d_flags, 0, sizeof(int)));
+      return fail(GCM_ERR_NUMERIC, "NaN event weight");
+    }
+  }
+  return GCM_OK;
+}
+
+int gcm_get_hist(gcm_context* c, int32_t k, double* sums, double* sums2) {
+  if (!c) return fail(GCM_ERR_ARG, "ctx is NULL");
+  if (!c->built) return fail(GCM_ERR_STATE, "gcm_build first");
+  if (k < 0 || k >= std::max(1, c->k_points)) return fail(GCM_ERR_ARG, "point out of range");
+  GCM_HIP(hipSetDevice(c->device));
+  hipStream_t s = c->last_stream ? c->last_stream : c->stream;
+  GCM_HIP(hipStreamSynchronize(s));
+  if (sums) GCM_HIP(hipMemcpy(sums, c->d_hist + (size_t)k * c->n_bins, c->n_bins * sizeof(double), hipMemcpyDeviceToHost));
+  if (sums2) GCM_HIP(hipMemcpy(sums2, c->d_hist2 + (size_t)k * c->n_bins, c->n_bins * sizeof(double), hipMemcpyDeviceToHost));
+  return GCM_OK;
+}
+
+int gcm_get_weights(gcm_context* c, double* weights) {
+  if (!c || !weights) return fail(GCM_ERR_ARG, "NULL argument");
+  if (!c->weights_enabled || !c->d_weights) return fail(GCM_ERR_STATE, "gcm_enable_weights first");
+  GCM_HIP(hipSetDevice(c->device));
+  GCM_HIP(hipStreamSynchronize(c->last_stream ? c->last_stream : c->stream));
+  // only binned events are written; keep the caller's values elsewhere
+  std::vector<double> tmp((size_t)c->n_events_all);
+  GCM_HIP(hipMemcpy(tmp.data(), c->d_weights, tmp.size() * sizeof(double), hipMemcpyDeviceToHost));
+  std::vector<int32_t> res((size_t)c->n_events);
+  GCM_HIP(hipMemcpy(res.data(), c->d_ev_result, res.size() * sizeof(int32_t), hipMemcpyDeviceToHost));
+  for (int32_t r : res) weights[r] = tmp[r];
+  return GCM_OK;
+}
+
+int gcm_get_llh(gcm_context* c, double* total, double* stat, double* penalty) {
+  if (!c) return fail(GCM_ERR_ARG, "ctx is NULL");
+  if (!c->built || c->k_points < 1) return fail(GCM_ERR_STATE, "nothing evaluated yet");
+  GCM_HIP(hipSetDevice(c->device));
+  GCM_HIP(hipStreamSynchronize(c->last_stream ? c->last_stream : c->stream));
+  const size_t K = (size_t)c->k_points;
+  if (total) GCM_HIP(hipMemcpy(total, c->d_total, K * sizeof(double), hipMemcpyDeviceToHost));
+  if (stat) GCM_HIP(hipMemcpy(stat, c->d_stat, K * sizeof(double), hipMemcpyDeviceToHost));
+  if (penalty) GCM_HIP(hipMemcpy(penalty, c->d_pen, K * sizeof(double), hipMemcpyDeviceToHost));
+  return GCM_OK;
+}
+
+int gcm_enable_timing(gcm_context* c, int enable) {
+  if (!c) return fail(GCM_ERR_ARG, "ctx is NULL");
+  GCM_HIP(hipSetDevice(c->device));
+  if (enable && !c->tev[0][0]) {
+    for (auto& tr : c->tev)
+      for (auto& e : tr) GCM_HIP(hipEventCreate(&e));
+  }
+  if (c->timing) GCM_TRY(harvest_timing(c));
+  c->timing = enable != 0;
+  c->t_issued = c->t_done = 0;
+  c->t_fill_ms = c->t_fin_ms = 0;
+  return GCM_OK;
+}
+
+int gcm_get_timing(gcm_context* c, double* fill_ms, double* finalize_ms, int64_t* count) {
+  if (!c) return fail(GCM_ERR_ARG, "ctx is NULL");
+  GCM_HIP(hipSetDevice(c->device));
+  GCM_TRY(harvest_timing(c));
+  if (fill_ms) *fill_ms = c->t_fill_ms;
+  if (finalize_ms) *finalize_ms = c->t_fin_ms;
+  if (count) *count = c->t_done;
+  return GCM_OK;
+}
+
+int gcm_bin_events(int device, int64_t n_events, int32_t n_vars, const double* values,
+                   int32_t n_bins, const double* edges, int32_t* out_bin) {
+  if (n_events < 0 || n_vars < 1 || n_vars > kMaxBinVars || n_bins < 0 ||
+      (n_events && (!values || !out_bin)) || (n_bins && !edges))
+    return fail(GCM_ERR_ARG, "invalid binning arguments");
+  if (n_events == 0) return GCM_OK;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n)
+    return fail(GCM_ERR_HIP, "no such HIP device");
+  GCM_HIP(hipSetDevice(device));
+  double *dv = nullptr, *de = nullptr;
+  int32_t* db = nullptr;
+  const size_t nv = (size_t)n_events * n_vars, ne = (size_t)std::max(1, n_bins) * n_vars * 2;
+  GCM_HIP(hipMalloc((void**)&dv, nv * sizeof(double)));
+  hipError_t e1 = hipMalloc((void**)&de, ne * sizeof(double));
+  hipError_t e2 = hipMalloc((void**)&db, (size_t)n_events * sizeof(int32_t));
+  int rc = GCM_OK;
+  if (e1 != hipSuccess || e2 != hipSuccess) rc = fail(GCM_ERR_NOMEM, "hipMalloc failed");
+  if (rc == GCM_OK && hipMemcpy(dv, values, nv * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) rc = fail(GCM_ERR_HIP, "copy failed");
+  if (rc == GCM_OK && n_bins && hipMemcpy(de, edges, (size_t)n_bins * n_vars * 2 * sizeof(double), hipMemcpyHostToDevice) != hipSuccess) rc = fail(GCM_ERR_HIP, "copy failed");
+  if (rc == GCM_OK) {
+    const int grid = (int)std::min<int64_t>((n_events + 255) / 256, 65536);
+    bin_kernel<<<grid, 256>>>(n_events, n_vars, dv, n_bins, de, db);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess) rc = fail(GCM_ERR_HIP, "binning kernel failed");
+  }
+  if (rc == GCM_OK && hipMemcpy(out_bin, db, (size_t)n_events * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess) rc = fail(GCM_ERR_HIP, "copy failed");
+  (void)hipFree(dv);
+  (void)hipFree(de);
+  (void)hipFree(db);
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,227 @@
+/*
+ * gcm.h -- C ABI of the MI355X event-reweighting cache ("gcm" = GUNDAM cache
+ * manager).  This is the drop-in boundary that replaces the reference's
+ * Cache::Manager (src/CacheManager/include/CacheManager.h:56-410): parameter
+ * upload, dial-table build, weight and histogram fetch, plus the binned
+ * likelihood that the reference evaluates on the host after the copy-back
+ * (src/StatisticalInference/JointProbability/include/JointProbabilityBase.h:46-54).
+ *
+ * Conventions
+ *   - Every function returns an int status (GCM_OK == 0); on failure the
+ *     message is available from gcm_last_error() (thread-local).  Nothing is
+ *     thrown across the ABI and nothing calls exit().
+ *   - Plain pointers and sizes only.  Host pointers unless a name ends in
+ *     _device.  All floating point is IEEE binary64 at the boundary.
+ *   - One context owns one set of device-resident tables on one GPU.
+ *
+ * Dial payload layouts (one record per dial, in gcm_tables.dial_data).  They
+ * are the reference's own record layouts so the facade can pass
+ * DialBase::getDialData() through unchanged:
+ *   NORM       : (empty)                      Norm.h:21 -- response = input value
+ *   MONOTONIC  : x0, step, y[0..n-1]          MonotonicSpline.cpp:12-69
+ *   UNIFORM    : x0, step, (y,slope)[0..n-1]  UniformSpline.cpp:14-45
+ *   GENERAL    : x0, step, (y,slope,x)[0..n-1] GeneralSpline.cpp:12-31
+ *   COMPACT    : x0, step, y[0..n-1]          CompactSpline.cpp:11-69
+ *   GRAPH      : (y,x)[0..n-1]                Graph.cpp:32-39
+ *   TABULATED  : table, index, fraction       Tabulated.h:16-28 (fraction is
+ *                rounded to float as the reference does)
+ *   KRIGED     : table, (index, weight)[0..k-1]  Kriged.cpp:3-21 (weights are
+ *                rounded to float as the reference does)
+ *   SHIFT      : value                        Shift.h:21
+ * "table" is an index into gcm_tables.table_begin; "index" is relative to the
+ * start of that table.
+ */
+#ifndef GCM_H
+#define GCM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define GCM_OK 0
+#define GCM_ERR_ARG 1         /* invalid argument or size                   */
+#define GCM_ERR_STATE 2       /* call order violated (e.g. fill before build) */
+#define GCM_ERR_HIP 3         /* HIP runtime error                          */
+#define GCM_ERR_UNSUPPORTED 4 /* dial type or feature not supported          */
+#define GCM_ERR_NUMERIC 5     /* NaN weight or non-finite likelihood         */
+#define GCM_ERR_NOMEM 6       /* device or host allocation failed            */
+
+/* ---- dial types (DialBase subclasses with device support,
+ *      CacheManager.cpp:570-667) ------------------------------------------ */
+#define GCM_DIAL_NORM 0
+#define GCM_DIAL_MONOTONIC 1
+#define GCM_DIAL_UNIFORM 2
+#define GCM_DIAL_GENERAL 3
+#define GCM_DIAL_COMPACT 4
+#define GCM_DIAL_GRAPH 5
+#define GCM_DIAL_TABULATED 6
+#define GCM_DIAL_KRIGED 7
+#define GCM_DIAL_SHIFT 8
+#define GCM_DIAL_NTYPES 9
+
+/* ---- input flags --------------------------------------------------------- */
+#define GCM_INPUT_NO_EXTRAPOLATION 1 /* DialBase::setAllowExtrapolation(false) */
+
+/* ---- precision modes ----------------------------------------------------- */
+#define GCM_PRECISION_FP64 0 /* FP64 records and dial evaluation (reference)  */
+#define GCM_PRECISION_FP32 1 /* FP32 records and dial evaluation; FP64 clamp,
+                                compensated FP64 bin sums and FP64 LLH       */
+
+/* ---- likelihood types (JointProbability.cpp:61-94) ---------------------- */
+#define GCM_LLH_POISSON 0           /* PoissonLogLikelihood.h:19-53          */
+#define GCM_LLH_BARLOW_OA2021 1     /* BarlowBeestonBanff2022.h:92-294      */
+
+/* One dial input: the parameter reference of a DialInputBuffer with its
+ * mirror edges (DialInputBuffer.cpp:44-89), and the DialResponseSupervisor of
+ * the dial collection (DialResponseSupervisor.cpp:10-18).  Dials of different
+ * collections that read the same parameter use different inputs, which keeps
+ * the CPU path's per-collection clamp semantics. */
+typedef struct gcm_input {
+  int32_t parameter;    /* index into the parameter vector                 */
+  int32_t flags;        /* GCM_INPUT_*                                      */
+  double mirror_lower;  /* NaN: no mirroring                                */
+  double mirror_range;  /* mirror range (upper - lower)                     */
+  double min_response;  /* NaN: unset (collections default to 0)            */
+  double max_response;  /* NaN: unset                                       */
+} gcm_input;
+
+/* Dial tables in the reference's event-major EventDialCache order
+ * (EventDialCache.h:64-67): event e owns dials [dial_begin[e], dial_begin[e+1])
+ * in dial-list order, which is the order of the response product
+ * (EventDialCache.cpp:156-171). */
+typedef struct gcm_tables {
+  int64_t n_events;
+  const double* base_weight;  /* [n_events] Event::Weights::base          */
+  const int32_t* bin;         /* [n_events] global bin (sample offset
+                                 included), -1 = unbinned (dropped,
+                                 CacheManager.cpp:510)                     */
+  const int64_t* dial_begin;  /* [n_events+1]                               */
+  int64_t n_dials;
+  const uint8_t* dial_type;   /* [n_dials] GCM_DIAL_*                       */
+  const int32_t* dial_input;  /* [n_dials] index into inputs                */
+  const int64_t* data_begin;  /* [n_dials+1] offsets into dial_data         */
+  const double* dial_data;    /* payloads, layouts above                    */
+  int32_t n_inputs;
+  const gcm_input* inputs;    /* [n_inputs]                                 */
+  int32_t n_tables;
+  const int64_t* table_begin; /* [n_tables+1] offsets into the table vector */
+  int32_t n_parameters;
+  int32_t n_bins;
+  double reweight_cap;        /* GlobalEventReweightCap (EventDialCache.h:26-35);
+                                 NaN or +inf: disabled                     */
+} gcm_tables;
+
+/* Likelihood configuration: the BarlowLLH_BANFF_OA2021 options
+ * (BarlowBeestonBanff2022.h:29-53) that change arithmetic. */
+typedef struct gcm_llh_config {
+  int32_t type;                   /* GCM_LLH_*                              */
+  int32_t use_poisson_likelihood; /* BB: drop the beta penalty              */
+  double fractional_error_limit;  /* BB default 1e150                        */
+  double expected_value_minimum;  /* BB default -1                           */
+} gcm_llh_config;
+
+/* Kernel launch shape.  Zero fields mean "use the built-in default".  The
+ * tile record capacity changes the device layout, so it is fixed at build. */
+typedef struct gcm_launch_config {
+  int32_t block_threads;    /* threads per block of the fill kernel         */
+  int32_t tile_words;       /* record words (8 B) staged in LDS per tile    */
+  int32_t blocks_per_cu;    /* resident blocks per CU for the grid size     */
+  int32_t reserved;
+} gcm_launch_config;
+
+typedef struct gcm_context gcm_context;
+
+/* ---- library ------------------------------------------------------------ */
+/* Last error message of the calling thread ("" if none). */
+const char* gcm_last_error(void);
+/* ABI version (major*10000 + minor*100 + patch). */
+int gcm_version(void);
+/* Replaces Cache::Manager::HasGPU (CacheManager.h:73-78): number of visible
+ * devices. */
+int gcm_device_count(int* count);
+
+/* ---- lifetime (Cache::Manager ctor/Build, CacheManager.cpp:46-188,198-475) */
+int gcm_create(int device, gcm_context** out);
+int gcm_destroy(gcm_context* ctx);
+/* Precision and launch shape; call before gcm_build. */
+int gcm_set_precision(gcm_context* ctx, int precision);
+int gcm_set_launch_config(gcm_context* ctx, const gcm_launch_config* cfg);
+int gcm_get_launch_config(gcm_context* ctx, gcm_launch_config* cfg);
+/* Replaces Cache::Manager::Build + Update (CacheManager.cpp:198-475,477-740):
+ * validates the tables, lays them out for the device and uploads them.  The
+ * tables are copied; the caller's arrays may be freed afterwards.
+ * max_points is the largest K that gcm_set_parameters will be given. */
+int gcm_build(gcm_context* ctx, const gcm_tables* tables, int32_t max_points);
+/* Device-resident bytes (Cache::Manager::GetResidentMemory,
+ * CacheManager.h:192). */
+int gcm_resident_bytes(gcm_context* ctx, int64_t* bytes);
+/* Events kept after dropping unbinned ones; dials kept. */
+int gcm_counts(gcm_context* ctx, int64_t* events, int64_t* dials, int64_t* tiles);
+
+/* ---- per-evaluation inputs ----------------------------------------------- */
+/* Replaces Cache::Parameters::SetParameter (CacheParameters.cpp:141-154) for K
+ * parameter points at once: theta is [k_points][n_parameters], row-major. */
+int gcm_set_parameters(gcm_context* ctx, const double* theta,
+                       int32_t n_parameters, int32_t k_points);
+/* Tabulated/Kriged lookup tables for every point: [k_points][table_len]
+ * (the per-Fill host upload of WeightTabulated.cpp:164-171 and
+ * WeightKriged.cpp:216-223). */
+int gcm_set_table_values(gcm_context* ctx, const double* values,
+                         int64_t table_len, int32_t k_points);
+/* Likelihood: data histogram [n_bins], Gaussian priors [n_parameters]
+ * (sigma <= 0 or NaN: no penalty term; LikelihoodInterface.cpp:179-215 with a
+ * diagonal covariance). */
+int gcm_set_likelihood(gcm_context* ctx, const gcm_llh_config* cfg,
+                       const double* data_hist, const double* prior_mean,
+                       const double* prior_sigma);
+/* Keep per-event weights on the device so gcm_get_weights can fetch them
+ * (Cache::Manager::SetIsEventWeightCopyEnabled, CacheManager.h:104-109). */
+int gcm_enable_weights(gcm_context* ctx, int enable);
+
+/* ---- evaluation (Cache::Manager::Fill, CacheManager.cpp:750-810) --------- */
+/* Enqueue mirror/clamp, dial evaluation, weight product, histogram and
+ * likelihood for the K points set last, on `stream` (a hipStream_t, NULL for
+ * the context's own stream).  Asynchronous. */
+int gcm_fill(gcm_context* ctx, void* stream);
+/* Wait for the last fill; reports NaN weights (CacheWeights.cpp:83-86) as
+ * GCM_ERR_NUMERIC. */
+int gcm_sync(gcm_context* ctx);
+
+/* ---- fetch (CopyHistogramContents / CopyEventWeights,
+ *      CacheManager.cpp:832-866) ------------------------------------------- */
+/* Sum of weights and sum of squared weights per bin for point k. */
+int gcm_get_hist(gcm_context* ctx, int32_t k, double* sums, double* sums2);
+/* Per-event weights of point 0 in the caller's event order; unbinned events
+ * are left untouched. */
+int gcm_get_weights(gcm_context* ctx, double* weights);
+/* Total likelihood (stat + penalty) per point, and the two parts; any pointer
+ * may be NULL. */
+int gcm_get_llh(gcm_context* ctx, double* total, double* stat,
+                double* penalty);
+
+/* ---- kernel timing ------------------------------------------------------ */
+/* Record HIP events around the fill and finalize kernels of every gcm_fill on
+ * the stream they run on (replaces the AveragedTimer cacheFillTimer,
+ * CacheManager.h:128-135).  Enabling resets the totals. */
+int gcm_enable_timing(gcm_context* ctx, int enable);
+/* Totals over the completed fills since timing was enabled. */
+int gcm_get_timing(gcm_context* ctx, double* fill_ms, double* finalize_ms,
+                   int64_t* count);
+
+/* ---- event -> bin indexing (LoaderUtils.cpp:32-40, Bin.cpp:167-176) ------ */
+/* First bin whose [min, max) box holds the event on every variable (min ==
+ * max means equality); -1 if none.  values is [n_events][n_vars]; edges is
+ * [n_bins][n_vars][2].  Runs on `device`. */
+int gcm_bin_events(int device, int64_t n_events, int32_t n_vars,
+                   const double* values, int32_t n_bins, const double* edges,
+                   int32_t* out_bin);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GCM_H */

@@ -1,0 +1,564 @@
+// oracle.cpp -- TEST INFRASTRUCTURE (the oracle), never the product.
+//
+// A faithful CPU restatement of the reference's CPU reweight-and-fill path,
+// with the reference's own data structures, so that it serves both as the
+// parity oracle for the HIP path and as the timed CPU baseline:
+//
+//   Propagator::reweightEvents / refillHistograms  Propagator.cpp:360-393
+//     EventDialCache::reweightEntry                EventDialCache.cpp:156-171
+//       DialResponseCache::getResponse             EventDialCache.h:50-59
+//         DialInterface::evalResponse              DialInterface.h:57
+//           DialResponseSupervisor::process        DialResponseSupervisor.cpp:10-18
+//           <Dial>::evalResponse (virtual)         DialDefinitions/src/*.cpp
+//       DialInputBuffer::update (mirror, fmod)     DialInputBuffer.cpp:44-89
+//     Histogram::refillHistogram                   Histogram.cpp:107-135
+//   LLH: PoissonLogLikelihood::eval                PoissonLogLikelihood.h:19-53
+//        BarlowBeestonBanff2022::eval              BarlowBeestonBanff2022.h:92-294
+//        LikelihoodInterface penalty + total       LikelihoodInterface.cpp:165-215
+//   Event -> bin: LoaderUtils::fillBinIndex        LoaderUtils.cpp:32-40
+//                 Bin::isBetweenEdges              Bin.cpp:167-176
+//
+// Threading follows the reference partitioning: contiguous event ranges per
+// thread for the reweight, contiguous bin ranges per thread for the refill,
+// and a single-threaded LLH.  Only tests/, the smoke check and the bench's
+// CPU-baseline leg load this library.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../include/gcm.h"
+#include "oracle_calc.hpp"
+
+namespace oracle {
+
+// ---------------------------------------------------------------------------
+// Dial inputs (DialInputBuffer) and response supervisors.
+// ---------------------------------------------------------------------------
+struct DialInputBuffer {
+  int parameter{-1};
+  double mirror_lower{std::nan("")};
+  double mirror_range{std::nan("")};
+  double buffer{std::nan("unset")};
+  bool update_requested{true};
+
+  // DialInputBuffer.cpp:44-89.
+  void update(const double* theta) {
+    update_requested = false;
+    double v = theta[parameter];
+    if (!std::isnan(mirror_lower)) {
+      v = std::abs(std::fmod(v - mirror_lower, 2 * mirror_range));
+      if (v > mirror_range) {
+        v -= 2 * mirror_range;
+        v = -v;
+      }
+      v += mirror_lower;
+    }
+    if (buffer != v) {
+      update_requested = true;
+      buffer = v;
+    }
+  }
+};
+
+struct DialResponseSupervisor {
+  double min_response{std::nan("unset")};
+  double max_response{std::nan("unset")};
+  // DialResponseSupervisor.cpp:10-18.
+  double process(double r) const {
+    if (!std::isnan(min_response) && r < min_response) return min_response;
+    else if (!std::isnan(max_response) && r > max_response) return max_response;
+    return r;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Dials (DialBase hierarchy), evaluated through virtual dispatch as the
+// reference does.
+// ---------------------------------------------------------------------------
+struct DialBase {
+  virtual ~DialBase() = default;
+  virtual double evalResponse(const DialInputBuffer& in) const = 0;
+};
+
+struct NormDial : DialBase {  // Norm.h:21
+  double evalResponse(const DialInputBuffer& in) const override {
+    return in.buffer;
+  }
+};
+
+struct ShiftDial : DialBase {  // Shift.h:21
+  double shift{1};
+  double evalResponse(const DialInputBuffer&) const override { return shift; }
+};
+
+// Shared by the uniformly spaced splines: the record and the bounds used when
+// extrapolation is forbidden (MonotonicSpline.cpp:78-81).
+struct UniformKnotDial : DialBase {
+  std::vector<double> data;
+  bool allow_extrapolation{true};
+  double bound_min{0}, bound_max{0};
+  double input(const DialInputBuffer& in) const {
+    double x = in.buffer;
+    if (!allow_extrapolation) {
+      if (x <= bound_min) x = bound_min;
+      else if (x >= bound_max) x = bound_max;
+    }
+    return x;
+  }
+};
+
+struct MonotonicDial : UniformKnotDial {  // MonotonicSpline.cpp:71-84
+  double evalResponse(const DialInputBuffer& in) const override {
+    return calc_monotonic(input(in), -1E20, 1E20, data.data(),
+                          int(data.size() - 2));
+  }
+};
+
+struct CompactDial : UniformKnotDial {  // CompactSpline.cpp:71-84
+  double evalResponse(const DialInputBuffer& in) const override {
+    return calc_compact(input(in), -1E20, 1E20, data.data(),
+                        int(data.size() - 2));
+  }
+};
+
+struct UniformDial : UniformKnotDial {  // UniformSpline.cpp:47-60
+  double evalResponse(const DialInputBuffer& in) const override {
+    return calc_uniform(input(in), -1E20, 1E20, data.data(), int(data.size()));
+  }
+};
+
+struct GeneralDial : UniformKnotDial {  // GeneralSpline.cpp:33-42
+  double evalResponse(const DialInputBuffer& in) const override {
+    return calc_general(input(in), -1E20, 1E20, data.data(), int(data.size()));
+  }
+};
+
+struct GraphDial : DialBase {  // Graph.cpp:41-54
+  std::vector<double> data;
+  bool allow_extrapolation{true};
+  double evalResponse(const DialInputBuffer& in) const override {
+    double x = in.buffer;
+    if (!allow_extrapolation) {
+      if (x <= data[1]) return data[0];
+      else if (x >= data.back()) return data[data.size() - 2];
+    }
+    return calc_graph(x, -1E20, 1E20, data.data(), int(data.size()));
+  }
+};
+
+struct TabulatedDial : DialBase {  // Tabulated.h:16-28
+  const std::vector<double>* table{nullptr};
+  double fraction{0};  // holds a float-rounded value
+  int index{0};
+  double evalResponse(const DialInputBuffer&) const override {
+    return fraction * (*table)[index] + (1.0 - fraction) * (*table)[index + 1];
+  }
+};
+
+struct KrigedDial : DialBase {  // Kriged.cpp:14-21
+  const std::vector<double>* table{nullptr};
+  std::vector<std::pair<int, float>> weights;
+  double evalResponse(const DialInputBuffer&) const override {
+    const std::vector<double>& tbl = *table;
+    double v{0.0};
+    for (const std::pair<int, float>& w : weights) v += w.second * tbl[w.first];
+    return v;
+  }
+};
+
+// DialInterface.h:57.
+struct DialInterface {
+  std::unique_ptr<DialBase> dial;
+  DialInputBuffer* input{nullptr};
+  const DialResponseSupervisor* supervisor{nullptr};
+  double evalResponse() const {
+    return supervisor->process(dial->evalResponse(*input));
+  }
+};
+
+struct Event {
+  double base{1};
+  double current{1};
+  int bin{-1};
+};
+
+// EventDialCache.h:38-60 and EventDialCache.cpp:156-171.
+struct DialResponseCache {
+  DialInterface* iface;
+  double response{std::nan("unset")};
+  bool* update_requested;
+  double getResponse() {
+    if (*update_requested) response = iface->evalResponse();
+    return response;
+  }
+};
+
+struct CacheEntry {
+  Event* event;
+  std::vector<DialResponseCache> dials;
+};
+
+struct BinContent {
+  double sumWeights{0};
+  double sqrtSumSqWeights{0};
+};
+
+struct Model {
+  int n_threads{1};
+  int n_parameters{0};
+  int n_bins{0};
+  bool cap_enabled{false};
+  double cap{0};
+  std::vector<DialInputBuffer> inputs;
+  std::vector<DialResponseSupervisor> supervisors;
+  std::vector<std::vector<double>> tables;
+  std::vector<Event> events;
+  std::vector<DialInterface> interfaces;
+  std::vector<CacheEntry> cache;
+  std::vector<std::vector<Event*>> bin_events;  // Histogram BinContext lists
+  std::vector<BinContent> hist;
+
+  void reweightEntry(CacheEntry& entry) {
+    double temp{1};
+    for (auto& d : entry.dials) temp *= d.getResponse();
+    if (cap_enabled && temp > cap) temp = cap;
+    entry.event->current = entry.event->base;
+    entry.event->current *= temp;
+  }
+
+  // Histogram.cpp:107-135 for the bins [b0, b1).
+  void refill(int b0, int b1) {
+    double w;
+    for (int b = b0; b < b1; ++b) {
+      BinContent& c = hist[b];
+      c.sumWeights = 0;
+      c.sqrtSumSqWeights = 0;
+      for (Event* e : bin_events[b]) {
+        w = e->current;
+        c.sumWeights += w;
+        c.sqrtSumSqWeights += w * w;
+      }
+      c.sqrtSumSqWeights = std::sqrt(c.sqrtSumSqWeights);
+    }
+  }
+};
+
+// Contiguous [begin, end) share of n items for thread i of n_threads.
+static void thread_bounds(int i, int n_threads, int64_t n, int64_t* b,
+                          int64_t* e) {
+  int64_t chunk = n / n_threads;
+  int64_t rest = n % n_threads;
+  *b = i * chunk + std::min<int64_t>(i, rest);
+  *e = *b + chunk + (i < rest ? 1 : 0);
+}
+
+template <typename F>
+static void parallel(int n_threads, F&& f) {
+  if (n_threads <= 1) {
+    f(0);
+    return;
+  }
+  std::vector<std::thread> pool;
+  pool.reserve(n_threads);
+  for (int i = 0; i < n_threads; ++i) pool.emplace_back([&f, i] { f(i); });
+  for (auto& t : pool) t.join();
+}
+
+thread_local std::string g_error;
+
+}  // namespace oracle
+
+using namespace oracle;
+
+extern "C" {
+
+const char* oracle_last_error(void) { return g_error.c_str(); }
+
+// Build the reference object graph from the same tables the device consumes.
+void* oracle_create(const gcm_tables* t, int n_threads) {
+  try {
+    auto m = std::make_unique<Model>();
+    m->n_threads = std::max(1, n_threads);
+    m->n_parameters = t->n_parameters;
+    m->n_bins = t->n_bins;
+    m->cap_enabled = std::isfinite(t->reweight_cap);
+    m->cap = t->reweight_cap;
+    m->inputs.resize(t->n_inputs);
+    m->supervisors.resize(t->n_inputs);
+    for (int i = 0; i < t->n_inputs; ++i) {
+      const gcm_input& in = t->inputs[i];
+      if (in.parameter < 0 || in.parameter >= t->n_parameters) {
+        g_error = "input parameter out of range";
+        return nullptr;
+      }
+      m->inputs[i].parameter = in.parameter;
+      m->inputs[i].mirror_lower = in.mirror_lower;
+      m->inputs[i].mirror_range = in.mirror_range;
+      m->supervisors[i].min_response = in.min_response;
+      m->supervisors[i].max_response = in.max_response;
+    }
+    m->tables.resize(t->n_tables);
+    for (int i = 0; i < t->n_tables; ++i)
+      m->tables[i].assign(t->table_begin[i + 1] - t->table_begin[i], 1.0);
+
+    m->events.resize(t->n_events);
+    m->interfaces.resize(t->n_dials);
+    for (int64_t d = 0; d < t->n_dials; ++d) {
+      const double* p = t->dial_data + t->data_begin[d];
+      const int64_t len = t->data_begin[d + 1] - t->data_begin[d];
+      const int in = t->dial_input[d];
+      if (in < 0 || in >= t->n_inputs) {
+        g_error = "dial input out of range";
+        return nullptr;
+      }
+      const bool extrap = !(t->inputs[in].flags & GCM_INPUT_NO_EXTRAPOLATION);
+      std::unique_ptr<DialBase> dial;
+      switch (t->dial_type[d]) {
+        case GCM_DIAL_NORM: dial = std::make_unique<NormDial>(); break;
+        case GCM_DIAL_SHIFT: {
+          auto s = std::make_unique<ShiftDial>();
+          s->shift = p[0];
+          dial = std::move(s);
+          break;
+        }
+        case GCM_DIAL_MONOTONIC:
+        case GCM_DIAL_COMPACT:
+        case GCM_DIAL_UNIFORM:
+        case GCM_DIAL_GENERAL: {
+          std::unique_ptr<UniformKnotDial> s;
+          int knots;
+          switch (t->dial_type[d]) {
+            case GCM_DIAL_MONOTONIC: s = std::make_unique<MonotonicDial>(); knots = int(len - 2); break;
+            case GCM_DIAL_COMPACT: s = std::make_unique<CompactDial>(); knots = int(len - 2); break;
+            case GCM_DIAL_UNIFORM: s = std::make_unique<UniformDial>(); knots = int((len - 2) / 2); break;
+            default: s = std::make_unique<GeneralDial>(); knots = int((len - 2) / 3); break;
+          }
+          s->data.assign(p, p + len);
+          s->allow_extrapolation = extrap;
+          s->bound_min = p[0];
+          if (t->dial_type[d] == GCM_DIAL_GENERAL)
+            s->bound_max = p[2 + 3 * (knots - 1) + 2];
+          else
+            s->bound_max = p[0] + (knots - 1) * p[1];
+          dial = std::move(s);
+          break;
+        }
+        case GCM_DIAL_GRAPH: {
+          auto g = std::make_unique<GraphDial>();
+          g->data.assign(p, p + len);
+          g->allow_extrapolation = extrap;
+          dial = std::move(g);
+          break;
+        }
+        case GCM_DIAL_TABULATED: {
+          auto tb = std::make_unique<TabulatedDial>();
+          tb->table = &m->tables.at(int(p[0]));
+          tb->index = int(p[1]);
+          tb->fraction = (float)p[2];
+          dial = std::move(tb);
+          break;
+        }
+        case GCM_DIAL_KRIGED: {
+          auto kr = std::make_unique<KrigedDial>();
+          kr->table = &m->tables.at(int(p[0]));
+          for (int64_t k = 1; k + 1 < len; k += 2)
+            kr->weights.emplace_back(int(p[k]), float(p[k + 1]));
+          dial = std::move(kr);
+          break;
+        }
+        default:
+          g_error = "unsupported dial type";
+          return nullptr;
+      }
+      m->interfaces[d].dial = std::move(dial);
+      m->interfaces[d].input = &m->inputs[in];
+      m->interfaces[d].supervisor = &m->supervisors[in];
+    }
+    m->cache.resize(t->n_events);
+    m->bin_events.resize(t->n_bins);
+    for (int64_t e = 0; e < t->n_events; ++e) {
+      Event& ev = m->events[e];
+      ev.base = t->base_weight[e];
+      ev.current = ev.base;
+      ev.bin = t->bin[e];
+      CacheEntry& ce = m->cache[e];
+      ce.event = &ev;
+      for (int64_t d = t->dial_begin[e]; d < t->dial_begin[e + 1]; ++d) {
+        DialInterface* di = &m->interfaces[d];
+        ce.dials.push_back(DialResponseCache{di, std::nan("unset"),
+                                             &di->input->update_requested});
+      }
+      if (ev.bin >= 0) {
+        if (ev.bin >= t->n_bins) {
+          g_error = "bin out of range";
+          return nullptr;
+        }
+        m->bin_events[ev.bin].push_back(&ev);
+      }
+    }
+    m->hist.resize(t->n_bins);
+    return m.release();
+  } catch (const std::exception& ex) {
+    g_error = ex.what();
+    return nullptr;
+  }
+}
+
+void oracle_destroy(void* h) { delete static_cast<Model*>(h); }
+
+// Tabulated/Kriged tables for the next evaluation (the factory callback
+// output, TabulatedDialFactory.cpp:133-141).
+int oracle_set_table_values(void* h, const double* values, int64_t len) {
+  Model* m = static_cast<Model*>(h);
+  int64_t off = 0;
+  for (auto& t : m->tables) {
+    if (off + (int64_t)t.size() > len) return 1;
+    std::copy(values + off, values + off + t.size(), t.begin());
+    off += t.size();
+  }
+  return 0;
+}
+
+// One propagation: input-buffer update, reweight (event ranges per thread),
+// refill (bin ranges per thread).  Outputs are optional.
+int oracle_propagate(void* h, const double* theta, double* weights,
+                     double* sums, double* sqrt_sum_sq) {
+  Model* m = static_cast<Model*>(h);
+  for (auto& in : m->inputs) in.update(theta);
+  const int64_t n = (int64_t)m->cache.size();
+  parallel(m->n_threads, [&](int i) {
+    int64_t b, e;
+    thread_bounds(i, m->n_threads, n, &b, &e);
+    for (int64_t k = b; k < e; ++k) m->reweightEntry(m->cache[k]);
+  });
+  parallel(m->n_threads, [&](int i) {
+    int64_t b, e;
+    thread_bounds(i, m->n_threads, m->n_bins, &b, &e);
+    m->refill(int(b), int(e));
+  });
+  if (weights)
+    for (int64_t k = 0; k < n; ++k) weights[k] = m->events[k].current;
+  for (int b = 0; b < m->n_bins; ++b) {
+    if (sums) sums[b] = m->hist[b].sumWeights;
+    if (sqrt_sum_sq) sqrt_sum_sq[b] = m->hist[b].sqrtSumSqWeights;
+  }
+  return 0;
+}
+
+// PoissonLogLikelihood.h:19-53 (err unused).
+double oracle_llh_poisson_bin(double data, double pred) {
+  double llh{0};
+  if (pred <= 0) llh = std::numeric_limits<double>::infinity();
+  else if (data <= 0) llh = 2.0 * pred;
+  else llh = 2.0 * (pred - data + data * std::log(data / pred));
+  return llh;
+}
+
+// BarlowBeestonBanff2022.h:150-294 with BBNoUpdateWeights = false.
+// err is sqrtSumSqWeights; the reference squares it (:125-126).
+double oracle_llh_barlow_bin(double data, double pred, double err,
+                             const gcm_llh_config* cfg) {
+  double mcuncert = err;
+  mcuncert *= mcuncert;
+  if (!std::isfinite(mcuncert) || mcuncert < 0.0)
+    return std::numeric_limits<double>::infinity();
+  if (pred <= cfg->expected_value_minimum) {
+    if (pred <= 0.0 && data > 0.0) return std::numeric_limits<double>::infinity();
+    return 0.0;
+  }
+  double newmc = pred;
+  double penalty = 0.0;
+  if (mcuncert > 0.0) {
+    double fractional =
+        std::min(std::sqrt(mcuncert) / pred, cfg->fractional_error_limit);
+    double temp = pred * fractional * fractional - 1;
+    double temp2 = temp * temp + 4 * data * fractional * fractional;
+    if (temp2 < 0) return std::numeric_limits<double>::infinity();
+    double beta = (-1 * temp + std::sqrt(temp2)) / 2.;
+    newmc = std::max(pred * beta, std::numeric_limits<double>::min());
+    penalty = (beta - 1) * (beta - 1) / (2 * fractional * fractional);
+  }
+  double stat = 0;
+  if (data <= std::numeric_limits<double>::epsilon()) {
+    stat = newmc;
+  } else if (newmc > std::numeric_limits<double>::min()) {
+    stat = newmc - data + data * (std::log(data) - std::log(newmc));
+  } else {
+    const double mc = std::numeric_limits<double>::min();
+    stat = mc - data + data * (std::log(data) - std::log(mc));
+  }
+  double chisq = 2.0 * stat;
+  if (!cfg->use_poisson_likelihood) chisq += 2.0 * penalty;
+  return chisq;
+}
+
+// JointProbabilityBase::eval(SamplePair) summed over bins in order, plus the
+// diagonal-prior penalty of LikelihoodInterface.cpp:179-215; total =
+// stat + penalty.
+int oracle_llh(const gcm_llh_config* cfg, int n_bins, const double* data,
+               const double* sums, const double* sqrt_sum_sq, int n_par,
+               const double* theta, const double* prior_mean,
+               const double* prior_sigma, double* total, double* stat_out,
+               double* penalty_out) {
+  double stat = 0;
+  for (int b = 0; b < n_bins; ++b) {
+    if (cfg->type == GCM_LLH_POISSON)
+      stat += oracle_llh_poisson_bin(data[b], sums[b]);
+    else
+      stat += oracle_llh_barlow_bin(data[b], sums[b], sqrt_sum_sq[b], cfg);
+  }
+  double pen = 0;
+  for (int p = 0; p < n_par; ++p) {
+    if (!prior_sigma || !(prior_sigma[p] > 0)) continue;
+    double z = (theta[p] - prior_mean[p]) / prior_sigma[p];
+    pen += z * z;
+  }
+  if (stat_out) *stat_out = stat;
+  if (penalty_out) *penalty_out = pen;
+  if (total) *total = stat + pen;
+  return 0;
+}
+
+// LoaderUtils.cpp:32-40 with Bin::isBetweenEdges (Bin.cpp:167-176), default
+// [min, max) edges.
+int oracle_bin_events(int64_t n_events, int n_vars, const double* values,
+                      int n_bins, const double* edges, int32_t* out) {
+  for (int64_t e = 0; e < n_events; ++e) {
+    const double* v = values + e * n_vars;
+    int found = -1;
+    for (int b = 0; b < n_bins && found < 0; ++b) {
+      bool in = true;
+      for (int k = 0; k < n_vars && in; ++k) {
+        const double lo = edges[(int64_t(b) * n_vars + k) * 2];
+        const double hi = edges[(int64_t(b) * n_vars + k) * 2 + 1];
+        if (lo == hi) in = (lo == v[k]);
+        else if (lo > v[k]) in = false;
+        else if (hi <= v[k]) in = false;
+      }
+      if (in) found = b;
+    }
+    out[e] = found;
+  }
+  return 0;
+}
+
+// Direct access to the restated Calculate* functions for the golden checks.
+double oracle_calc(int type, double x, double lo, double hi, const double* data,
+                   int dim) {
+  switch (type) {
+    case GCM_DIAL_MONOTONIC: return calc_monotonic(x, lo, hi, data, dim);
+    case GCM_DIAL_COMPACT: return calc_compact(x, lo, hi, data, dim);
+    case GCM_DIAL_UNIFORM: return calc_uniform(x, lo, hi, data, dim);
+    case GCM_DIAL_GENERAL: return calc_general(x, lo, hi, data, dim);
+    case GCM_DIAL_GRAPH: return calc_graph(x, lo, hi, data, dim);
+    default: return std::nan("");
+  }
+}
+
+}  // extern "C"
