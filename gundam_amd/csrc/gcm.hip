@@ -315,6 +315,30 @@ __device__ double llh_bin(const LlhParams& c, double data, double pred,
   return chisq;
 }
 
+// Wave-level sum of (s, c) pairs with a fixed butterfly shape: the result is
+// the same on every run.  With kComp the pairs are combined by two-sum.
+template <bool kComp>
+__device__ __forceinline__ void wave_sum(double& s, double& c) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    const double so = __shfl_xor(s, off);
+    if (kComp) {
+      const double co = __shfl_xor(c, off);
+      double t, e;
+      // combine in lane order so both partners compute the same value
+      if ((threadIdx.x & off) == 0) two_sum(s, so, t, e);
+      else two_sum(so, s, t, e);
+      s = t;
+      c = ((threadIdx.x & off) == 0 ? c + co : co + c) + e;
+    } else {
+      s = ((threadIdx.x & off) == 0) ? s + so : so + s;
+    }
+  }
+}
+
+// One wave per bin: the lanes read the bin's partial slots coalesced, then a
+// fixed-shape wave reduction; the block's bins are summed in bin order, and
+// the last block of each point reduces the block results with a fixed tree.
 template <bool kComp>
 __global__ void __launch_bounds__(kFinalizeBlock)
     finalize_kernel(const double* __restrict__ psum,
@@ -327,16 +351,17 @@ __global__ void __launch_bounds__(kFinalizeBlock)
                     double* llh_part, unsigned* tickets,
                     const double* __restrict__ penalty, double* stat_out,
                     double* pen_out, double* total_out) {
+  constexpr int kBinsPerBlock = kFinalizeBlock / kWave;
   __shared__ double s_red[kFinalizeBlock];
   __shared__ int s_last;
   const int k = blockIdx.y;
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int b = blockIdx.x * kBinsPerBlock + wave;
   const size_t kp = (size_t)k * n_partials;
-  double term = 0.0;
   if (b < n_bins) {
     double s = 0.0, s2 = 0.0, c = 0.0, c2 = 0.0;
     const uint32_t p0 = bin_pbeg[b], p1 = bin_pbeg[b + 1];
-    for (uint32_t p = p0; p < p1; ++p) {
+    for (uint32_t p = p0 + lane; p < p1; p += kWave) {
       if (kComp) {
         double t, e;
         two_sum(s, psum[kp + p], t, e);
@@ -350,36 +375,48 @@ __global__ void __launch_bounds__(kFinalizeBlock)
         s2 += psum2[kp + p];
       }
     }
+    wave_sum<kComp>(s, c);
+    wave_sum<kComp>(s2, c2);
     if (kComp) {
       s = s + c;
       s2 = s2 + c2;
     }
-    hist[(size_t)k * n_bins + b] = s;
-    hist2[(size_t)k * n_bins + b] = s2;
-    term = data ? llh_bin(cfg, data[b], s, sqrt(s2)) : 0.0;
+    if (lane == 0) {
+      hist[(size_t)k * n_bins + b] = s;
+      hist2[(size_t)k * n_bins + b] = s2;
+      s_red[wave] = data ? llh_bin(cfg, data[b], s, sqrt(s2)) : 0.0;
+    }
+  } else if (lane == 0) {
+    s_red[wave] = 0.0;
   }
-  s_red[threadIdx.x] = term;
   __syncthreads();
-  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-    if (threadIdx.x < st) s_red[threadIdx.x] += s_red[threadIdx.x + st];
-    __syncthreads();
-  }
   if (threadIdx.x == 0) {
-    llh_part[(size_t)k * gridDim.x + blockIdx.x] = s_red[0];
+    double part = 0.0;
+    for (int w = 0; w < kBinsPerBlock; ++w) part += s_red[w];
+    llh_part[(size_t)k * gridDim.x + blockIdx.x] = part;
     __threadfence();
     const unsigned ticket = atomicAdd(&tickets[k], 1u);
     s_last = (ticket == gridDim.x - 1);
   }
   __syncthreads();
-  if (s_last && threadIdx.x == 0) {
+  if (s_last) {
     __threadfence();
-    const volatile double* parts = llh_part + (size_t)k * gridDim.x;
-    double stat = 0.0;
-    for (unsigned j = 0; j < gridDim.x; ++j) stat += parts[j];
-    stat_out[k] = stat;
-    pen_out[k] = penalty[k];
-    total_out[k] = stat + penalty[k];
-    tickets[k] = 0;  // ready for the next launch, no memset needed
+    const double* parts = llh_part + (size_t)k * gridDim.x;
+    double acc = 0.0;
+    for (unsigned j = threadIdx.x; j < gridDim.x; j += blockDim.x) acc += parts[j];
+    s_red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+      if (threadIdx.x < st) s_red[threadIdx.x] += s_red[threadIdx.x + st];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      const double stat = s_red[0];
+      stat_out[k] = stat;
+      pen_out[k] = penalty[k];
+      total_out[k] = stat + penalty[k];
+      tickets[k] = 0;  // ready for the next launch, no memset needed
+    }
   }
 }
 
@@ -884,6 +921,9 @@ int gcm_build(gcm_context* c, const gcm_tables* t, int32_t max_points) {
   c->table_len = t->n_tables ? t->table_begin[t->n_tables] : 0;
   c->max_points = max_points;
   c->cap = std::isfinite(t->reweight_cap) ? t->reweight_cap : __builtin_huge_val();
+  // dial slots per tile: enough for the densest events (norm dials have no
+  // record words), bounded so that descriptors + responses stay small in LDS
+  c->max_dials = std::min(1024, std::max(256, c->launch.tile_words / 4));
   int rc = (c->precision == GCM_PRECISION_FP32) ? build_layout<float>(c, t)
                                                  : build_layout<double>(c, t);
   if (rc != GCM_OK) {
@@ -904,7 +944,7 @@ int gcm_build(gcm_context* c, const gcm_tables* t, int32_t max_points) {
     mx[i] = std::isnan(in.max_response) ? inf : in.max_response;
   }
   const size_t K = (size_t)max_points;
-  c->fin_blocks = (c->n_bins + kFinalizeBlock - 1) / kFinalizeBlock;
+  c->fin_blocks = (c->n_bins + kFinalizeBlock / kWave - 1) / (kFinalizeBlock / kWave);
   rc = GCM_OK;
   auto step = [&](int r) { if (rc == GCM_OK) rc = r; };
   step(upload(c, &c->d_in_par, par.data(), par.size()));
