@@ -93,12 +93,14 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--batch", type=int, default=64,
                     help="points per launch for the batched figure (0: skip)")
-    ap.add_argument("--cpu-sample", type=int, default=100000)
+    ap.add_argument("--cpu-sample", type=int, default=1000000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--tile-words", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--tune-events", type=int, default=500000)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -124,6 +126,19 @@ def main():
     fill_bytes = alg_bytes - 16 * t.n_bins  # the finalize kernel's share
 
     K = max(1, args.batch)
+    nominal = wl.nominal()
+    tuned = None
+    if not args.no_autotune and not (args.block or args.tile_words or args.blocks_per_cu):
+        from gundam_amd.autotune import autotune
+        tp = wl.random_points(1, 0.5, seed=args.seed + 5)
+        tuned = autotune(t, tp, wl.table_values(tp) if t.table_len else None,
+                         precision=args.precision, n_events=args.tune_events,
+                         reps=3, log=log)
+        args.block, args.tile_words, args.blocks_per_cu = (
+            tuned["block_threads"], tuned["tile_words"], tuned["blocks_per_cu"])
+        log(f"autotuned in {tuned['tune_seconds']:.1f}s over {tuned['candidates']} "
+            f"shapes: {args.tile_words} words/tile, {args.block}+64 threads, "
+            f"{args.blocks_per_cu} blocks/CU")
     t0 = time.time()
     cm = CacheManager(0, precision=args.precision, block_threads=args.block,
                       tile_words=args.tile_words,
@@ -133,7 +148,6 @@ def main():
     log(f"built in {time.time()-t0:.1f}s: {counts}, resident "
         f"{cm.resident_memory()/2**30:.2f} GiB, launch {cm.launch_config()}")
 
-    nominal = wl.nominal()
     cm.propagate_parameters(nominal, wl.table_values(nominal))
     data = cm.hist(0)[0].copy()  # Asimov data at nominal
     cm.set_likelihood("barlow", data, wl.prior_mean, wl.prior_sigma)
@@ -229,6 +243,7 @@ def main():
             "parameters": int(t.n_parameters),
             "precision": args.precision,
             "launch": cm.launch_config(),
+            "autotune": tuned,
         },
         "roofline": {
             "bound": "hbm",

@@ -1,0 +1,87 @@
+"""Launch-shape autotuner for the fill kernel.
+
+The tile record capacity fixes the device layout, so every candidate is built
+afresh on a prefix of the caller's events (same dial mix, smaller count) and
+timed with the library's own kernel timers (HIP events around the fill
+kernel).  The candidate with the smallest fill time per evaluation wins.
+"""
+from __future__ import annotations
+
+import itertools
+import time
+from typing import Iterable, Optional
+
+import numpy as np
+
+from .cache_manager import CacheManager
+from .tables import Tables
+
+LDS_BYTES_PER_CU = 160 * 1024
+
+
+def lds_per_block(tile_words: int, n_inputs: int, k_points: int = 1) -> int:
+    """LDS of one fill block (mirrors the host computation in gcm.hip)."""
+    max_dials = min(1024, max(256, (tile_words // 8 + 127) // 128 * 128))
+    in_bytes = k_points * n_inputs * 32
+    in_pad = (in_bytes + 1023) // 1024 * 1024 if in_bytes <= 16384 else 0
+    return in_pad + 2 * (tile_words * 8 + max_dials * 8) + 2 * max_dials * 8
+
+
+def head(t: Tables, n_events: int) -> Tables:
+    """The first n_events events of t with their dials (views, no copy)."""
+    n = min(n_events, t.n_events)
+    nd = int(t.dial_begin[n])
+    return Tables(base_weight=t.base_weight[:n], bin=t.bin[:n],
+                  dial_begin=t.dial_begin[:n + 1], dial_type=t.dial_type[:nd],
+                  dial_input=t.dial_input[:nd], data_begin=t.data_begin[:nd + 1],
+                  dial_data=t.dial_data, inputs=t.inputs,
+                  table_begin=t.table_begin, n_parameters=t.n_parameters,
+                  n_bins=t.n_bins, reweight_cap=t.reweight_cap,
+                  _keep=[t])
+
+
+def candidates(n_inputs: int, precision: str = "fp64") -> list:
+    out = []
+    for tile, block, bpc in itertools.product((1024, 2048, 3072, 4096),
+                                              (128, 192, 256, 384),
+                                              (1, 2, 3, 4, 5, 6, 8)):
+        if lds_per_block(tile, n_inputs) * bpc > LDS_BYTES_PER_CU:
+            continue
+        if (block + 64) * bpc > 2048:  # 32 waves per CU
+            continue
+        out.append({"tile_words": tile, "block_threads": block, "blocks_per_cu": bpc})
+    return out
+
+
+def autotune(tables: Tables, theta: np.ndarray, table_values: Optional[np.ndarray],
+             precision: str = "fp64", n_events: int = 1_000_000, reps: int = 5,
+             cands: Optional[Iterable[dict]] = None, device: int = 0,
+             log=None) -> dict:
+    sub = head(tables, n_events)
+    cands = list(cands) if cands is not None else candidates(
+        int(tables.inputs.shape[0]), precision)
+    results = []
+    t_start = time.time()
+    for c in cands:
+        cm = CacheManager(device, precision=precision, **c)
+        try:
+            cm.build(sub)
+            cm.propagate_parameters(theta, table_values)
+            cm.enable_timing(True)
+            for _ in range(reps):
+                cm.fill()
+            cm.sync()
+            tm = cm.timing()
+            ms = tm["fill_ms"] / max(1, tm["count"])
+        finally:
+            cm.close()
+        results.append((ms, c))
+        if log:
+            log(f"autotune {c}: fill {ms:.4f} ms")
+    results.sort(key=lambda r: r[0])
+    best = dict(results[0][1])
+    best["fill_ms_subset"] = results[0][0]
+    best["subset_events"] = sub.n_events
+    best["candidates"] = len(results)
+    best["tune_seconds"] = time.time() - t_start
+    return best

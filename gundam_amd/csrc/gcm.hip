@@ -45,9 +45,10 @@ namespace gcm {
 constexpr int kWave = 64;
 constexpr int kMaxTileEvents = 64;
 constexpr int kDefaultBlock = 256;
-constexpr int kDefaultTileWords = 4096;  // 32 KiB of records per tile
+constexpr int kDefaultTileWords = 2048;  // 16 KiB of records per tile
 constexpr int kDefaultBlocksPerCU = 3;
 constexpr int kFinalizeBlock = 256;
+constexpr size_t kInputsLdsMax = 16384;  // stage input records in LDS up to this
 
 struct TileHdr {
   uint32_t first_event;
@@ -79,12 +80,26 @@ int fail(int code, const std::string& msg) {
 // Kernels
 // ---------------------------------------------------------------------------
 
+// Per-point record of one dial input: the mirrored value and the response
+// supervisor bounds (unset bounds are -+inf).
+struct InputRec {
+  double x;
+  double lo;
+  double hi;
+  int32_t flags;
+  int32_t pad;
+};
+static_assert(sizeof(InputRec) == 32, "input record is 32 bytes");
+
 // Per point k (one block each): mirrored input values and the prior penalty.
 __global__ void prologue_kernel(const double* __restrict__ theta, int n_par,
                                 const int32_t* __restrict__ in_par,
                                 const double* __restrict__ in_mlo,
                                 const double* __restrict__ in_mrange,
-                                int n_inputs, double* __restrict__ xin,
+                                const double* __restrict__ in_min,
+                                const double* __restrict__ in_max,
+                                const int32_t* __restrict__ in_flags,
+                                int n_inputs, InputRec* __restrict__ xin,
                                 const double* __restrict__ prior_mean,
                                 const double* __restrict__ prior_sigma,
                                 double* __restrict__ penalty) {
@@ -102,7 +117,13 @@ __global__ void prologue_kernel(const double* __restrict__ theta, int n_par,
       }
       v += lo;
     }
-    xin[(size_t)k * n_inputs + i] = v;
+    InputRec r;
+    r.x = v;
+    r.lo = in_min[i];
+    r.hi = in_max[i];
+    r.flags = in_flags[i];
+    r.pad = 0;
+    xin[(size_t)k * n_inputs + i] = r;
   }
   if (threadIdx.x == 0) {
     // LikelihoodInterface.cpp:186-206 with a diagonal covariance, summed in
@@ -137,11 +158,8 @@ struct FillArgs {
   const int32_t* ev_bin;
   const uint32_t* ev_dial;
   const int32_t* ev_result;
-  const double* xin;
+  const InputRec* xin;
   int n_inputs;
-  const int32_t* in_flags;
-  const double* in_min;
-  const double* in_max;
   const double* tab;
   int64_t table_len;
   double cap;
@@ -153,114 +171,174 @@ struct FillArgs {
   int64_t n_partials;
   double* weights_out;
   int* flags;
-  int tile_words;  // record words per tile (capacity, type R)
-  int max_dials;
+  int rec_bytes;   // LDS record capacity per buffer (multiple of 1 KiB)
+  int max_dials;   // LDS descriptor/response capacity (multiple of 128)
+  int inputs_lds;  // input records staged in LDS (K * n_inputs * 32 bytes)
 };
 
-// R: record/evaluation type.  kComp: compensated (two-sum) bin partials.
-template <typename R, bool kComp>
-__global__ void __launch_bounds__(512) fill_kernel(FillArgs a) {
-  extern __shared__ __align__(16) unsigned char lds[];
-  R* s_rec = reinterpret_cast<R*>(lds);
-  uint64_t* s_desc =
-      reinterpret_cast<uint64_t*>(lds + (size_t)a.tile_words * sizeof(R));
-  double* s_resp = reinterpret_cast<double*>(s_desc + a.max_dials);
-  const R* rec = reinterpret_cast<const R*>(a.rec);
+typedef __attribute__((address_space(3))) void* lds_void_ptr;
+typedef __attribute__((address_space(1))) void* global_void_ptr;
 
-  for (int64_t t = blockIdx.x; t < a.n_tiles; t += gridDim.x) {
+// Block barrier that waits only for this wave's LDS traffic (not for its
+// global stores or, in the producer, for anything it has not waited on).
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// Producer wave: copy `bytes` (rounded up to 1 KiB; the sources are padded)
+// from global memory straight into LDS with 16-byte-per-lane DMA loads.
+__device__ __forceinline__ void dma_to_lds(const unsigned char* src,
+                                           unsigned char* dst, int bytes,
+                                           int lane) {
+  for (int j = 0; j < bytes; j += 1024) {
+    __builtin_amdgcn_global_load_lds((global_void_ptr)(src + j + lane * 16),
+                                     (lds_void_ptr)(dst + j), 16, 0, 0);
+  }
+}
+
+// The fill kernel.  Waves 0..C-1 are consumers, the last wave is the
+// producer.  Step s covers (tile, point); the producer prefetches the next
+// tile into the other LDS buffer while the consumers evaluate the current
+// one, and one barrier per step hands both the records and the responses
+// over.  Responses are double-buffered so that consumer wave 0 can run the
+// event phase of step s while the others evaluate step s+1.
+template <typename R, bool kComp, bool kInputsLds>
+__global__ void __launch_bounds__(576) fill_kernel(FillArgs a) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const int nwaves = blockDim.x / kWave;
+  const int wave = threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
+  const bool producer = wave == nwaves - 1;
+  const int cons_threads = (nwaves - 1) * kWave;
+  const size_t in_bytes = kInputsLds ? (size_t)a.k_points * a.n_inputs * sizeof(InputRec) : 0;
+  const size_t in_pad = (in_bytes + 1023) / 1024 * 1024;
+  InputRec* s_in = reinterpret_cast<InputRec*>(lds);
+  unsigned char* s_buf[2];
+  const size_t buf_bytes = (size_t)a.rec_bytes + (size_t)a.max_dials * 8;
+  s_buf[0] = lds + in_pad;
+  s_buf[1] = s_buf[0] + buf_bytes;
+  double* s_resp[2];
+  s_resp[0] = reinterpret_cast<double*>(s_buf[1] + buf_bytes);
+  s_resp[1] = s_resp[0] + a.max_dials;
+  const unsigned char* rec = reinterpret_cast<const unsigned char*>(a.rec);
+  const unsigned char* desc = reinterpret_cast<const unsigned char*>(a.desc);
+
+  if (kInputsLds) {
+    const int n = a.k_points * a.n_inputs;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s_in[i] = a.xin[i];
+  }
+  int64_t t = blockIdx.x;
+  if (producer && t < a.n_tiles) {
     const TileHdr h = a.tiles[t];
-    // Stage the tile's records and descriptors: 16-byte coalesced loads.
-    {
-      const int n16 = (int)((h.rec_words * sizeof(R) + 15) / 16);
-      const uint4* src = reinterpret_cast<const uint4*>(rec + h.rec_begin);
-      uint4* dst = reinterpret_cast<uint4*>(s_rec);
-      for (int i = threadIdx.x; i < n16; i += blockDim.x) dst[i] = src[i];
-      const int nd2 = (h.n_dials + 1) / 2;
-      const uint4* dsrc = reinterpret_cast<const uint4*>(a.desc + h.first_dial);
-      uint4* ddst = reinterpret_cast<uint4*>(s_desc);
-      for (int i = threadIdx.x; i < nd2; i += blockDim.x) ddst[i] = dsrc[i];
-    }
-    __syncthreads();
-    for (int k = 0; k < a.k_points; ++k) {
-      const double* xk = a.xin + (size_t)k * a.n_inputs;
-      const double* tk = a.tab + (size_t)k * a.table_len;
-      for (int i = threadIdx.x; i < h.n_dials; i += blockDim.x) {
-        const uint64_t d = s_desc[i];
-        const int in = (int)(d & 0xffff);
-        const int type = (int)((d >> 16) & 0xf);
-        const int n = (int)((d >> 20) & 0xff);
-        const int dest = (int)((d >> 32) & 0xffff);
-        const int off = (int)(d >> 48);
-        double r = eval_dial<R>(type, n, s_rec + off, xk[in], a.in_flags[in], tk);
-        // DialResponseSupervisor::process; unset bounds are stored as -+inf.
-        const double lo = a.in_min[in], hi = a.in_max[in];
-        if (r < lo) r = lo;
-        else if (r > hi) r = hi;
-        s_resp[dest] = r;
-      }
-      __syncthreads();
-      if (threadIdx.x < kWave) {
-        const int lane = threadIdx.x;
-        double w = 0.0, w2 = 0.0, c = 0.0, c2 = 0.0;
-        int bin = INT_MAX;
-        if (lane < h.n_events) {
-          const int64_t e = (int64_t)h.first_event + lane;
-          const uint32_t ed = a.ev_dial[e];
-          const int b0 = (int)(ed & 0xffff), cnt = (int)(ed >> 16);
-          double temp = 1;
-          for (int j = 0; j < cnt; ++j) temp *= s_resp[b0 + j];
-          if (temp > a.cap) temp = a.cap;
-          w = a.ev_base[e];
-          w *= temp;
-          bin = a.ev_bin[e];
-          if (k == 0 && a.weights_out) a.weights_out[a.ev_result[e]] = w;
-          if (isnan(w)) atomicOr(a.flags, 1);
-          w2 = w * w;
+    dma_to_lds(rec + h.rec_begin * sizeof(R), s_buf[0], (int)(h.rec_words * sizeof(R)), lane);
+    dma_to_lds(desc + (size_t)h.first_dial * 8, s_buf[0] + a.rec_bytes, h.n_dials * 8, lane);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  }
+  lds_barrier();
+
+  int b = 0;      // LDS buffer of the current tile
+  int step = 0;   // response buffer parity
+  for (; t < a.n_tiles; t += gridDim.x, b ^= 1) {
+    TileHdr h;
+    if (!producer) h = a.tiles[t];
+    for (int k = 0; k < a.k_points; ++k, ++step) {
+      double* resp = s_resp[step & 1];
+      if (producer) {
+        const int64_t tn = t + gridDim.x;
+        if (k == 0 && tn < a.n_tiles) {
+          const TileHdr hn = a.tiles[tn];
+          dma_to_lds(rec + hn.rec_begin * sizeof(R), s_buf[b ^ 1],
+                     (int)(hn.rec_words * sizeof(R)), lane);
+          dma_to_lds(desc + (size_t)hn.first_dial * 8, s_buf[b ^ 1] + a.rec_bytes,
+                     hn.n_dials * 8, lane);
         }
-        // Inclusive segmented scan over the wave; bins are sorted, so equal
-        // bins are one contiguous segment.
+        if (k == a.k_points - 1) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        lds_barrier();
+        continue;
+      }
+      // ---- consumers: evaluate every dial of the tile for point k ----
+      {
+        const R* s_rec = reinterpret_cast<const R*>(s_buf[b]);
+        const uint64_t* s_desc = reinterpret_cast<const uint64_t*>(s_buf[b] + a.rec_bytes);
+        const InputRec* inp = kInputsLds ? s_in + (size_t)k * a.n_inputs
+                                         : a.xin + (size_t)k * a.n_inputs;
+        const double* tk = a.tab + (size_t)k * a.table_len;
+        for (int i = threadIdx.x; i < h.n_dials; i += cons_threads) {
+          const uint64_t d = s_desc[i];
+          const int in = (int)(d & 0xffff);
+          const int type = (int)((d >> 16) & 0xf);
+          const int n = (int)((d >> 20) & 0xff);
+          const int dest = (int)((d >> 32) & 0xffff);
+          const int off = (int)(d >> 48);
+          const InputRec ir = inp[in];
+          double r = eval_dial<R>(type, n, s_rec + off, ir.x, ir.flags, tk);
+          // DialResponseSupervisor::process (DialResponseSupervisor.cpp:10-18)
+          if (r < ir.lo) r = ir.lo;
+          else if (r > ir.hi) r = ir.hi;
+          resp[dest] = r;
+        }
+      }
+      lds_barrier();
+      if (wave != 0) continue;
+      // ---- consumer wave 0: event weights and bin partials ----
+      double w = 0.0, w2 = 0.0, c = 0.0, c2 = 0.0;
+      int bin = INT_MAX;
+      if (lane < h.n_events) {
+        const int64_t e = (int64_t)h.first_event + lane;
+        const uint32_t ed = a.ev_dial[e];
+        const int b0 = (int)(ed & 0xffff), cnt = (int)(ed >> 16);
+        double temp = 1;  // EventDialCache.cpp:159-170
+        for (int j = 0; j < cnt; ++j) temp *= resp[b0 + j];
+        if (temp > a.cap) temp = a.cap;
+        w = a.ev_base[e];
+        w *= temp;
+        bin = a.ev_bin[e];
+        if (k == 0 && a.weights_out) a.weights_out[a.ev_result[e]] = w;
+        if (isnan(w)) atomicOr(a.flags, 1);
+        w2 = w * w;
+      }
+      // Inclusive segmented scan over the wave; bins are sorted, so equal
+      // bins are one contiguous segment.
 #pragma unroll
-        for (int dlt = 1; dlt < kWave; dlt <<= 1) {
-          const double ws = __shfl_up(w, dlt);
-          const double w2s = __shfl_up(w2, dlt);
-          const int bs = __shfl_up(bin, dlt);
-          if (kComp) {
-            const double cs = __shfl_up(c, dlt);
-            const double c2s = __shfl_up(c2, dlt);
-            if (lane >= dlt && bs == bin) {
-              double s, e;
-              two_sum(ws, w, s, e);
-              w = s;
-              c = c + cs + e;
-              two_sum(w2s, w2, s, e);
-              w2 = s;
-              c2 = c2 + c2s + e;
-            }
-          } else {
-            if (lane >= dlt && bs == bin) {
-              w = ws + w;
-              w2 = w2s + w2;
-            }
+      for (int dlt = 1; dlt < kWave; dlt <<= 1) {
+        const double ws = __shfl_up(w, dlt);
+        const double w2s = __shfl_up(w2, dlt);
+        const int bs = __shfl_up(bin, dlt);
+        if (kComp) {
+          const double cs = __shfl_up(c, dlt);
+          const double c2s = __shfl_up(c2, dlt);
+          if (lane >= dlt && bs == bin) {
+            double sum, e;
+            two_sum(ws, w, sum, e);
+            w = sum;
+            c = c + cs + e;
+            two_sum(w2s, w2, sum, e);
+            w2 = sum;
+            c2 = c2 + c2s + e;
           }
-        }
-        const int prev = __shfl_up(bin, 1);
-        const int next = __shfl_down(bin, 1);
-        const bool valid = lane < h.n_events;
-        const bool head = valid && (lane == 0 || prev != bin);
-        const bool tail = valid && (lane == h.n_events - 1 || next != bin);
-        const unsigned long long heads = __ballot(head);
-        const int seg = __popcll(heads & ((2ull << lane) - 1ull)) - 1;
-        if (tail) {
-          const size_t slot = (size_t)k * a.n_partials + h.first_partial + seg;
-          a.psum[slot] = w;
-          a.psum2[slot] = w2;
-          if (kComp) {
-            a.pcomp[slot] = c;
-            a.pcomp2[slot] = c2;
+        } else {
+          if (lane >= dlt && bs == bin) {
+            w = ws + w;
+            w2 = w2s + w2;
           }
         }
       }
-      __syncthreads();
+      const int prev = __shfl_up(bin, 1);
+      const int next = __shfl_down(bin, 1);
+      const bool valid = lane < h.n_events;
+      const bool head = valid && (lane == 0 || prev != bin);
+      const bool tail = valid && (lane == h.n_events - 1 || next != bin);
+      const unsigned long long heads = __ballot(head);
+      const int seg = __popcll(heads & ((2ull << lane) - 1ull)) - 1;
+      if (tail) {
+        const size_t slot = (size_t)k * a.n_partials + h.first_partial + seg;
+        a.psum[slot] = w;
+        a.psum2[slot] = w2;
+        if (kComp) {
+          a.pcomp[slot] = c;
+          a.pcomp2[slot] = c2;
+        }
+      }
     }
   }
 }
@@ -515,7 +593,8 @@ struct gcm_context {
   int32_t *d_in_par = nullptr, *d_in_flags = nullptr;
   double *d_in_mlo = nullptr, *d_in_mrange = nullptr, *d_in_min = nullptr,
          *d_in_max = nullptr;
-  double *d_theta = nullptr, *d_xin = nullptr, *d_tab = nullptr;
+  double *d_theta = nullptr, *d_tab = nullptr;
+  InputRec* d_xin = nullptr;
   double *d_data = nullptr, *d_prior_mean = nullptr, *d_prior_sigma = nullptr;
   double *d_hist = nullptr, *d_hist2 = nullptr;
   double *d_llh_part = nullptr, *d_stat = nullptr, *d_pen = nullptr,
@@ -791,6 +870,9 @@ int build_layout(gcm_context* c, const gcm_tables* t) {
   c->n_dials = t->n_dials;
   c->n_tiles = T;
   c->n_partials = np;
+  // the producer wave copies whole KiB chunks: pad both streams by 1 KiB
+  desc.resize(desc.size() + 1024 / sizeof(uint64_t), 0);
+  rec.resize(rec.size() + 1024 / sizeof(R), R(0));
   GCM_TRY(upload(c, &c->d_tiles, tiles.data(), tiles.size()));
   GCM_TRY(upload(c, &c->d_desc, desc.data(), desc.size()));
   {
@@ -881,8 +963,8 @@ int gcm_set_launch_config(gcm_context* c, const gcm_launch_config* cfg) {
   if (cfg->blocks_per_cu) l.blocks_per_cu = cfg->blocks_per_cu;
   if (l.block_threads < 64 || l.block_threads > 512 || l.block_threads % 64)
     return fail(GCM_ERR_ARG, "block_threads must be a multiple of 64 in [64, 512]");
-  if (l.tile_words < 256 || l.tile_words > 16384 || l.tile_words % 64)
-    return fail(GCM_ERR_ARG, "tile_words must be a multiple of 64 in [256, 16384]");
+  if (l.tile_words < 256 || l.tile_words > 8192 || l.tile_words % 128)
+    return fail(GCM_ERR_ARG, "tile_words must be a multiple of 128 in [256, 8192]");
   if (l.blocks_per_cu < 1 || l.blocks_per_cu > 32)
     return fail(GCM_ERR_ARG, "blocks_per_cu must be in [1, 32]");
   c->launch = l;
@@ -923,7 +1005,9 @@ int gcm_build(gcm_context* c, const gcm_tables* t, int32_t max_points) {
   c->cap = std::isfinite(t->reweight_cap) ? t->reweight_cap : __builtin_huge_val();
   // dial slots per tile: enough for the densest events (norm dials have no
   // record words), bounded so that descriptors + responses stay small in LDS
-  c->max_dials = std::min(1024, std::max(256, c->launch.tile_words / 4));
+  // (a multiple of 128, so the descriptor buffer is whole KiB: the producer
+  // wave's DMA writes whole KiB chunks)
+  c->max_dials = std::min(1024, std::max(256, (c->launch.tile_words / 8 + 127) / 128 * 128));
   int rc = (c->precision == GCM_PRECISION_FP32) ? build_layout<float>(c, t)
                                                  : build_layout<double>(c, t);
   if (rc != GCM_OK) {
@@ -1106,7 +1190,8 @@ int gcm_fill(gcm_context* c, void* stream_ptr) {
   }
   GCM_HIP(hipEventRecord(c->ev_staged, s));
   prologue_kernel<<<K, 256, 0, s>>>(c->d_theta, c->n_par, c->d_in_par, c->d_in_mlo,
-                                    c->d_in_mrange, c->n_inputs, c->d_xin,
+                                    c->d_in_mrange, c->d_in_min, c->d_in_max,
+                                    c->d_in_flags, c->n_inputs, c->d_xin,
                                     c->llh_set ? c->d_prior_mean : nullptr,
                                     c->llh_set ? c->d_prior_sigma : nullptr, c->d_penalty);
   GCM_HIP(hipGetLastError());
@@ -1122,9 +1207,6 @@ int gcm_fill(gcm_context* c, void* stream_ptr) {
   a.ev_result = c->d_ev_result;
   a.xin = c->d_xin;
   a.n_inputs = c->n_inputs;
-  a.in_flags = c->d_in_flags;
-  a.in_min = c->d_in_min;
-  a.in_max = c->d_in_max;
   a.tab = c->d_tab;
   a.table_len = c->table_len;
   a.cap = c->cap;
@@ -1136,10 +1218,13 @@ int gcm_fill(gcm_context* c, void* stream_ptr) {
   a.n_partials = c->n_partials;
   a.weights_out = c->weights_enabled ? c->d_weights : nullptr;
   a.flags = c->d_flags;
-  const int rec_size = fp32 ? 4 : 8;
-  a.tile_words = c->launch.tile_words * 8 / rec_size;
+  a.rec_bytes = c->launch.tile_words * 8;
   a.max_dials = c->max_dials;
-  const size_t lds = (size_t)c->launch.tile_words * 8 + (size_t)c->max_dials * 16;
+  const size_t in_bytes = (size_t)K * c->n_inputs * sizeof(InputRec);
+  a.inputs_lds = in_bytes <= kInputsLdsMax;
+  const size_t in_pad = a.inputs_lds ? (in_bytes + 1023) / 1024 * 1024 : 0;
+  const size_t lds = in_pad + 2 * ((size_t)a.rec_bytes + (size_t)a.max_dials * 8) +
+                     2 * (size_t)a.max_dials * 8;
   hipEvent_t* tev = nullptr;
   if (c->timing) {
     if (c->t_issued - c->t_done >= gcm_context::kTimingRing) GCM_TRY(harvest_timing(c));
@@ -1149,18 +1234,21 @@ int gcm_fill(gcm_context* c, void* stream_ptr) {
   if (c->n_tiles > 0) {
     const int64_t want = (int64_t)c->launch.blocks_per_cu * c->n_cu;
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(c->n_tiles, want));
-    if (lds > 65536) {
-      if (fp32)
-        GCM_HIP(hipFuncSetAttribute((const void*)fill_kernel<float, true>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-      else
-        GCM_HIP(hipFuncSetAttribute((const void*)fill_kernel<double, false>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    const int threads = c->launch.block_threads + kWave;  // + the producer wave
+    const void* fn;
+    if (fp32) fn = a.inputs_lds ? (const void*)fill_kernel<float, true, true>
+                                : (const void*)fill_kernel<float, true, false>;
+    else fn = a.inputs_lds ? (const void*)fill_kernel<double, false, true>
+                           : (const void*)fill_kernel<double, false, false>;
+    if (lds > 65536)
+      GCM_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    if (fp32) {
+      if (a.inputs_lds) fill_kernel<float, true, true><<<grid, threads, lds, s>>>(a);
+      else fill_kernel<float, true, false><<<grid, threads, lds, s>>>(a);
+    } else {
+      if (a.inputs_lds) fill_kernel<double, false, true><<<grid, threads, lds, s>>>(a);
+      else fill_kernel<double, false, false><<<grid, threads, lds, s>>>(a);
     }
-    if (fp32)
-      fill_kernel<float, true><<<grid, c->launch.block_threads, lds, s>>>(a);
-    else
-      fill_kernel<double, false><<<grid, c->launch.block_threads, lds, s>>>(a);
     GCM_HIP(hipGetLastError());
   }
   if (tev) GCM_HIP(hipEventRecord(tev[1], s));
