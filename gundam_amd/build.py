@@ -46,9 +46,20 @@ def build_synth() -> str:
     return out
 
 
+def build_hbm_copy() -> str:
+    out = os.path.join(HERE, "hbm_copy")
+    tmp = out + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang",
+          "-target-feature", "-Xclang", "-packed-fp32-ops", "-o", tmp,
+          os.path.join(CSRC, "hbm_copy.hip")])
+    os.replace(tmp, out)
+    return out
+
+
 def build_all() -> None:
     build_synth()
     build_gcm()
+    build_hbm_copy()
 
 
 if __name__ == "__main__":
