@@ -98,6 +98,7 @@ def main():
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--tile-words", type=int, default=0)
     ap.add_argument("--blocks-per-cu", type=int, default=0)
+    ap.add_argument("--ring", type=int, default=0, help="LDS tile buffers (2 or 3)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tune-events", type=int, default=500000)
@@ -134,15 +135,16 @@ def main():
         tuned = autotune(t, tp, wl.table_values(tp) if t.table_len else None,
                          precision=args.precision, n_events=args.tune_events,
                          reps=3, log=log)
-        args.block, args.tile_words, args.blocks_per_cu = (
-            tuned["block_threads"], tuned["tile_words"], tuned["blocks_per_cu"])
+        args.block, args.tile_words, args.blocks_per_cu, args.ring = (
+            tuned["block_threads"], tuned["tile_words"], tuned["blocks_per_cu"],
+            tuned["ring_buffers"])
         log(f"autotuned in {tuned['tune_seconds']:.1f}s over {tuned['candidates']} "
             f"shapes: {args.tile_words} words/tile, {args.block}+64 threads, "
-            f"{args.blocks_per_cu} blocks/CU")
+            f"{args.blocks_per_cu} blocks/CU, {args.ring} buffers")
     t0 = time.time()
     cm = CacheManager(0, precision=args.precision, block_threads=args.block,
                       tile_words=args.tile_words,
-                      blocks_per_cu=args.blocks_per_cu)
+                      blocks_per_cu=args.blocks_per_cu, ring_buffers=args.ring)
     cm.build(t, max_points=K)
     counts = cm.counts()
     log(f"built in {time.time()-t0:.1f}s: {counts}, resident "

@@ -69,7 +69,7 @@ class GcmLlhConfig(C.Structure):
 
 class GcmLaunchConfig(C.Structure):
     _fields_ = [("block_threads", C.c_int32), ("tile_words", C.c_int32),
-                ("blocks_per_cu", C.c_int32), ("reserved", C.c_int32)]
+                ("blocks_per_cu", C.c_int32), ("ring_buffers", C.c_int32)]
 
 
 class GcmSynthView(C.Structure):

@@ -19,12 +19,14 @@ from .tables import Tables
 LDS_BYTES_PER_CU = 160 * 1024
 
 
-def lds_per_block(tile_words: int, n_inputs: int, k_points: int = 1) -> int:
+def lds_per_block(tile_words: int, n_inputs: int, k_points: int = 1,
+                  ring_buffers: int = 2) -> int:
     """LDS of one fill block (mirrors the host computation in gcm.hip)."""
     max_dials = min(1024, max(256, (tile_words // 8 + 127) // 128 * 128))
     in_bytes = k_points * n_inputs * 32
     in_pad = (in_bytes + 1023) // 1024 * 1024 if in_bytes <= 16384 else 0
-    return in_pad + 2 * (tile_words * 8 + max_dials * 8) + 2 * max_dials * 8
+    return (in_pad + ring_buffers * (tile_words * 8 + max_dials * 8 + 64 * 16 + 32)
+            + 2 * max_dials * 8)
 
 
 def head(t: Tables, n_events: int) -> Tables:
@@ -42,14 +44,17 @@ def head(t: Tables, n_events: int) -> Tables:
 
 def candidates(n_inputs: int, precision: str = "fp64") -> list:
     out = []
-    for tile, block, bpc in itertools.product((1024, 2048, 3072, 4096),
-                                              (128, 192, 256, 384),
-                                              (1, 2, 3, 4, 5, 6, 8)):
-        if lds_per_block(tile, n_inputs) * bpc > LDS_BYTES_PER_CU:
+    for nb, tile, block, bpc in itertools.product((2, 3), (1024, 1536, 2048, 2560, 3072, 4096),
+                                                  (128, 256, 384, 512, 768),
+                                                  (1, 2, 3, 4, 5, 6)):
+        lds = lds_per_block(tile, n_inputs, ring_buffers=nb)
+        # fill the CU's LDS: only the largest block count that fits
+        if lds * bpc > LDS_BYTES_PER_CU or lds * (bpc + 1) <= LDS_BYTES_PER_CU:
             continue
-        if (block + 64) * bpc > 2048:  # 32 waves per CU
+        if (block + 64 * nb) * bpc > 2048:  # 32 waves per CU
             continue
-        out.append({"tile_words": tile, "block_threads": block, "blocks_per_cu": bpc})
+        out.append({"tile_words": tile, "block_threads": block, "blocks_per_cu": bpc,
+                    "ring_buffers": nb})
     return out
 
 

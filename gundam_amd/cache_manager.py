@@ -45,7 +45,7 @@ class Future:
 class CacheManager:
     def __init__(self, device: int = 0, precision: str = "fp64",
                  block_threads: int = 0, tile_words: int = 0,
-                 blocks_per_cu: int = 0):
+                 blocks_per_cu: int = 0, ring_buffers: int = 0):
         self._lib = N.lib()
         h = C.c_void_p()
         N.check(self._lib.gcm_create(int(device), C.byref(h)))
@@ -54,8 +54,8 @@ class CacheManager:
         prec = {"fp64": N.PRECISION_FP64, "fp32": N.PRECISION_FP32}[precision]
         N.check(self._lib.gcm_set_precision(self._h, prec))
         self.precision = precision
-        if block_threads or tile_words or blocks_per_cu:
-            self.set_launch(block_threads, tile_words, blocks_per_cu)
+        if block_threads or tile_words or blocks_per_cu or ring_buffers:
+            self.set_launch(block_threads, tile_words, blocks_per_cu, ring_buffers)
         self.n_bins = 0
         self.n_parameters = 0
         self.n_events = 0
@@ -64,16 +64,17 @@ class CacheManager:
         self.max_points = 1
 
     # -- configuration ----------------------------------------------------
-    def set_launch(self, block_threads=0, tile_words=0, blocks_per_cu=0):
+    def set_launch(self, block_threads=0, tile_words=0, blocks_per_cu=0,
+                   ring_buffers=0):
         cfg = N.GcmLaunchConfig(int(block_threads), int(tile_words),
-                                int(blocks_per_cu), 0)
+                                int(blocks_per_cu), int(ring_buffers))
         N.check(self._lib.gcm_set_launch_config(self._h, C.byref(cfg)))
 
     def launch_config(self) -> dict:
         cfg = N.GcmLaunchConfig()
         N.check(self._lib.gcm_get_launch_config(self._h, C.byref(cfg)))
         return {"block_threads": cfg.block_threads, "tile_words": cfg.tile_words,
-                "blocks_per_cu": cfg.blocks_per_cu}
+                "blocks_per_cu": cfg.blocks_per_cu, "ring_buffers": cfg.ring_buffers}
 
     # -- Build --------------------------------------------------------------
     def build(self, tables: Tables, max_points: int = 1) -> None:

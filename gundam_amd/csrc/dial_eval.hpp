@@ -220,24 +220,30 @@ __device__ __forceinline__ double eval_dial(int type, int n, const R* d,
       return (double)(f * (float)tab[slot] + (1.0f - f) * (float)tab[slot + 1]);
     }
     case GCM_DIAL_KRIGED: {  // Kriged.cpp:14-21
-      if (sizeof(R) == 8) {
-        double v = 0.0;
-        for (int j = 0; j < n; ++j) {
-          uint32_t slot;
-          float w;
-          TableEntry<R>::get(d, j, &slot, &w);
-          v += (double)w * tab[slot];
+      // The table gathers are issued eight at a time (they hit L2 or HBM);
+      // the sum is still accumulated in entry order.
+      double v = 0.0;
+      float vf = 0.0f;
+      for (int j0 = 0; j0 < n; j0 += 8) {
+        double tv[8];
+        float wv[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          uint32_t slot = 0;
+          float w = 0.0f;
+          if (j0 + q < n) TableEntry<R>::get(d, j0 + q, &slot, &w);
+          wv[q] = w;
+          tv[q] = (j0 + q < n) ? tab[slot] : 0.0;
         }
-        return v;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          if (j0 + q < n) {
+            if (sizeof(R) == 8) v += (double)wv[q] * tv[q];
+            else vf += wv[q] * (float)tv[q];
+          }
+        }
       }
-      float v = 0.0f;
-      for (int j = 0; j < n; ++j) {
-        uint32_t slot;
-        float w;
-        TableEntry<R>::get(d, j, &slot, &w);
-        v += w * (float)tab[slot];
-      }
-      return (double)v;
+      return sizeof(R) == 8 ? v : (double)vf;
     }
     default:
       return __builtin_nan("");

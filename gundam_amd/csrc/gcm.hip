@@ -48,7 +48,18 @@ constexpr int kDefaultBlock = 256;
 constexpr int kDefaultTileWords = 2048;  // 16 KiB of records per tile
 constexpr int kDefaultBlocksPerCU = 3;
 constexpr int kFinalizeBlock = 256;
+constexpr int kFinalizeMaxBlocks = 128;
 constexpr size_t kInputsLdsMax = 16384;  // stage input records in LDS up to this
+constexpr size_t kHeaderBytes = 32;      // tile header copy in each LDS buffer
+
+// One binned event, in bin-sorted order: base weight, global bin, and its
+// dials as (first event-major slot in the tile, count).
+struct EventRec {
+  double base;
+  int32_t bin;
+  uint32_t dials;  // slot | count << 16
+};
+static_assert(sizeof(EventRec) == 16, "event record is 16 bytes");
 
 struct TileHdr {
   uint32_t first_event;
@@ -58,7 +69,7 @@ struct TileHdr {
   uint16_t n_events;
   uint16_t n_dials;
   uint32_t first_partial;
-  uint32_t max_dials_unused;
+  uint32_t pad;
 };
 static_assert(sizeof(TileHdr) == 32, "tile header is 32 bytes");
 
@@ -91,7 +102,10 @@ struct InputRec {
 };
 static_assert(sizeof(InputRec) == 32, "input record is 32 bytes");
 
-// Per point k (one block each): mirrored input values and the prior penalty.
+constexpr int kPenaltyChunk = 256;
+
+// Per point k (one block each, 256 threads): mirrored input values and the
+// prior penalty.
 __global__ void prologue_kernel(const double* __restrict__ theta, int n_par,
                                 const int32_t* __restrict__ in_par,
                                 const double* __restrict__ in_mlo,
@@ -125,20 +139,31 @@ __global__ void prologue_kernel(const double* __restrict__ theta, int n_par,
     r.pad = 0;
     xin[(size_t)k * n_inputs + i] = r;
   }
-  if (threadIdx.x == 0) {
-    // LikelihoodInterface.cpp:186-206 with a diagonal covariance, summed in
-    // parameter order.
-    double pen = 0;
-    if (prior_sigma) {
-      for (int p = 0; p < n_par; ++p) {
+  // LikelihoodInterface.cpp:186-206 with a diagonal covariance: the terms are
+  // formed in parallel and summed in parameter order by one thread.
+  __shared__ double s_z[kPenaltyChunk];
+  double pen = 0;
+  if (prior_sigma) {
+    for (int p0 = 0; p0 < n_par; p0 += kPenaltyChunk) {
+      const int p = p0 + threadIdx.x;
+      if (threadIdx.x < kPenaltyChunk && p < n_par) {
         const double s = prior_sigma[p];
-        if (!(s > 0)) continue;
-        const double z = (th[p] - prior_mean[p]) / s;
-        pen += z * z;
+        double z2 = 0.0;  // no term: adding +0.0 leaves the sum unchanged
+        if (s > 0) {
+          const double z = (th[p] - prior_mean[p]) / s;
+          z2 = z * z;
+        }
+        s_z[threadIdx.x] = z2;
       }
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        const int n = min(kPenaltyChunk, n_par - p0);
+        for (int j = 0; j < n; ++j) pen += s_z[j];
+      }
+      __syncthreads();
     }
-    penalty[k] = pen;
   }
+  if (threadIdx.x == 0) penalty[k] = pen;
 }
 
 // Knuth two-sum: s + e == a + b exactly.
@@ -154,9 +179,7 @@ struct FillArgs {
   int64_t n_tiles;
   const uint64_t* desc;
   const void* rec;
-  const double* ev_base;
-  const int32_t* ev_bin;
-  const uint32_t* ev_dial;
+  const EventRec* ev;
   const int32_t* ev_result;
   const InputRec* xin;
   int n_inputs;
@@ -171,10 +194,17 @@ struct FillArgs {
   int64_t n_partials;
   double* weights_out;
   int* flags;
-  int rec_bytes;   // LDS record capacity per buffer (multiple of 1 KiB)
+  int rec_bytes;   // LDS record capacity per buffer (multiple of 16 bytes)
   int max_dials;   // LDS descriptor/response capacity (multiple of 128)
   int inputs_lds;  // input records staged in LDS (K * n_inputs * 32 bytes)
 };
+
+// Cache policy of the tile stream: non-temporal, so the streamed records do
+// not evict the per-point lookup tables and input records from L2.
+#ifndef GCM_STREAM_POLICY
+#define GCM_STREAM_POLICY 2
+#endif
+constexpr int kStreamPolicy = GCM_STREAM_POLICY;
 
 typedef __attribute__((address_space(3))) void* lds_void_ptr;
 typedef __attribute__((address_space(1))) void* global_void_ptr;
@@ -185,86 +215,211 @@ __device__ __forceinline__ void lds_barrier() {
   asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-// Producer wave: copy `bytes` (rounded up to 1 KiB; the sources are padded)
-// from global memory straight into LDS with 16-byte-per-lane DMA loads.
+// Producer wave: copy `bytes` (a multiple of 16) from global memory straight
+// into LDS with 16-byte-per-lane DMA loads; lane i of a chunk lands at
+// dst + chunk + 16 i, and lanes past the end are masked off.
 __device__ __forceinline__ void dma_to_lds(const unsigned char* src,
                                            unsigned char* dst, int bytes,
                                            int lane) {
   for (int j = 0; j < bytes; j += 1024) {
-    __builtin_amdgcn_global_load_lds((global_void_ptr)(src + j + lane * 16),
-                                     (lds_void_ptr)(dst + j), 16, 0, 0);
+    if (j + lane * 16 < bytes)
+      __builtin_amdgcn_global_load_lds((global_void_ptr)(src + j + lane * 16),
+                                       (lds_void_ptr)(dst + j), 16, 0, kStreamPolicy);
   }
 }
 
-// The fill kernel.  Waves 0..C-1 are consumers, the last wave is the
-// producer.  Step s covers (tile, point); the producer prefetches the next
-// tile into the other LDS buffer while the consumers evaluate the current
-// one, and one barrier per step hands both the records and the responses
-// over.  Responses are double-buffered so that consumer wave 0 can run the
-// event phase of step s while the others evaluate step s+1.
-template <typename R, bool kComp, bool kInputsLds>
-__global__ void __launch_bounds__(576) fill_kernel(FillArgs a) {
+template <typename R>
+__device__ __forceinline__ void dma_tile(const TileHdr& h, const unsigned char* rec,
+                                         const unsigned char* desc,
+                                         const EventRec* ev, unsigned char* buf,
+                                         int rec_bytes, int max_dials, int lane) {
+  dma_to_lds(rec + h.rec_begin * sizeof(R), buf,
+             (int)((h.rec_words * sizeof(R) + 15) / 16 * 16), lane);
+  dma_to_lds(desc + (size_t)h.first_dial * 8, buf + rec_bytes,
+             (h.n_dials + 1) / 2 * 16, lane);
+  dma_to_lds(reinterpret_cast<const unsigned char*>(ev + h.first_event),
+             buf + rec_bytes + (size_t)max_dials * 8, h.n_events * 16, lane);
+}
+
+// The tile header travels with the tile (evaluators and the event wave read
+// it from LDS instead of waiting on a global load after each barrier).
+__device__ __forceinline__ void dma_header(const TileHdr* hdr, unsigned char* buf,
+                                           size_t hdr_off, int lane) {
+  dma_to_lds(reinterpret_cast<const unsigned char*>(hdr), buf + hdr_off,
+             (int)sizeof(TileHdr), lane);
+}
+
+// The fill kernel.  Wave roles in a block of C + NB waves:
+//   waves [0, C)   evaluate every dial of the current tile (type-sorted
+//                  records, one dial per lane) into the step's response buffer;
+//   wave C         forms the event weights and the bin partials of the
+//                  previous step while the evaluators work on the next one;
+//   waves C+1 ..   NB-1 producers: producer p streams the tiles whose ordinal
+//                  is p mod (NB-1) HBM -> LDS with DMA loads, NB-1 tiles
+//                  ahead, into a ring of NB buffers.
+// A step is one (tile, point); every wave meets one barrier per step.
+template <typename R, bool kComp, bool kInputsLds, int NB>
+__global__ void __launch_bounds__(1024, 6) fill_kernel(FillArgs a) {
+  constexpr int P = NB - 1;  // producer waves
   extern __shared__ __align__(16) unsigned char lds[];
   const int nwaves = blockDim.x / kWave;
-  const int wave = threadIdx.x / kWave;
+  // wave-uniform role index in a scalar register, so that the role branches
+  // are scalar branches rather than exec-masked divergent code
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int lane = threadIdx.x % kWave;
-  const bool producer = wave == nwaves - 1;
-  const int cons_threads = (nwaves - 1) * kWave;
+  const int n_eval = nwaves - 1 - P;
+  const int eval_threads = n_eval * kWave;
+  const int ev_wave = n_eval;
+  const int prod0 = n_eval + 1;
   const size_t in_bytes = kInputsLds ? (size_t)a.k_points * a.n_inputs * sizeof(InputRec) : 0;
   const size_t in_pad = (in_bytes + 1023) / 1024 * 1024;
   InputRec* s_in = reinterpret_cast<InputRec*>(lds);
-  unsigned char* s_buf[2];
-  const size_t buf_bytes = (size_t)a.rec_bytes + (size_t)a.max_dials * 8;
-  s_buf[0] = lds + in_pad;
-  s_buf[1] = s_buf[0] + buf_bytes;
-  double* s_resp[2];
-  s_resp[0] = reinterpret_cast<double*>(s_buf[1] + buf_bytes);
-  s_resp[1] = s_resp[0] + a.max_dials;
+  const size_t hdr_off = (size_t)a.rec_bytes + (size_t)a.max_dials * 8 +
+                         kMaxTileEvents * sizeof(EventRec);
+  const size_t buf_bytes = hdr_off + kHeaderBytes;
+  unsigned char* const ring = lds + in_pad;  // NB tile buffers
+  double* const resp_base = reinterpret_cast<double*>(ring + NB * buf_bytes);
   const unsigned char* rec = reinterpret_cast<const unsigned char*>(a.rec);
   const unsigned char* desc = reinterpret_cast<const unsigned char*>(a.desc);
+  const int64_t G = gridDim.x;
+  const int64_t t0 = blockIdx.x;
 
   if (kInputsLds) {
     const int n = a.k_points * a.n_inputs;
     for (int i = threadIdx.x; i < n; i += blockDim.x) s_in[i] = a.xin[i];
   }
-  int64_t t = blockIdx.x;
-  if (producer && t < a.n_tiles) {
-    const TileHdr h = a.tiles[t];
-    dma_to_lds(rec + h.rec_begin * sizeof(R), s_buf[0], (int)(h.rec_words * sizeof(R)), lane);
-    dma_to_lds(desc + (size_t)h.first_dial * 8, s_buf[0] + a.rec_bytes, h.n_dials * 8, lane);
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  // producer p loads tile ordinal p into buffer p; tile 0 is waited for now,
+  // tile p > 0 by producer p at the end of tile p - 1.  Each producer keeps
+  // the header of its next tile in registers, fetched one DMA period ahead,
+  // so that no DMA waits on a header load.
+  TileHdr hq{};
+  if (wave >= prod0) {
+    const int p = wave - prod0;
+    const int64_t tp = t0 + p * G;
+    if (tp < a.n_tiles) {
+      dma_tile<R>(a.tiles[tp], rec, desc, a.ev, ring + (size_t)p * buf_bytes,
+                  a.rec_bytes, a.max_dials, lane);
+      dma_header(a.tiles + tp, ring + (size_t)p * buf_bytes, hdr_off, lane);
+    }
+    if (tp + P * G < a.n_tiles) hq = a.tiles[tp + P * G];
+    if (p == 0) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   }
   lds_barrier();
 
-  int b = 0;      // LDS buffer of the current tile
-  int step = 0;   // response buffer parity
-  for (; t < a.n_tiles; t += gridDim.x, b ^= 1) {
+  int step = 0;
+  int64_t i = 0;  // ordinal of the tile within this block
+  for (int64_t t = t0; t < a.n_tiles; t += G, ++i) {
+    unsigned char* cur = ring + (size_t)(i % NB) * buf_bytes;
     TileHdr h;
-    if (!producer) h = a.tiles[t];
+    if (wave <= ev_wave) h = *reinterpret_cast<const TileHdr*>(cur + hdr_off);
     for (int k = 0; k < a.k_points; ++k, ++step) {
-      double* resp = s_resp[step & 1];
-      if (producer) {
-        const int64_t tn = t + gridDim.x;
-        if (k == 0 && tn < a.n_tiles) {
-          const TileHdr hn = a.tiles[tn];
-          dma_to_lds(rec + hn.rec_begin * sizeof(R), s_buf[b ^ 1],
-                     (int)(hn.rec_words * sizeof(R)), lane);
-          dma_to_lds(desc + (size_t)hn.first_dial * 8, s_buf[b ^ 1] + a.rec_bytes,
-                     hn.n_dials * 8, lane);
+      double* resp = resp_base + (size_t)(step & 1) * a.max_dials;
+      if (wave >= prod0) {
+        // at the first step of tile i, producer i mod P loads tile i + P into
+        // the buffer of tile i - 1; at its last step, producer (i + 1) mod P
+        // waits for tile i + 1
+        const int me = wave - prod0;
+        if (k == 0 && me == (int)(i % P)) {
+          const int64_t tn = t + P * G;
+          if (tn < a.n_tiles) {
+            unsigned char* nbuf = ring + (size_t)((i + P) % NB) * buf_bytes;
+            dma_tile<R>(hq, rec, desc, a.ev, nbuf, a.rec_bytes, a.max_dials, lane);
+            dma_header(a.tiles + tn, nbuf, hdr_off, lane);
+            if (tn + P * G < a.n_tiles) hq = a.tiles[tn + P * G];
+          }
         }
-        if (k == a.k_points - 1) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        if (k == a.k_points - 1 && me == (int)((i + 1) % P))
+          asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         lds_barrier();
         continue;
       }
-      // ---- consumers: evaluate every dial of the tile for point k ----
+      if (wave == ev_wave) {
+        // The event record is read before the barrier: after it, a producer
+        // may refill this buffer.
+        EventRec er{0.0, INT_MAX, 0u};
+        if (lane < h.n_events)
+          er = reinterpret_cast<const EventRec*>(cur + a.rec_bytes +
+                                                 (size_t)a.max_dials * 8)[lane];
+        lds_barrier();
+        double w = 0.0, w2 = 0.0, c = 0.0, c2 = 0.0;
+        int bin = INT_MAX;
+        if (lane < h.n_events) {
+          const int b0 = (int)(er.dials & 0xffff), cnt = (int)(er.dials >> 16);
+          // EventDialCache.cpp:159-170: the product in dial-list order; the
+          // loads are batched four at a time, the multiplies stay in order.
+          double temp = 1;
+          for (int j = 0; j < cnt; j += 4) {
+            const double r0 = resp[b0 + j];
+            const double r1 = (j + 1 < cnt) ? resp[b0 + j + 1] : 1.0;
+            const double r2 = (j + 2 < cnt) ? resp[b0 + j + 2] : 1.0;
+            const double r3 = (j + 3 < cnt) ? resp[b0 + j + 3] : 1.0;
+            temp *= r0;
+            if (j + 1 < cnt) temp *= r1;
+            if (j + 2 < cnt) temp *= r2;
+            if (j + 3 < cnt) temp *= r3;
+          }
+          if (temp > a.cap) temp = a.cap;
+          w = er.base;
+          w *= temp;
+          bin = er.bin;
+          if (k == 0 && a.weights_out)
+            a.weights_out[a.ev_result[(int64_t)h.first_event + lane]] = w;
+          if (isnan(w)) atomicOr(a.flags, 1);
+          w2 = w * w;
+        }
+        // Inclusive segmented scan over the wave; bins are sorted, so equal
+        // bins are one contiguous segment.
+#pragma unroll
+        for (int dlt = 1; dlt < kWave; dlt <<= 1) {
+          const double ws = __shfl_up(w, dlt);
+          const double w2s = __shfl_up(w2, dlt);
+          const int bs = __shfl_up(bin, dlt);
+          if (kComp) {
+            const double cs = __shfl_up(c, dlt);
+            const double c2s = __shfl_up(c2, dlt);
+            if (lane >= dlt && bs == bin) {
+              double sum, e;
+              two_sum(ws, w, sum, e);
+              w = sum;
+              c = c + cs + e;
+              two_sum(w2s, w2, sum, e);
+              w2 = sum;
+              c2 = c2 + c2s + e;
+            }
+          } else {
+            if (lane >= dlt && bs == bin) {
+              w = ws + w;
+              w2 = w2s + w2;
+            }
+          }
+        }
+        const int prev = __shfl_up(bin, 1);
+        const int next = __shfl_down(bin, 1);
+        const bool valid = lane < h.n_events;
+        const bool head = valid && (lane == 0 || prev != bin);
+        const bool tail = valid && (lane == h.n_events - 1 || next != bin);
+        const unsigned long long heads = __ballot(head);
+        const int seg = __popcll(heads & ((2ull << lane) - 1ull)) - 1;
+        if (tail) {
+          const size_t slot = (size_t)k * a.n_partials + h.first_partial + seg;
+          a.psum[slot] = w;
+          a.psum2[slot] = w2;
+          if (kComp) {
+            a.pcomp[slot] = c;
+            a.pcomp2[slot] = c2;
+          }
+        }
+        continue;
+      }
+      // ---- evaluators: every dial of the tile for point k ----
       {
-        const R* s_rec = reinterpret_cast<const R*>(s_buf[b]);
-        const uint64_t* s_desc = reinterpret_cast<const uint64_t*>(s_buf[b] + a.rec_bytes);
+        const R* s_rec = reinterpret_cast<const R*>(cur);
+        const uint64_t* s_desc = reinterpret_cast<const uint64_t*>(cur + a.rec_bytes);
         const InputRec* inp = kInputsLds ? s_in + (size_t)k * a.n_inputs
                                          : a.xin + (size_t)k * a.n_inputs;
         const double* tk = a.tab + (size_t)k * a.table_len;
-        for (int i = threadIdx.x; i < h.n_dials; i += cons_threads) {
-          const uint64_t d = s_desc[i];
+        for (int d0 = threadIdx.x; d0 < h.n_dials; d0 += eval_threads) {
+          const uint64_t d = s_desc[d0];
           const int in = (int)(d & 0xffff);
           const int type = (int)((d >> 16) & 0xf);
           const int n = (int)((d >> 20) & 0xff);
@@ -279,66 +434,6 @@ __global__ void __launch_bounds__(576) fill_kernel(FillArgs a) {
         }
       }
       lds_barrier();
-      if (wave != 0) continue;
-      // ---- consumer wave 0: event weights and bin partials ----
-      double w = 0.0, w2 = 0.0, c = 0.0, c2 = 0.0;
-      int bin = INT_MAX;
-      if (lane < h.n_events) {
-        const int64_t e = (int64_t)h.first_event + lane;
-        const uint32_t ed = a.ev_dial[e];
-        const int b0 = (int)(ed & 0xffff), cnt = (int)(ed >> 16);
-        double temp = 1;  // EventDialCache.cpp:159-170
-        for (int j = 0; j < cnt; ++j) temp *= resp[b0 + j];
-        if (temp > a.cap) temp = a.cap;
-        w = a.ev_base[e];
-        w *= temp;
-        bin = a.ev_bin[e];
-        if (k == 0 && a.weights_out) a.weights_out[a.ev_result[e]] = w;
-        if (isnan(w)) atomicOr(a.flags, 1);
-        w2 = w * w;
-      }
-      // Inclusive segmented scan over the wave; bins are sorted, so equal
-      // bins are one contiguous segment.
-#pragma unroll
-      for (int dlt = 1; dlt < kWave; dlt <<= 1) {
-        const double ws = __shfl_up(w, dlt);
-        const double w2s = __shfl_up(w2, dlt);
-        const int bs = __shfl_up(bin, dlt);
-        if (kComp) {
-          const double cs = __shfl_up(c, dlt);
-          const double c2s = __shfl_up(c2, dlt);
-          if (lane >= dlt && bs == bin) {
-            double sum, e;
-            two_sum(ws, w, sum, e);
-            w = sum;
-            c = c + cs + e;
-            two_sum(w2s, w2, sum, e);
-            w2 = sum;
-            c2 = c2 + c2s + e;
-          }
-        } else {
-          if (lane >= dlt && bs == bin) {
-            w = ws + w;
-            w2 = w2s + w2;
-          }
-        }
-      }
-      const int prev = __shfl_up(bin, 1);
-      const int next = __shfl_down(bin, 1);
-      const bool valid = lane < h.n_events;
-      const bool head = valid && (lane == 0 || prev != bin);
-      const bool tail = valid && (lane == h.n_events - 1 || next != bin);
-      const unsigned long long heads = __ballot(head);
-      const int seg = __popcll(heads & ((2ull << lane) - 1ull)) - 1;
-      if (tail) {
-        const size_t slot = (size_t)k * a.n_partials + h.first_partial + seg;
-        a.psum[slot] = w;
-        a.psum2[slot] = w2;
-        if (kComp) {
-          a.pcomp[slot] = c;
-          a.pcomp2[slot] = c2;
-        }
-      }
     }
   }
 }
@@ -429,14 +524,15 @@ __global__ void __launch_bounds__(kFinalizeBlock)
                     double* llh_part, unsigned* tickets,
                     const double* __restrict__ penalty, double* stat_out,
                     double* pen_out, double* total_out) {
-  constexpr int kBinsPerBlock = kFinalizeBlock / kWave;
+  constexpr int kWavesPerBlock = kFinalizeBlock / kWave;
   __shared__ double s_red[kFinalizeBlock];
   __shared__ int s_last;
   const int k = blockIdx.y;
   const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-  const int b = blockIdx.x * kBinsPerBlock + wave;
   const size_t kp = (size_t)k * n_partials;
-  if (b < n_bins) {
+  double wave_llh = 0.0;  // this wave's bins, summed in bin order
+  for (int b = blockIdx.x * kWavesPerBlock + wave; b < n_bins;
+       b += gridDim.x * kWavesPerBlock) {
     double s = 0.0, s2 = 0.0, c = 0.0, c2 = 0.0;
     const uint32_t p0 = bin_pbeg[b], p1 = bin_pbeg[b + 1];
     for (uint32_t p = p0 + lane; p < p1; p += kWave) {
@@ -462,15 +558,14 @@ __global__ void __launch_bounds__(kFinalizeBlock)
     if (lane == 0) {
       hist[(size_t)k * n_bins + b] = s;
       hist2[(size_t)k * n_bins + b] = s2;
-      s_red[wave] = data ? llh_bin(cfg, data[b], s, sqrt(s2)) : 0.0;
+      if (data) wave_llh += llh_bin(cfg, data[b], s, sqrt(s2));
     }
-  } else if (lane == 0) {
-    s_red[wave] = 0.0;
   }
+  if (lane == 0) s_red[wave] = wave_llh;
   __syncthreads();
   if (threadIdx.x == 0) {
     double part = 0.0;
-    for (int w = 0; w < kBinsPerBlock; ++w) part += s_red[w];
+    for (int w = 0; w < kWavesPerBlock; ++w) part += s_red[w];
     llh_part[(size_t)k * gridDim.x + blockIdx.x] = part;
     __threadfence();
     const unsigned ticket = atomicAdd(&tickets[k], 1u);
@@ -562,7 +657,7 @@ struct gcm_context {
   int device = 0;
   hipStream_t stream = nullptr;
   int precision = GCM_PRECISION_FP64;
-  gcm_launch_config launch{kDefaultBlock, kDefaultTileWords, kDefaultBlocksPerCU, 0};
+  gcm_launch_config launch{kDefaultBlock, kDefaultTileWords, kDefaultBlocksPerCU, 2};
   int n_cu = 256;
   bool built = false;
   bool weights_enabled = false;
@@ -583,9 +678,7 @@ struct gcm_context {
   TileHdr* d_tiles = nullptr;
   uint64_t* d_desc = nullptr;
   void* d_rec = nullptr;
-  double* d_ev_base = nullptr;
-  int32_t* d_ev_bin = nullptr;
-  uint32_t* d_ev_dial = nullptr;
+  EventRec* d_ev = nullptr;
   int32_t* d_ev_result = nullptr;
   uint32_t* d_bin_pbeg = nullptr;
   double *d_psum = nullptr, *d_psum2 = nullptr, *d_pcomp = nullptr,
@@ -787,9 +880,8 @@ int build_layout(gcm_context* c, const gcm_tables* t) {
   // fill records, descriptors and event arrays, tiles in parallel
   std::vector<R> rec((size_t)std::max<int64_t>(total_words, 1), R(0));
   std::vector<uint64_t> desc((size_t)std::max<int64_t>(total_desc, 1), 0);
-  std::vector<double> ev_base(NB);
-  std::vector<int32_t> ev_bin(NB), ev_result(NB);
-  std::vector<uint32_t> ev_dial(NB);
+  std::vector<EventRec> ev(NB);
+  std::vector<int32_t> ev_result(NB);
   int nthreads = (int)std::min<unsigned>(16, std::max(1u, std::thread::hardware_concurrency()));
   std::vector<std::thread> pool;
   std::vector<std::string> errs(nthreads);
@@ -806,10 +898,10 @@ int build_layout(gcm_context* c, const gcm_tables* t) {
           const int64_t se = h.first_event + j;
           const int64_t e = order[se];
           const int64_t d0 = t->dial_begin[e], d1 = t->dial_begin[e + 1];
-          ev_base[se] = t->base_weight[e];
-          ev_bin[se] = t->bin[e];
+          ev[se].base = t->base_weight[e];
+          ev[se].bin = t->bin[e];
           ev_result[se] = (int32_t)e;
-          ev_dial[se] = (uint32_t)local | ((uint32_t)(d1 - d0) << 16);
+          ev[se].dials = (uint32_t)local | ((uint32_t)(d1 - d0) << 16);
           for (int64_t d = d0; d < d1; ++d) {
             dial_ids.push_back(d);
             slot.push_back(local++);
@@ -880,9 +972,7 @@ int build_layout(gcm_context* c, const gcm_tables* t) {
     GCM_TRY(upload(c, &p, rec.data(), rec.size()));
     c->d_rec = p;
   }
-  GCM_TRY(upload(c, &c->d_ev_base, ev_base.data(), ev_base.size()));
-  GCM_TRY(upload(c, &c->d_ev_bin, ev_bin.data(), ev_bin.size()));
-  GCM_TRY(upload(c, &c->d_ev_dial, ev_dial.data(), ev_dial.size()));
+  GCM_TRY(upload(c, &c->d_ev, ev.data(), ev.size()));
   GCM_TRY(upload(c, &c->d_ev_result, ev_result.data(), ev_result.size()));
   GCM_TRY(upload(c, &c->d_bin_pbeg, bin_np.data(), bin_np.size()));
   return GCM_OK;
@@ -961,8 +1051,11 @@ int gcm_set_launch_config(gcm_context* c, const gcm_launch_config* cfg) {
     l.tile_words = cfg->tile_words;
   }
   if (cfg->blocks_per_cu) l.blocks_per_cu = cfg->blocks_per_cu;
-  if (l.block_threads < 64 || l.block_threads > 512 || l.block_threads % 64)
-    return fail(GCM_ERR_ARG, "block_threads must be a multiple of 64 in [64, 512]");
+  if (cfg->ring_buffers) l.ring_buffers = cfg->ring_buffers;
+  if (l.ring_buffers != 2 && l.ring_buffers != 3)
+    return fail(GCM_ERR_ARG, "ring_buffers must be 2 or 3");
+  if (l.block_threads < 64 || l.block_threads > 832 || l.block_threads % 64)
+    return fail(GCM_ERR_ARG, "block_threads must be a multiple of 64 in [64, 832]");
   if (l.tile_words < 256 || l.tile_words > 8192 || l.tile_words % 128)
     return fail(GCM_ERR_ARG, "tile_words must be a multiple of 128 in [256, 8192]");
   if (l.blocks_per_cu < 1 || l.blocks_per_cu > 32)
@@ -1028,7 +1121,9 @@ int gcm_build(gcm_context* c, const gcm_tables* t, int32_t max_points) {
     mx[i] = std::isnan(in.max_response) ? inf : in.max_response;
   }
   const size_t K = (size_t)max_points;
-  c->fin_blocks = (c->n_bins + kFinalizeBlock / kWave - 1) / (kFinalizeBlock / kWave);
+  // one wave per bin, grid-strided; few blocks keep the ticket count low
+  c->fin_blocks = std::max(1, std::min(kFinalizeMaxBlocks,
+                                       (c->n_bins + kFinalizeBlock / kWave - 1) / (kFinalizeBlock / kWave)));
   rc = GCM_OK;
   auto step = [&](int r) { if (rc == GCM_OK) rc = r; };
   step(upload(c, &c->d_in_par, par.data(), par.size()));
@@ -1201,9 +1296,7 @@ int gcm_fill(gcm_context* c, void* stream_ptr) {
   a.n_tiles = c->n_tiles;
   a.desc = c->d_desc;
   a.rec = c->d_rec;
-  a.ev_base = c->d_ev_base;
-  a.ev_bin = c->d_ev_bin;
-  a.ev_dial = c->d_ev_dial;
+  a.ev = c->d_ev;
   a.ev_result = c->d_ev_result;
   a.xin = c->d_xin;
   a.n_inputs = c->n_inputs;
@@ -1223,7 +1316,9 @@ int gcm_fill(gcm_context* c, void* stream_ptr) {
   const size_t in_bytes = (size_t)K * c->n_inputs * sizeof(InputRec);
   a.inputs_lds = in_bytes <= kInputsLdsMax;
   const size_t in_pad = a.inputs_lds ? (in_bytes + 1023) / 1024 * 1024 : 0;
-  const size_t lds = in_pad + 2 * ((size_t)a.rec_bytes + (size_t)a.max_dials * 8) +
+  const size_t lds = in_pad +
+                     (size_t)c->launch.ring_buffers * ((size_t)a.rec_bytes + (size_t)a.max_dials * 8 +
+                          kMaxTileEvents * sizeof(EventRec) + kHeaderBytes) +
                      2 * (size_t)a.max_dials * 8;
   hipEvent_t* tev = nullptr;
   if (c->timing) {
@@ -1234,21 +1329,22 @@ int gcm_fill(gcm_context* c, void* stream_ptr) {
   if (c->n_tiles > 0) {
     const int64_t want = (int64_t)c->launch.blocks_per_cu * c->n_cu;
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(c->n_tiles, want));
-    const int threads = c->launch.block_threads + kWave;  // + the producer wave
-    const void* fn;
-    if (fp32) fn = a.inputs_lds ? (const void*)fill_kernel<float, true, true>
-                                : (const void*)fill_kernel<float, true, false>;
-    else fn = a.inputs_lds ? (const void*)fill_kernel<double, false, true>
-                           : (const void*)fill_kernel<double, false, false>;
+    const int nb = c->launch.ring_buffers;
+    // evaluators + the event wave + nb - 1 producer waves
+    const int threads = c->launch.block_threads + nb * kWave;
+    const void* fn = nullptr;
+#define GCM_FILL_FN(RT, CMP, INL, NBV) (const void*)fill_kernel<RT, CMP, INL, NBV>
+    if (fp32)
+      fn = nb == 3 ? (a.inputs_lds ? GCM_FILL_FN(float, true, true, 3) : GCM_FILL_FN(float, true, false, 3))
+                   : (a.inputs_lds ? GCM_FILL_FN(float, true, true, 2) : GCM_FILL_FN(float, true, false, 2));
+    else
+      fn = nb == 3 ? (a.inputs_lds ? GCM_FILL_FN(double, false, true, 3) : GCM_FILL_FN(double, false, false, 3))
+                   : (a.inputs_lds ? GCM_FILL_FN(double, false, true, 2) : GCM_FILL_FN(double, false, false, 2));
+#undef GCM_FILL_FN
     if (lds > 65536)
       GCM_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
-    if (fp32) {
-      if (a.inputs_lds) fill_kernel<float, true, true><<<grid, threads, lds, s>>>(a);
-      else fill_kernel<float, true, false><<<grid, threads, lds, s>>>(a);
-    } else {
-      if (a.inputs_lds) fill_kernel<double, false, true><<<grid, threads, lds, s>>>(a);
-      else fill_kernel<double, false, false><<<grid, threads, lds, s>>>(a);
-    }
+    void* args[] = {&a};
+    GCM_HIP(hipLaunchKernel(fn, dim3(grid), dim3(threads), args, lds, s));
     GCM_HIP(hipGetLastError());
   }
   if (tev) GCM_HIP(hipEventRecord(tev[1], s));

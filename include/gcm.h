@@ -130,7 +130,8 @@ typedef struct gcm_launch_config {
   int32_t block_threads;    /* threads per block of the fill kernel         */
   int32_t tile_words;       /* record words (8 B) staged in LDS per tile    */
   int32_t blocks_per_cu;    /* resident blocks per CU for the grid size     */
-  int32_t reserved;
+  int32_t ring_buffers;     /* LDS tile buffers (2 or 3); producers prefetch
+                               ring_buffers - 1 tiles ahead                 */
 } gcm_launch_config;
 
 typedef struct gcm_context gcm_context;
