@@ -56,10 +56,22 @@ def build_hbm_copy() -> str:
     return out
 
 
+def build_facade_demo() -> str:
+    """The C++ facade example, linked against the in-tree libraries."""
+    out = os.path.join(HERE, "facade_demo")
+    tmp = out + ".tmp"
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-o", tmp,
+          os.path.join(CSRC, "facade_demo.cpp"), "-L" + HERE, "-lgcm",
+          "-lgcm_synth", "-Wl,-rpath,$ORIGIN", "-lpthread"])
+    os.replace(tmp, out)
+    return out
+
+
 def build_all() -> None:
     build_synth()
     build_gcm()
     build_hbm_copy()
+    build_facade_demo()
 
 
 if __name__ == "__main__":

@@ -1,0 +1,151 @@
+// gcm_cache_manager.hpp -- header-only C++ facade over the C ABI in
+// include/gcm.h with the shape of the reference's Cache::Manager
+// (src/CacheManager/include/CacheManager.h:56-410): Build, Fill returning a
+// deferred std::future<bool>, PropagateParameters, CopyHistogramContents,
+// CopyEventWeights, GetResidentMemory and the static toggles.  GUNDAM's
+// Propagator can hold one of these instead of the singleton; the tables it
+// passes are the EventDialCache content laid out as gcm_tables.
+#pragma once
+
+#include <cmath>
+#include <cstdint>
+#include <future>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/gcm.h"
+
+namespace gcm {
+
+class Error : public std::runtime_error {
+ public:
+  Error(int code, const std::string& what) : std::runtime_error(what), code_(code) {}
+  int code() const { return code_; }
+
+ private:
+  int code_;
+};
+
+inline void check(int rc, const char* where) {
+  if (rc != GCM_OK)
+    throw Error(rc, std::string(where) + ": " + gcm_last_error());
+}
+
+class CacheManager {
+ public:
+  explicit CacheManager(int device = 0, int precision = GCM_PRECISION_FP64) {
+    check(gcm_create(device, &ctx_), "gcm_create");
+    check(gcm_set_precision(ctx_, precision), "gcm_set_precision");
+  }
+  ~CacheManager() { gcm_destroy(ctx_); }
+  CacheManager(const CacheManager&) = delete;
+  CacheManager& operator=(const CacheManager&) = delete;
+
+  // Cache::Manager::HasGPU (CacheManager.h:73-78)
+  static bool HasGPU() {
+    int n = 0;
+    return gcm_device_count(&n) == GCM_OK && n > 0;
+  }
+
+  void SetLaunchConfig(const gcm_launch_config& cfg) {
+    check(gcm_set_launch_config(ctx_, &cfg), "gcm_set_launch_config");
+  }
+
+  // Cache::Manager::Build + Update (CacheManager.cpp:198-740)
+  bool Build(const gcm_tables& tables, int max_points = 1) {
+    check(gcm_build(ctx_, &tables, max_points), "gcm_build");
+    n_bins_ = tables.n_bins;
+    n_events_ = tables.n_events;
+    n_par_ = tables.n_parameters;
+    table_len_ = tables.n_tables ? tables.table_begin[tables.n_tables] : 0;
+    built_ = true;
+    return true;
+  }
+  bool IsBuilt() const { return built_; }
+
+  // SetIsEventWeightCopyEnabled (CacheManager.h:104-109)
+  bool SetIsEventWeightCopyEnabled(bool enable) {
+    const bool old = weight_copy_;
+    check(gcm_enable_weights(ctx_, enable ? 1 : 0), "gcm_enable_weights");
+    weight_copy_ = enable;
+    return old;
+  }
+
+  void SetLikelihood(const gcm_llh_config& cfg, const std::vector<double>& data,
+                     const std::vector<double>& prior_mean,
+                     const std::vector<double>& prior_sigma) {
+    check(gcm_set_likelihood(ctx_, &cfg, data.data(),
+                             prior_mean.empty() ? nullptr : prior_mean.data(),
+                             prior_sigma.empty() ? nullptr : prior_sigma.data()),
+          "gcm_set_likelihood");
+  }
+
+  // Cache::Manager::Fill (CacheManager.cpp:750-810): upload the parameter
+  // point(s) and the lookup tables, enqueue the evaluation, and return a
+  // deferred future whose get() synchronises and copies the histograms when
+  // the copy is enabled.
+  std::future<bool> Fill(const std::vector<double>& theta, int k_points = 1,
+                         const std::vector<double>* table_values = nullptr) {
+    check(gcm_set_parameters(ctx_, theta.data(), n_par_, k_points), "gcm_set_parameters");
+    if (table_values && table_len_)
+      check(gcm_set_table_values(ctx_, table_values->data(), table_len_, k_points),
+            "gcm_set_table_values");
+    check(gcm_fill(ctx_, nullptr), "gcm_fill");
+    return std::async(std::launch::deferred, [this] {
+      check(gcm_sync(ctx_), "gcm_sync");
+      if (hist_copy_) CopyHistogramContents();
+      if (weight_copy_) CopyEventWeights();
+      return true;
+    });
+  }
+
+  bool PropagateParameters(const std::vector<double>& theta,
+                           const std::vector<double>* table_values = nullptr) {
+    return Fill(theta, 1, table_values).get();
+  }
+
+  // CopyHistogramContents -> CacheSampleHistFiller::copyHistogram
+  // (CacheSampleHistFiller.cpp:9-19): sumWeights and sqrtSumSqWeights.
+  bool CopyHistogramContents(int k = 0) {
+    sum_w_.resize(n_bins_);
+    std::vector<double> s2(n_bins_);
+    check(gcm_get_hist(ctx_, k, sum_w_.data(), s2.data()), "gcm_get_hist");
+    sqrt_sum_w2_.resize(n_bins_);
+    for (int b = 0; b < n_bins_; ++b) sqrt_sum_w2_[b] = std::sqrt(s2[b]);
+    return true;
+  }
+  bool CopyEventWeights() {
+    weights_.assign((size_t)n_events_, std::nan(""));
+    check(gcm_get_weights(ctx_, weights_.data()), "gcm_get_weights");
+    return true;
+  }
+
+  std::vector<double> Likelihood(int k_points = 1) {
+    std::vector<double> tot(k_points);
+    check(gcm_get_llh(ctx_, tot.data(), nullptr, nullptr), "gcm_get_llh");
+    return tot;
+  }
+
+  std::size_t GetResidentMemory() const {
+    int64_t b = 0;
+    gcm_resident_bytes(ctx_, &b);
+    return (std::size_t)b;
+  }
+
+  const std::vector<double>& SumWeights() const { return sum_w_; }
+  const std::vector<double>& SqrtSumSqWeights() const { return sqrt_sum_w2_; }
+  const std::vector<double>& EventWeights() const { return weights_; }
+  gcm_context* context() { return ctx_; }
+
+ private:
+  gcm_context* ctx_ = nullptr;
+  bool built_ = false;
+  bool hist_copy_ = true;  // fIsHistContentCopyEnabled default true
+  bool weight_copy_ = false;
+  int n_bins_ = 0, n_par_ = 0;
+  int64_t n_events_ = 0, table_len_ = 0;
+  std::vector<double> sum_w_, sqrt_sum_w2_, weights_;
+};
+
+}  // namespace gcm

@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "../../include/gcm.h"
+#include "synth.h"
 
 namespace {
 
@@ -434,33 +435,6 @@ thread_local std::string g_err;
 
 extern "C" {
 
-typedef struct gcm_synth_view {
-  int64_t n_events;
-  int32_t n_vars;
-  int32_t n_bins;
-  int32_t n_parameters;
-  int32_t n_inputs;
-  int32_t n_tables;
-  int32_t n_samples;
-  int64_t n_dials;
-  int64_t n_data;
-  int64_t table_len;
-  double reweight_cap;
-  const double* vars;         // [n_events][n_vars]
-  const double* edges;        // [n_bins][n_vars][2]
-  const double* base_weight;
-  int32_t* bin;               // filled by the caller (binning)
-  const int64_t* dial_begin;
-  const uint8_t* dial_type;
-  const int32_t* dial_input;
-  const int64_t* data_begin;
-  const double* dial_data;
-  const gcm_input* inputs;
-  const int64_t* table_begin;
-  const double* prior_mean;
-  const double* prior_sigma;
-  const int32_t* par_type;    // internal parameter kind per parameter
-} gcm_synth_view;
 
 const char* gcm_synth_last_error(void) { return g_err.c_str(); }
 

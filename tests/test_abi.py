@@ -81,3 +81,17 @@ def test_algorithmic_bytes_hand_count():
     # monotonic 9 words*8 + 2, norm 2, kriged (k=2) 8*2+4+2, one binned event
     want = (72 + 2) + 2 + (20 + 2) + 16 + 8 * 16 + 8 * 3 + 16 * 4
     assert t.algorithmic_bytes(8) == want
+
+
+def test_cpp_facade_compiles_and_links():
+    """gcm_cache_manager.hpp compiles as C++17 and links against libgcm."""
+    import subprocess
+    import tempfile
+    csrc = os.path.join(ROOT, "gundam_amd", "csrc")
+    with tempfile.TemporaryDirectory() as d:
+        out = os.path.join(d, "demo")
+        r = subprocess.run(["g++", "-std=c++17", "-O0", "-o", out,
+                            os.path.join(csrc, "facade_demo.cpp"),
+                            "-L" + os.path.join(ROOT, "gundam_amd"), "-lgcm", "-lgcm_synth",
+                            "-lpthread"], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr

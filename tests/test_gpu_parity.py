@@ -112,3 +112,16 @@ def test_fp32_mode_error_bounds():
     rel = np.abs(s[nz] - s_ref[nz]) / s_ref[nz]
     print("fp32 max per-bin rel err", rel.max())
     assert rel.max() < 1e-4
+
+
+def test_cpp_facade_demo_runs():
+    """The C++ Cache::Manager-shaped facade drives the library end to end."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "gundam_amd", "facade_demo")
+    print("facade demo", flush=True)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    vals = [float(l.split("LLH")[1]) for l in r.stdout.splitlines() if "LLH" in l and "point" in l]
+    assert len(vals) == 4 and all(np.isfinite(vals)) and all(v > 0 for v in vals)
