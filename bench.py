@@ -83,6 +83,56 @@ def cpu_baseline(workload, n_sample, n_full, seed, threads, evals=20):
     }
 
 
+def batched_workload3(args, log):
+    """The batched measurement on the 100k-event set re-split into samples of
+    50k/30k/15k/5k (20x20 bins each): K = 64 points per launch -- 32 chain
+    points (prior + N(0,1)) and a 32-point central-difference stencil (+-h,
+    h = 1e-3, on parameters 0-15 around one centre) -- captured in a hipGraph,
+    launch shape autotuned on K = 64 launches."""
+    from gundam_amd import CacheManager, Workload
+    from gundam_amd.autotune import autotune
+
+    wl = Workload(3, seed=args.seed)
+    t = wl.tables
+    wl.bin_events_on_device(0)
+    K = 64
+    rng = np.random.default_rng(args.seed + 3)
+    chains = wl.prior_mean + wl.prior_sigma * rng.standard_normal((32, wl.n_parameters))
+    centre = wl.prior_mean + 0.3 * wl.prior_sigma * rng.standard_normal(wl.n_parameters)
+    stencil = np.repeat(centre[None, :], 32, axis=0)
+    for j in range(16):
+        stencil[2 * j, j] += 1e-3
+        stencil[2 * j + 1, j] -= 1e-3
+    pts = np.concatenate([chains, stencil])
+    tuned = autotune(t, pts, None, n_events=t.n_events, reps=3)
+    cm = CacheManager(0, block_threads=tuned["block_threads"],
+                      tile_words=tuned["tile_words"],
+                      blocks_per_cu=tuned["blocks_per_cu"],
+                      ring_buffers=tuned["ring_buffers"])
+    cm.build(t, max_points=K)
+    cm.propagate_parameters(wl.nominal())
+    cm.set_likelihood("poisson", cm.hist(0)[0].copy(), wl.prior_mean, wl.prior_sigma)
+    cm.enable_graph(True)
+    cm.set_parameters(pts)
+    cm.fill().get()
+    reps = 20
+    lat = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        cm.set_parameters(pts)
+        cm.fill()
+        cm.llh()
+        lat.append(time.perf_counter() - t0)
+    med = float(np.median(lat))
+    log(f"batched C3 (100k events, K=64, graph): {K/med:.0f} evals/s, "
+        f"{med*1e3:.3f} ms/batch, launch {cm.launch_config()}")
+    return {"workload": "100k events in 4 samples (50k/30k/15k/5k) x 400 bins, 20 "
+                        "parameters, 4 monotonic splines + 1 norm per event, Poisson",
+            "points_per_launch": K, "evals_per_s": K / med,
+            "latency_ms_per_batch": med * 1e3, "hipgraph": True,
+            "launch": cm.launch_config()}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -101,6 +151,8 @@ def main():
     ap.add_argument("--ring", type=int, default=0, help="LDS tile buffers (2 or 3)")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-c3", action="store_true",
+                    help="skip the 100k-event K=64 batched measurement")
     ap.add_argument("--tune-events", type=int, default=500000)
     args = ap.parse_args()
 
@@ -209,6 +261,8 @@ def main():
                    "fill_kernel_ms": tmb["fill_ms"] / max(1, tmb["count"])}
         log(f"batched K={K}: {K/el:.1f} evals/s, {el*1e3:.2f} ms/batch")
 
+    batched_c3 = None if args.no_c3 else batched_workload3(args, log)
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -259,6 +313,7 @@ def main():
             "avg_kernel_ms": fill_ms,
         },
         "batched": batched,
+        "batched_c3": batched_c3,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -42,12 +42,12 @@ def head(t: Tables, n_events: int) -> Tables:
                   _keep=[t])
 
 
-def candidates(n_inputs: int, precision: str = "fp64") -> list:
+def candidates(n_inputs: int, precision: str = "fp64", k_points: int = 1) -> list:
     out = []
     for nb, tile, block, bpc in itertools.product((2, 3), (1024, 1536, 2048, 2560, 3072, 4096),
                                                   (128, 256, 384, 512, 768),
                                                   (1, 2, 3, 4, 5, 6)):
-        lds = lds_per_block(tile, n_inputs, ring_buffers=nb)
+        lds = lds_per_block(tile, n_inputs, k_points, ring_buffers=nb)
         # fill the CU's LDS: only the largest block count that fits
         if lds * bpc > LDS_BYTES_PER_CU or lds * (bpc + 1) <= LDS_BYTES_PER_CU:
             continue
@@ -62,15 +62,19 @@ def autotune(tables: Tables, theta: np.ndarray, table_values: Optional[np.ndarra
              precision: str = "fp64", n_events: int = 1_000_000, reps: int = 5,
              cands: Optional[Iterable[dict]] = None, device: int = 0,
              log=None) -> dict:
+    """theta may hold K points ([K, P]): the shapes are then timed on K-point
+    launches."""
+    theta = np.atleast_2d(theta)
+    k_points = theta.shape[0]
     sub = head(tables, n_events)
     cands = list(cands) if cands is not None else candidates(
-        int(tables.inputs.shape[0]), precision)
+        int(tables.inputs.shape[0]), precision, k_points)
     results = []
     t_start = time.time()
     for c in cands:
         cm = CacheManager(device, precision=precision, **c)
         try:
-            cm.build(sub)
+            cm.build(sub, max_points=k_points)
             cm.propagate_parameters(theta, table_values)
             cm.enable_timing(True)
             for _ in range(reps):

@@ -175,6 +175,10 @@ class CacheManager:
                                       pen.ctypes.data))
         return tot, st, pen
 
+    def enable_graph(self, enable: bool = True) -> None:
+        """Capture each point count's evaluation in a hipGraph and replay it."""
+        N.check(self._lib.gcm_enable_graph(self._h, int(bool(enable))))
+
     def enable_timing(self, enable: bool = True) -> None:
         N.check(self._lib.gcm_enable_timing(self._h, int(bool(enable))))
 

@@ -51,6 +51,7 @@ constexpr int kFinalizeBlock = 256;
 constexpr int kFinalizeMaxBlocks = 128;
 constexpr size_t kInputsLdsMax = 16384;  // stage input records in LDS up to this
 constexpr size_t kHeaderBytes = 32;      // tile header copy in each LDS buffer
+constexpr int kMaxDynamicLds = 160 * 1024;  // gfx950 LDS per CU
 
 // One binned event, in bin-sorted order: base weight, global bin, and its
 // dials as (first event-major slot in the tile, count).
@@ -701,6 +702,14 @@ struct gcm_context {
   bool tab_dirty = false, theta_dirty = false;
   hipEvent_t ev_staged = nullptr;  // staging buffers consumed by the device
   hipStream_t last_stream = nullptr;
+  bool lds_attr_set = false;
+  // hipGraph mode: one instantiated graph per point count K
+  bool graph_mode = false;
+  std::vector<std::pair<int, hipGraphExec_t>> graphs;
+  void drop_graphs() {
+    for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+    graphs.clear();
+  }
   // kernel timing: a ring of event triples {before fill, after fill, after
   // finalize}, harvested at every synchronisation
   static constexpr int kTimingRing = 64;
@@ -1020,6 +1029,7 @@ int gcm_destroy(gcm_context* c) {
   if (!c) return GCM_OK;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
+  c->drop_graphs();
   c->free_all();
   for (auto& tr : c->tev)
     for (auto& e : tr)
@@ -1252,6 +1262,7 @@ int gcm_set_likelihood(gcm_context* c, const gcm_llh_config* cfg,
   GCM_HIP(hipMemcpy(c->d_prior_mean, mean.data(), c->n_par * sizeof(double), hipMemcpyHostToDevice));
   GCM_HIP(hipMemcpy(c->d_prior_sigma, sig.data(), c->n_par * sizeof(double), hipMemcpyHostToDevice));
   c->llh_set = true;
+  c->drop_graphs();  // kernel arguments changed
   return GCM_OK;
 }
 
@@ -1262,28 +1273,42 @@ int gcm_enable_weights(gcm_context* c, int enable) {
     GCM_TRY(c->alloc(&c->d_weights, (size_t)std::max<int64_t>(1, c->n_events_all)));
     GCM_HIP(hipMemset(c->d_weights, 0, (size_t)std::max<int64_t>(1, c->n_events_all) * sizeof(double)));
   }
+  if (c->weights_enabled != (enable != 0)) {
+    GCM_HIP(hipStreamSynchronize(c->last_stream ? c->last_stream : c->stream));
+    c->drop_graphs();  // kernel arguments changed
+  }
   c->weights_enabled = enable != 0;
   return GCM_OK;
 }
 
-int gcm_fill(gcm_context* c, void* stream_ptr) {
-  if (!c) return fail(GCM_ERR_ARG, "ctx is NULL");
-  if (!c->built) return fail(GCM_ERR_STATE, "gcm_build first");
-  if (c->k_points < 1) return fail(GCM_ERR_STATE, "gcm_set_parameters first");
-  hipStream_t s = stream_ptr ? (hipStream_t)stream_ptr : c->stream;
-  GCM_HIP(hipSetDevice(c->device));
-  const int K = c->k_points;
-  if (c->theta_dirty) {
+// Per-kernel attributes that must be set outside stream capture.
+static int prepare_launch(gcm_context* c) {
+  if (c->lds_attr_set) return GCM_OK;
+  for (const void* f : {(const void*)fill_kernel<float, true, true, 3>,
+                        (const void*)fill_kernel<float, true, false, 3>,
+                        (const void*)fill_kernel<float, true, true, 2>,
+                        (const void*)fill_kernel<float, true, false, 2>,
+                        (const void*)fill_kernel<double, false, true, 3>,
+                        (const void*)fill_kernel<double, false, false, 3>,
+                        (const void*)fill_kernel<double, false, true, 2>,
+                        (const void*)fill_kernel<double, false, false, 2>})
+    GCM_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxDynamicLds));
+  c->lds_attr_set = true;
+  return GCM_OK;
+}
+
+// Enqueue one evaluation of K points on stream s: the staged uploads (when
+// `copies`), the prologue, fill and finalize kernels, and the timing events
+// (when tev is given).  Used eagerly and under hipGraph stream capture.
+static int enqueue_fill(gcm_context* c, hipStream_t s, int K, bool copies,
+                        hipEvent_t* tev) {
+  if (copies) {
     GCM_HIP(hipMemcpyAsync(c->d_theta, c->h_theta, (size_t)K * c->n_par * sizeof(double),
                            hipMemcpyHostToDevice, s));
-    c->theta_dirty = false;
+    if (c->table_len)
+      GCM_HIP(hipMemcpyAsync(c->d_tab, c->h_tab, (size_t)K * c->table_len * sizeof(double),
+                             hipMemcpyHostToDevice, s));
   }
-  if (c->tab_dirty) {
-    GCM_HIP(hipMemcpyAsync(c->d_tab, c->h_tab, (size_t)K * c->table_len * sizeof(double),
-                           hipMemcpyHostToDevice, s));
-    c->tab_dirty = false;
-  }
-  GCM_HIP(hipEventRecord(c->ev_staged, s));
   prologue_kernel<<<K, 256, 0, s>>>(c->d_theta, c->n_par, c->d_in_par, c->d_in_mlo,
                                     c->d_in_mrange, c->d_in_min, c->d_in_max,
                                     c->d_in_flags, c->n_inputs, c->d_xin,
@@ -1320,12 +1345,7 @@ int gcm_fill(gcm_context* c, void* stream_ptr) {
                      (size_t)c->launch.ring_buffers * ((size_t)a.rec_bytes + (size_t)a.max_dials * 8 +
                           kMaxTileEvents * sizeof(EventRec) + kHeaderBytes) +
                      2 * (size_t)a.max_dials * 8;
-  hipEvent_t* tev = nullptr;
-  if (c->timing) {
-    if (c->t_issued - c->t_done >= gcm_context::kTimingRing) GCM_TRY(harvest_timing(c));
-    tev = c->tev[c->t_issued % gcm_context::kTimingRing];
-    GCM_HIP(hipEventRecord(tev[0], s));
-  }
+  if (tev) GCM_HIP(hipEventRecord(tev[0], s));
   if (c->n_tiles > 0) {
     const int64_t want = (int64_t)c->launch.blocks_per_cu * c->n_cu;
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(c->n_tiles, want));
@@ -1341,8 +1361,7 @@ int gcm_fill(gcm_context* c, void* stream_ptr) {
       fn = nb == 3 ? (a.inputs_lds ? GCM_FILL_FN(double, false, true, 3) : GCM_FILL_FN(double, false, false, 3))
                    : (a.inputs_lds ? GCM_FILL_FN(double, false, true, 2) : GCM_FILL_FN(double, false, false, 2));
 #undef GCM_FILL_FN
-    if (lds > 65536)
-      GCM_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
+    (void)fn;
     void* args[] = {&a};
     GCM_HIP(hipLaunchKernel(fn, dim3(grid), dim3(threads), args, lds, s));
     GCM_HIP(hipGetLastError());
@@ -1360,13 +1379,67 @@ int gcm_fill(gcm_context* c, void* stream_ptr) {
         c->n_bins, c->llh_set ? c->d_data : nullptr, c->llh, c->d_hist, c->d_hist2,
         c->d_llh_part, c->d_tickets, c->d_penalty, c->d_stat, c->d_pen, c->d_total);
   GCM_HIP(hipGetLastError());
-  if (tev) {
-    GCM_HIP(hipEventRecord(tev[2], s));
-    c->t_issued++;
+  if (tev) GCM_HIP(hipEventRecord(tev[2], s));
+  return GCM_OK;
+}
+
+int gcm_fill(gcm_context* c, void* stream_ptr) {
+  if (!c) return fail(GCM_ERR_ARG, "ctx is NULL");
+  if (!c->built) return fail(GCM_ERR_STATE, "gcm_build first");
+  if (c->k_points < 1) return fail(GCM_ERR_STATE, "gcm_set_parameters first");
+  hipStream_t s = stream_ptr ? (hipStream_t)stream_ptr : c->stream;
+  GCM_HIP(hipSetDevice(c->device));
+  const int K = c->k_points;
+  if (c->graph_mode && !c->timing) {
+    // one captured graph per point count: uploads from the pinned staging
+    // buffers + the three kernels; replays pick up the staged values
+    hipGraphExec_t exec = nullptr;
+    for (auto& g : c->graphs)
+      if (g.first == K) exec = g.second;
+    if (!exec) {
+      GCM_TRY(prepare_launch(c));
+      GCM_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      const int rc = enqueue_fill(c, c->stream, K, true, nullptr);
+      hipGraph_t graph = nullptr;
+      const hipError_t e = hipStreamEndCapture(c->stream, &graph);
+      if (rc != GCM_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      if (e != hipSuccess) return fail(GCM_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+      const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ei != hipSuccess) return fail(GCM_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ei));
+      c->graphs.emplace_back(K, exec);
+    }
+    GCM_HIP(hipGraphLaunch(exec, s));
+    c->theta_dirty = c->tab_dirty = false;
+  } else {
+    hipEvent_t* tev = nullptr;
+    if (c->timing) {
+      if (c->t_issued - c->t_done >= gcm_context::kTimingRing) GCM_TRY(harvest_timing(c));
+      tev = c->tev[c->t_issued % gcm_context::kTimingRing];
+    }
+    GCM_TRY(prepare_launch(c));
+    GCM_TRY(enqueue_fill(c, s, K, c->theta_dirty || c->tab_dirty, tev));
+    c->theta_dirty = c->tab_dirty = false;
+    if (tev) c->t_issued++;
   }
+  GCM_HIP(hipEventRecord(c->ev_staged, s));
   c->last_stream = s;
   return GCM_OK;
 }
+
+int gcm_enable_graph(gcm_context* c, int enable) {
+  if (!c) return fail(GCM_ERR_ARG, "ctx is NULL");
+  GCM_HIP(hipSetDevice(c->device));
+  GCM_HIP(hipStreamSynchronize(c->last_stream ? c->last_stream : c->stream));
+  c->drop_graphs();
+  c->graph_mode = enable != 0;
+  return GCM_OK;
+}
+
+
 
 int gcm_sync(gcm_context* c) {
   if (!c) return fail(GCM_ERR_ARG, "ctx is NULL");

@@ -191,6 +191,11 @@ int gcm_fill(gcm_context* ctx, void* stream);
 /* Wait for the last fill; reports NaN weights (CacheWeights.cpp:83-86) as
  * GCM_ERR_NUMERIC. */
 int gcm_sync(gcm_context* ctx);
+/* hipGraph mode (no reference counterpart; the reference launches every
+ * kernel eagerly, CacheWeights.cpp:153-183): the first gcm_fill for a point
+ * count K captures the uploads and the kernels into a graph, later fills
+ * replay it.  Timing (gcm_enable_timing) forces eager launches. */
+int gcm_enable_graph(gcm_context* ctx, int enable);
 
 /* ---- fetch (CopyHistogramContents / CopyEventWeights,
  *      CacheManager.cpp:832-866) ------------------------------------------- */

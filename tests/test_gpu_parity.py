@@ -125,3 +125,37 @@ def test_cpp_facade_demo_runs():
     assert r.returncode == 0, r.stderr
     vals = [float(l.split("LLH")[1]) for l in r.stdout.splitlines() if "LLH" in l and "point" in l]
     assert len(vals) == 4 and all(np.isfinite(vals)) and all(v > 0 for v in vals)
+
+
+def test_graph_replays_match_eager():
+    """hipGraph mode: several replays with new parameters, interleaved with
+    eager work and synchronisations, each equal to the eager result."""
+    print("graph replays workload=3 K=4", flush=True)
+    wl, t = _setup(3, 0)
+    pts = wl.random_points(12, 0.4, seed=21)
+    eager = CacheManager(0)
+    eager.build(t, max_points=4)
+    graph = CacheManager(0)
+    graph.build(t, max_points=4)
+    data = np.full(t.n_bins, 40.0)
+    for cm in (eager, graph):
+        cm.set_likelihood("poisson", data, wl.prior_mean, wl.prior_sigma)
+    graph.enable_graph(True)
+    for r in range(3):
+        batch = pts[4 * r:4 * r + 4]
+        eager.set_parameters(batch)
+        eager.fill().get()
+        graph.set_parameters(batch)
+        graph.fill().get()
+        for k in range(4):
+            np.testing.assert_array_equal(graph.hist(k)[0], eager.hist(k)[0])
+            np.testing.assert_array_equal(graph.hist(k)[1], eager.hist(k)[1])
+        np.testing.assert_array_equal(graph.llh()[0], eager.llh()[0])
+        # eager single-point work on the graph context between replays
+        graph.enable_graph(False)
+        graph.set_parameters(pts[r])
+        graph.fill().get()
+        eager.set_parameters(pts[r])
+        eager.fill().get()
+        np.testing.assert_array_equal(graph.hist(0)[0], eager.hist(0)[0])
+        graph.enable_graph(True)
