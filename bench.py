@@ -111,7 +111,7 @@ def batched_workload3(args, log):
                       ring_buffers=tuned["ring_buffers"])
     cm.build(t, max_points=K)
     cm.propagate_parameters(wl.nominal())
-    cm.set_likelihood("poisson", cm.hist(0)[0].copy(), wl.prior_mean, wl.prior_sigma)
+    cm.set_likelihood("barlow", cm.hist(0)[0].copy(), wl.prior_mean, wl.prior_sigma)
     cm.enable_graph(True)
     cm.set_parameters(pts)
     cm.fill().get()
@@ -127,7 +127,7 @@ def batched_workload3(args, log):
     log(f"batched C3 (100k events, K=64, graph): {K/med:.0f} evals/s, "
         f"{med*1e3:.3f} ms/batch, launch {cm.launch_config()}")
     return {"workload": "100k events in 4 samples (50k/30k/15k/5k) x 400 bins, 20 "
-                        "parameters, 4 monotonic splines + 1 norm per event, Poisson",
+                        "parameters, 4 monotonic splines + 1 norm per event, BB OA2021",
             "points_per_launch": K, "evals_per_s": K / med,
             "latency_ms_per_batch": med * 1e3, "hipgraph": True,
             "launch": cm.launch_config()}

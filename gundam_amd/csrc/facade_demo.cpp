@@ -43,7 +43,9 @@ int main() {
   std::vector<double> nominal(v.prior_mean, v.prior_mean + v.n_parameters);
   cm.PropagateParameters(nominal);
   std::vector<double> data = cm.SumWeights();  // Asimov data
-  gcm_llh_config llh{GCM_LLH_POISSON, 0, 1e150, -1.0};
+  // Barlow-Beeston OA2021: finite for bins empty at the prior (the Poisson
+  // LLH of the reference returns +inf for a bin predicted <= 0)
+  gcm_llh_config llh{GCM_LLH_BARLOW_OA2021, 0, 1e150, -1.0};
   cm.SetLikelihood(llh, data, nominal,
                    std::vector<double>(v.prior_sigma, v.prior_sigma + v.n_parameters));
   std::mt19937_64 rng(7);
