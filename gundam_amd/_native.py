@@ -119,6 +119,15 @@ GCM_SIGNATURES = {
     "gcm_enable_timing": (C.c_int, [_P, C.c_int]),
     "gcm_get_timing": (C.c_int, [_P, C.POINTER(_D), C.POINTER(_D), C.POINTER(_I64)]),
     "gcm_bin_events": (C.c_int, [C.c_int, _I64, _I32, _P, _I32, _P, _P]),
+    "gcm_mlp_create": (C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64, C.POINTER(_P)]),
+    "gcm_mlp_destroy": (C.c_int, [_P]),
+    "gcm_mlp_param_count": (C.c_int, [_P, C.POINTER(_I64)]),
+    "gcm_mlp_get_params": (C.c_int, [_P, _P]),
+    "gcm_mlp_set_params": (C.c_int, [_P, _P]),
+    "gcm_mlp_predict": (C.c_int, [_P, _P, _I64, _P]),
+    "gcm_mlp_gradients": (C.c_int, [_P, _P, _P, C.c_int, _P, C.POINTER(_D)]),
+    "gcm_mlp_set_data": (C.c_int, [_P, _P, _P, _I64]),
+    "gcm_mlp_train": (C.c_int, [_P, C.c_int, C.c_int, C.c_float, C.c_uint64, _P]),
 }
 
 SYNTH_SIGNATURES = {

@@ -1,0 +1,527 @@
+// mlp.hip -- the learned surrogate: an MLP n_in -> H -> H -> n_out (GELU)
+// that maps a parameter point to all predicted log bin contents, trained on
+// the GPU with hand-written BF16 MFMA kernels (FP32 accumulation) and Adam.
+//
+// GEMM: C[M x N] = op(A)[M x K] . op(B)[K x N] (+ bias, optional GELU), with
+// A and B given by element strides, converted to BF16 while staged in LDS and
+// multiplied with v_mfma_f32_32x32x16_bf16 (gfx950).  Lane layout of that
+// instruction (checked on the device): lane l holds A[l%32][8(l/32)+j] and
+// B[8(l/32)+j][l%32] (j < 8); accumulator r of lane l is
+// C[8(r/4) + 4(l/32) + r%4][l%32].
+//
+// Parameters, flat FP32 master copy: W1[H][n_in] b1[H] W2[H][H] b2[H]
+// W3[n_out][H] b3[n_out]; the GEMMs read BF16 copies of the weights.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/gcm.h"
+
+namespace gcm {
+int fail(int code, const std::string& msg);  // gcm.hip
+}
+
+namespace {
+
+using gcm::fail;
+
+#define MLP_HIP(call)                                                        \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return fail(GCM_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int BM = 64, BN = 64, BK = 32;
+constexpr int LDSK = BK + 8;  // padded row (bf16 elements) against bank conflicts
+
+struct GemmArgs {
+  int M, N, K;
+  const void* A;  // element (m, k) at A[m * a_rs + k * a_cs]
+  int64_t a_rs, a_cs;
+  int a_bf16;
+  const void* B;  // element (k, n) at B[k * b_rs + n * b_cs]
+  int64_t b_rs, b_cs;
+  int b_bf16;
+  float* C;  // C[m * ldc + n]
+  int64_t ldc;
+  const float* bias;  // [N] or null
+  float* Z;           // pre-activation copy (GELU only) or null
+  int gelu;
+};
+
+__device__ __forceinline__ float load_elem(const void* p, int64_t i, int is_bf16) {
+  return is_bf16 ? (float)reinterpret_cast<const __bf16*>(p)[i]
+                 : reinterpret_cast<const float*>(p)[i];
+}
+
+// GELU with the exact error function, and its derivative.
+__device__ __forceinline__ float gelu(float z) {
+  return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f));
+}
+__device__ __forceinline__ float gelu_grad(float z) {
+  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) +
+         z * 0.39894228040143268f * expf(-0.5f * z * z);
+}
+
+__global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
+  __shared__ __align__(16) __bf16 As[BM * LDSK];
+  __shared__ __align__(16) __bf16 Bs[BN * LDSK];
+  const int tid = threadIdx.x;
+  const int lane = tid % 64, wave = tid / 64;
+  const int wm = wave / 2, wn = wave % 2;
+  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  f32x16 acc = {0};
+  // staging: thread -> (row, 8 consecutive k)
+  const int srow = tid / 4, sk = (tid % 4) * 8;
+  for (int k0 = 0; k0 < g.K; k0 += BK) {
+    {
+      const int m = m0 + srow;
+      __bf16 v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int k = k0 + sk + j;
+        v[j] = (__bf16)((m < g.M && k < g.K) ? load_elem(g.A, (int64_t)m * g.a_rs + (int64_t)k * g.a_cs, g.a_bf16) : 0.0f);
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) As[srow * LDSK + sk + j] = v[j];
+    }
+    {
+      const int n = n0 + srow;
+      __bf16 v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int k = k0 + sk + j;
+        v[j] = (__bf16)((n < g.N && k < g.K) ? load_elem(g.B, (int64_t)k * g.b_rs + (int64_t)n * g.b_cs, g.b_bf16) : 0.0f);
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j) Bs[srow * LDSK + sk + j] = v[j];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 16) {
+      bf16x8 a, b;
+      const __bf16* ap = As + (wm * 32 + lane % 32) * LDSK + kk + (lane / 32) * 8;
+      const __bf16* bp = Bs + (wn * 32 + lane % 32) * LDSK + kk + (lane / 32) * 8;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        a[j] = ap[j];
+        b[j] = bp[j];
+      }
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+    __syncthreads();
+  }
+  const int col = n0 + wn * 32 + lane % 32;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int row = m0 + wm * 32 + (r / 4) * 8 + (lane / 32) * 4 + r % 4;
+    if (row < g.M && col < g.N) {
+      float v = acc[r];
+      if (g.bias) v += g.bias[col];
+      if (g.gelu) {
+        g.Z[(int64_t)row * g.ldc + col] = v;
+        v = gelu(v);
+      }
+      g.C[(int64_t)row * g.ldc + col] = v;
+    }
+  }
+}
+
+__global__ void gelu_backward_kernel(float* __restrict__ d, const float* __restrict__ z,
+                                     int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    d[i] *= gelu_grad(z[i]);
+}
+
+// dY = 2 (Y - T) / (M N); per-block partial sums of the squared error.
+__global__ void mse_kernel(const float* __restrict__ y, const float* __restrict__ t,
+                           float* __restrict__ dy, int64_t n, float scale,
+                           float* __restrict__ partial) {
+  __shared__ float s[256];
+  float acc = 0.0f;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float e = y[i] - t[i];
+    dy[i] = 2.0f * e * scale;
+    acc += e * e;
+  }
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1) {
+    if (threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
+}
+
+// db[n] = sum over rows of d[m][n].
+__global__ void colsum_kernel(const float* __restrict__ d, int M, int N, float* __restrict__ db) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= N) return;
+  float s = 0.0f;
+  for (int m = 0; m < M; ++m) s += d[(int64_t)m * N + n];
+  db[n] = s;
+}
+
+__global__ void gather_rows_kernel(const float* __restrict__ src, const int32_t* __restrict__ idx,
+                                   int rows, int cols, float* __restrict__ dst) {
+  const int64_t n = (int64_t)rows * cols;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(i / cols), c = (int)(i % cols);
+    dst[i] = src[(int64_t)idx[r] * cols + c];
+  }
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 master copy; refreshes the BF16 copy.
+__global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g,
+                            float* __restrict__ m1, float* __restrict__ m2,
+                            __bf16* __restrict__ pb, int64_t n, float lr, float b1,
+                            float b2, float eps, float c1, float c2) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const float gi = g[i];
+    const float a = b1 * m1[i] + (1.0f - b1) * gi;
+    const float b = b2 * m2[i] + (1.0f - b2) * gi * gi;
+    m1[i] = a;
+    m2[i] = b;
+    const float v = p[i] - lr * (a / c1) / (sqrtf(b / c2) + eps);
+    p[i] = v;
+    pb[i] = (__bf16)v;
+  }
+}
+
+__global__ void to_bf16_kernel(const float* __restrict__ p, __bf16* __restrict__ pb, int64_t n) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    pb[i] = (__bf16)p[i];
+}
+
+int grid_for(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 4096); }
+
+}  // namespace
+
+struct gcm_mlp {
+  int device = 0;
+  int n_in = 0, H = 0, n_out = 0;
+  hipStream_t stream = nullptr;
+  int64_t n_params = 0;
+  int64_t off_w1 = 0, off_b1 = 0, off_w2 = 0, off_b2 = 0, off_w3 = 0, off_b3 = 0;
+  float *p = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr;
+  __bf16* pb = nullptr;
+  int64_t adam_t = 0;
+  // activations for a batch of up to cap rows
+  int cap = 0;
+  float *x = nullptr, *z1 = nullptr, *h1 = nullptr, *z2 = nullptr, *h2 = nullptr,
+        *y = nullptr, *t = nullptr, *dy = nullptr, *d2 = nullptr, *d1 = nullptr,
+        *partial = nullptr;
+  int32_t* idx = nullptr;
+  // resident training set
+  float *dx = nullptr, *dt = nullptr;
+  int64_t n_data = 0;
+  std::vector<void*> allocs;
+
+  template <typename T>
+  int alloc(T** ptr, size_t count) {
+    *ptr = nullptr;
+    if (hipMalloc((void**)ptr, std::max<size_t>(1, count) * sizeof(T)) != hipSuccess)
+      return fail(GCM_ERR_NOMEM, "hipMalloc failed (mlp)");
+    allocs.push_back(*ptr);
+    return GCM_OK;
+  }
+  int ensure_batch(int rows) {
+    if (rows <= cap) return GCM_OK;
+    for (void* q : {(void*)x, (void*)z1, (void*)h1, (void*)z2, (void*)h2, (void*)y,
+                    (void*)t, (void*)dy, (void*)d2, (void*)d1, (void*)idx}) {
+      if (!q) continue;
+      (void)hipFree(q);
+      allocs.erase(std::remove(allocs.begin(), allocs.end(), q), allocs.end());
+    }
+    cap = rows;
+    int rc = GCM_OK;
+    auto step = [&](int r) { if (rc == GCM_OK) rc = r; };
+    step(alloc(&x, (size_t)rows * n_in));
+    step(alloc(&z1, (size_t)rows * H));
+    step(alloc(&h1, (size_t)rows * H));
+    step(alloc(&z2, (size_t)rows * H));
+    step(alloc(&h2, (size_t)rows * H));
+    step(alloc(&y, (size_t)rows * n_out));
+    step(alloc(&t, (size_t)rows * n_out));
+    step(alloc(&dy, (size_t)rows * n_out));
+    step(alloc(&d2, (size_t)rows * H));
+    step(alloc(&d1, (size_t)rows * H));
+    step(alloc(&idx, (size_t)rows));
+    return rc;
+  }
+};
+
+namespace {
+
+int gemm(gcm_mlp* m, int M, int N, int K, const void* A, int64_t a_rs, int64_t a_cs,
+         int a_bf16, const void* B, int64_t b_rs, int64_t b_cs, int b_bf16, float* C,
+         int64_t ldc, const float* bias, float* Z) {
+  GemmArgs g{M, N, K, A, a_rs, a_cs, a_bf16, B, b_rs, b_cs, b_bf16, C, ldc, bias, Z, Z != nullptr};
+  dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM);
+  gemm_bf16_kernel<<<grid, 256, 0, m->stream>>>(g);
+  MLP_HIP(hipGetLastError());
+  return GCM_OK;
+}
+
+// Forward on rows already in m->x: h1 = gelu(x W1^T + b1), h2 = gelu(h1 W2^T
+// + b2), y = h2 W3^T + b3.
+int forward(gcm_mlp* m, int rows) {
+  const __bf16* w = m->pb;
+  int rc = gemm(m, rows, m->H, m->n_in, m->x, m->n_in, 1, 0, w + m->off_w1, 1, m->n_in, 1,
+                m->h1, m->H, m->p + m->off_b1, m->z1);
+  if (rc) return rc;
+  rc = gemm(m, rows, m->H, m->H, m->h1, m->H, 1, 0, w + m->off_w2, 1, m->H, 1, m->h2, m->H,
+            m->p + m->off_b2, m->z2);
+  if (rc) return rc;
+  return gemm(m, rows, m->n_out, m->H, m->h2, m->H, 1, 0, w + m->off_w3, 1, m->H, 1, m->y,
+              m->n_out, m->p + m->off_b3, nullptr);
+}
+
+// Loss and gradients into m->g for the rows in m->x / m->t.
+int backward(gcm_mlp* m, int rows, double* loss) {
+  const int64_t ny = (int64_t)rows * m->n_out;
+  const int nb = grid_for(ny);
+  mse_kernel<<<nb, 256, 0, m->stream>>>(m->y, m->t, m->dy, ny, 1.0f / (float)ny, m->partial);
+  MLP_HIP(hipGetLastError());
+  const __bf16* w = m->pb;
+  float* g = m->g;
+  // dW3 = dY^T h2 : (n_out x rows) . (rows x H)
+  int rc = gemm(m, m->n_out, m->H, rows, m->dy, 1, m->n_out, 0, m->h2, m->H, 1, 0,
+                g + m->off_w3, m->H, nullptr, nullptr);
+  if (rc) return rc;
+  colsum_kernel<<<(m->n_out + 255) / 256, 256, 0, m->stream>>>(m->dy, rows, m->n_out, g + m->off_b3);
+  // dH2 = dY W3 : (rows x n_out) . (n_out x H)
+  rc = gemm(m, rows, m->H, m->n_out, m->dy, m->n_out, 1, 0, w + m->off_w3, m->H, 1, 1, m->d2,
+            m->H, nullptr, nullptr);
+  if (rc) return rc;
+  gelu_backward_kernel<<<grid_for((int64_t)rows * m->H), 256, 0, m->stream>>>(m->d2, m->z2, (int64_t)rows * m->H);
+  rc = gemm(m, m->H, m->H, rows, m->d2, 1, m->H, 0, m->h1, m->H, 1, 0, g + m->off_w2, m->H,
+            nullptr, nullptr);
+  if (rc) return rc;
+  colsum_kernel<<<(m->H + 255) / 256, 256, 0, m->stream>>>(m->d2, rows, m->H, g + m->off_b2);
+  rc = gemm(m, rows, m->H, m->H, m->d2, m->H, 1, 0, w + m->off_w2, m->H, 1, 1, m->d1, m->H,
+            nullptr, nullptr);
+  if (rc) return rc;
+  gelu_backward_kernel<<<grid_for((int64_t)rows * m->H), 256, 0, m->stream>>>(m->d1, m->z1, (int64_t)rows * m->H);
+  rc = gemm(m, m->H, m->n_in, rows, m->d1, 1, m->H, 0, m->x, m->n_in, 1, 0, g + m->off_w1,
+            m->n_in, nullptr, nullptr);
+  if (rc) return rc;
+  colsum_kernel<<<(m->H + 255) / 256, 256, 0, m->stream>>>(m->d1, rows, m->H, g + m->off_b1);
+  MLP_HIP(hipGetLastError());
+  if (loss) {
+    std::vector<float> part(nb);
+    MLP_HIP(hipMemcpyAsync(part.data(), m->partial, nb * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+    MLP_HIP(hipStreamSynchronize(m->stream));
+    double s = 0;
+    for (float v : part) s += v;
+    *loss = s / (double)ny;
+  }
+  return GCM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int gcm_mlp_create(int device, int n_in, int n_hidden, int n_out, uint64_t seed,
+                   gcm_mlp** out) {
+  if (!out || n_in < 1 || n_hidden < 1 || n_out < 1) return fail(GCM_ERR_ARG, "invalid MLP shape");
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n)
+    return fail(GCM_ERR_HIP, "no such HIP device");
+  MLP_HIP(hipSetDevice(device));
+  auto* m = new gcm_mlp();
+  m->device = device;
+  m->n_in = n_in;
+  m->H = n_hidden;
+  m->n_out = n_out;
+  m->off_w1 = 0;
+  m->off_b1 = m->off_w1 + (int64_t)n_hidden * n_in;
+  m->off_w2 = m->off_b1 + n_hidden;
+  m->off_b2 = m->off_w2 + (int64_t)n_hidden * n_hidden;
+  m->off_w3 = m->off_b2 + n_hidden;
+  m->off_b3 = m->off_w3 + (int64_t)n_out * n_hidden;
+  m->n_params = m->off_b3 + n_out;
+  int rc = GCM_OK;
+  auto step = [&](int r) { if (rc == GCM_OK) rc = r; };
+  step(m->alloc(&m->p, m->n_params));
+  step(m->alloc(&m->g, m->n_params));
+  step(m->alloc(&m->m1, m->n_params));
+  step(m->alloc(&m->m2, m->n_params));
+  step(m->alloc(&m->pb, m->n_params));
+  step(m->alloc(&m->partial, 4096));
+  if (rc == GCM_OK && hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess)
+    rc = fail(GCM_ERR_HIP, "stream creation failed");
+  if (rc != GCM_OK) {
+    for (void* q : m->allocs) (void)hipFree(q);
+    delete m;
+    return rc;
+  }
+  // He-style initialisation, zero biases.
+  std::vector<float> host(m->n_params, 0.0f);
+  std::mt19937_64 rng(seed);
+  auto init = [&](int64_t off, int64_t count, int fan_in) {
+    std::normal_distribution<float> d(0.0f, std::sqrt(2.0f / fan_in));
+    for (int64_t i = 0; i < count; ++i) host[off + i] = d(rng);
+  };
+  init(m->off_w1, (int64_t)n_hidden * n_in, n_in);
+  init(m->off_w2, (int64_t)n_hidden * n_hidden, n_hidden);
+  init(m->off_w3, (int64_t)n_out * n_hidden, n_hidden);
+  MLP_HIP(hipMemcpy(m->p, host.data(), m->n_params * 4, hipMemcpyHostToDevice));
+  MLP_HIP(hipMemset(m->m1, 0, m->n_params * 4));
+  MLP_HIP(hipMemset(m->m2, 0, m->n_params * 4));
+  to_bf16_kernel<<<grid_for(m->n_params), 256, 0, m->stream>>>(m->p, m->pb, m->n_params);
+  MLP_HIP(hipStreamSynchronize(m->stream));
+  *out = m;
+  return GCM_OK;
+}
+
+int gcm_mlp_destroy(gcm_mlp* m) {
+  if (!m) return GCM_OK;
+  (void)hipSetDevice(m->device);
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  for (void* q : m->allocs) (void)hipFree(q);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+  return GCM_OK;
+}
+
+int gcm_mlp_param_count(gcm_mlp* m, int64_t* count) {
+  if (!m || !count) return fail(GCM_ERR_ARG, "NULL argument");
+  *count = m->n_params;
+  return GCM_OK;
+}
+
+int gcm_mlp_get_params(gcm_mlp* m, float* out) {
+  if (!m || !out) return fail(GCM_ERR_ARG, "NULL argument");
+  MLP_HIP(hipSetDevice(m->device));
+  MLP_HIP(hipStreamSynchronize(m->stream));
+  MLP_HIP(hipMemcpy(out, m->p, m->n_params * 4, hipMemcpyDeviceToHost));
+  return GCM_OK;
+}
+
+int gcm_mlp_set_params(gcm_mlp* m, const float* in) {
+  if (!m || !in) return fail(GCM_ERR_ARG, "NULL argument");
+  MLP_HIP(hipSetDevice(m->device));
+  MLP_HIP(hipStreamSynchronize(m->stream));
+  MLP_HIP(hipMemcpy(m->p, in, m->n_params * 4, hipMemcpyHostToDevice));
+  to_bf16_kernel<<<grid_for(m->n_params), 256, 0, m->stream>>>(m->p, m->pb, m->n_params);
+  MLP_HIP(hipStreamSynchronize(m->stream));
+  return GCM_OK;
+}
+
+int gcm_mlp_predict(gcm_mlp* m, const float* x, int64_t rows, float* y) {
+  if (!m || (rows && (!x || !y)) || rows < 0) return fail(GCM_ERR_ARG, "invalid arguments");
+  MLP_HIP(hipSetDevice(m->device));
+  const int chunk = 8192;
+  int rc = m->ensure_batch((int)std::min<int64_t>(rows, chunk));
+  if (rc) return rc;
+  for (int64_t r0 = 0; r0 < rows; r0 += chunk) {
+    const int n = (int)std::min<int64_t>(chunk, rows - r0);
+    MLP_HIP(hipMemcpyAsync(m->x, x + r0 * m->n_in, (size_t)n * m->n_in * 4, hipMemcpyHostToDevice, m->stream));
+    if ((rc = forward(m, n))) return rc;
+    MLP_HIP(hipMemcpyAsync(y + r0 * m->n_out, m->y, (size_t)n * m->n_out * 4, hipMemcpyDeviceToHost, m->stream));
+  }
+  MLP_HIP(hipStreamSynchronize(m->stream));
+  return GCM_OK;
+}
+
+// Loss and gradient of the mean squared error on the given rows (no update).
+int gcm_mlp_gradients(gcm_mlp* m, const float* x, const float* t, int rows, float* grad,
+                      double* loss) {
+  if (!m || !x || !t || rows < 1) return fail(GCM_ERR_ARG, "invalid arguments");
+  MLP_HIP(hipSetDevice(m->device));
+  int rc = m->ensure_batch(rows);
+  if (rc) return rc;
+  MLP_HIP(hipMemcpyAsync(m->x, x, (size_t)rows * m->n_in * 4, hipMemcpyHostToDevice, m->stream));
+  MLP_HIP(hipMemcpyAsync(m->t, t, (size_t)rows * m->n_out * 4, hipMemcpyHostToDevice, m->stream));
+  if ((rc = forward(m, rows))) return rc;
+  if ((rc = backward(m, rows, loss))) return rc;
+  if (grad) {
+    MLP_HIP(hipMemcpyAsync(grad, m->g, m->n_params * 4, hipMemcpyDeviceToHost, m->stream));
+    MLP_HIP(hipStreamSynchronize(m->stream));
+  }
+  return GCM_OK;
+}
+
+// Upload a training set (rows of normalised inputs and targets).
+int gcm_mlp_set_data(gcm_mlp* m, const float* x, const float* t, int64_t rows) {
+  if (!m || !x || !t || rows < 1) return fail(GCM_ERR_ARG, "invalid arguments");
+  MLP_HIP(hipSetDevice(m->device));
+  MLP_HIP(hipStreamSynchronize(m->stream));
+  if (m->dx) {
+    (void)hipFree(m->dx);
+    (void)hipFree(m->dt);
+    m->allocs.erase(std::remove(m->allocs.begin(), m->allocs.end(), (void*)m->dx), m->allocs.end());
+    m->allocs.erase(std::remove(m->allocs.begin(), m->allocs.end(), (void*)m->dt), m->allocs.end());
+    m->dx = m->dt = nullptr;
+  }
+  int rc = m->alloc(&m->dx, (size_t)rows * m->n_in);
+  if (rc == GCM_OK) rc = m->alloc(&m->dt, (size_t)rows * m->n_out);
+  if (rc) return rc;
+  MLP_HIP(hipMemcpy(m->dx, x, (size_t)rows * m->n_in * 4, hipMemcpyHostToDevice));
+  MLP_HIP(hipMemcpy(m->dt, t, (size_t)rows * m->n_out * 4, hipMemcpyHostToDevice));
+  m->n_data = rows;
+  return GCM_OK;
+}
+
+// Adam epochs over the resident training set in shuffled minibatches; the
+// mean minibatch loss of each epoch goes to loss_out[epoch] (optional).
+int gcm_mlp_train(gcm_mlp* m, int epochs, int batch, float lr, uint64_t seed,
+                  double* loss_out) {
+  if (!m || epochs < 0 || batch < 1) return fail(GCM_ERR_ARG, "invalid arguments");
+  if (!m->dx) return fail(GCM_ERR_STATE, "gcm_mlp_set_data first");
+  MLP_HIP(hipSetDevice(m->device));
+  int rc = m->ensure_batch(batch);
+  if (rc) return rc;
+  std::vector<int32_t> order(m->n_data);
+  for (int64_t i = 0; i < m->n_data; ++i) order[i] = (int32_t)i;
+  std::mt19937_64 rng(seed);
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  for (int ep = 0; ep < epochs; ++ep) {
+    std::shuffle(order.begin(), order.end(), rng);
+    double acc = 0;
+    int nbatch = 0, nsampled = 0;
+    for (int64_t r0 = 0; r0 + batch <= m->n_data; r0 += batch, ++nbatch) {
+      MLP_HIP(hipMemcpyAsync(m->idx, order.data() + r0, batch * 4, hipMemcpyHostToDevice, m->stream));
+      gather_rows_kernel<<<grid_for((int64_t)batch * m->n_in), 256, 0, m->stream>>>(m->dx, m->idx, batch, m->n_in, m->x);
+      gather_rows_kernel<<<grid_for((int64_t)batch * m->n_out), 256, 0, m->stream>>>(m->dt, m->idx, batch, m->n_out, m->t);
+      if ((rc = forward(m, batch))) return rc;
+      // the loss is fetched (a synchronisation) on every 16th minibatch only
+      const bool want = loss_out != nullptr && (nbatch % 16) == 0;
+      double l = 0;
+      if ((rc = backward(m, batch, want ? &l : nullptr))) return rc;
+      if (want) {
+        acc += l;
+        ++nsampled;
+      }
+      m->adam_t++;
+      const float c1 = 1.0f - std::pow(b1, (float)m->adam_t);
+      const float c2 = 1.0f - std::pow(b2, (float)m->adam_t);
+      adam_kernel<<<grid_for(m->n_params), 256, 0, m->stream>>>(m->p, m->g, m->m1, m->m2, m->pb,
+                                                                m->n_params, lr, b1, b2, eps, c1, c2);
+      MLP_HIP(hipGetLastError());
+    }
+    if (loss_out) loss_out[ep] = acc / std::max(1, nsampled);
+  }
+  MLP_HIP(hipStreamSynchronize(m->stream));
+  return GCM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,197 @@
+"""Metropolis chains on the exact likelihood of the cache, plain and with the
+surrogate as the first stage of delayed acceptance.
+
+Target: log pi(theta) = -LLH_total(theta) / 2, as GUNDAM's SimpleMcmc proxy
+(SimpleMcmc.h:255-267: -0.5 x evalFit).  Proposals are Gaussian random walks
+with a covariance adapted during burn-in (Haario, Saksman & Tamminen 2001,
+Bernoulli 7:223) and frozen afterwards.
+
+Delayed acceptance (Christen & Fox 2005, J. Comput. Graph. Stat. 14:795): a
+proposal passes stage 1 with probability min(1, exp(s(theta') - s(theta)))
+where s is the surrogate log target; only then is the exact log target L
+evaluated, and the move is accepted with probability
+min(1, exp((L(theta') - L(theta)) - (s(theta') - s(theta)))).  The chain's
+stationary distribution is the exact target; the surrogate only decides
+which proposals are worth an exact evaluation.  Many chains run in lockstep,
+so each iteration is one batched MLP forward and one batched exact launch.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass
+from typing import Callable, Optional
+
+import numpy as np
+
+from .cache_manager import CacheManager
+from .surrogate import Surrogate, barlow_llh
+
+
+class ExactTarget:
+    """-LLH/2 from the device path, K points per launch."""
+
+    def __init__(self, cm: CacheManager, table_values: Optional[Callable] = None):
+        self.cm = cm
+        self.table_values = table_values
+        self.calls = 0
+        self.points = 0
+
+    def __call__(self, thetas: np.ndarray) -> np.ndarray:
+        thetas = np.atleast_2d(thetas)
+        out = np.empty(thetas.shape[0])
+        K = self.cm.max_points
+        for i in range(0, thetas.shape[0], K):
+            th = thetas[i:i + K]
+            self.cm.set_parameters(th)
+            if self.table_values is not None and self.cm.table_len:
+                self.cm.set_table_values(self.table_values(th))
+            self.cm.fill()
+            out[i:i + th.shape[0]] = -0.5 * self.cm.llh()[0][:th.shape[0]]
+            self.calls += 1
+            self.points += th.shape[0]
+        return out
+
+    def sums(self, thetas: np.ndarray) -> np.ndarray:
+        """Bin contents for each point (training pairs for the surrogate)."""
+        thetas = np.atleast_2d(thetas)
+        out = np.empty((thetas.shape[0], self.cm.n_bins))
+        K = self.cm.max_points
+        for i in range(0, thetas.shape[0], K):
+            th = thetas[i:i + K]
+            self.cm.set_parameters(th)
+            if self.table_values is not None and self.cm.table_len:
+                self.cm.set_table_values(self.table_values(th))
+            self.cm.fill().get()
+            for k in range(th.shape[0]):
+                out[i + k] = self.cm.hist(k)[0]
+        return out
+
+
+class SurrogateTarget:
+    """-LLH/2 with the surrogate's predicted bin contents: Barlow-Beeston
+    against the data with MC variance = predicted content x the nominal
+    (sum w^2 / sum w) ratio of each bin, plus the diagonal prior penalty."""
+
+    def __init__(self, sur: Surrogate, data: np.ndarray, var_ratio: np.ndarray,
+                 prior_mean: np.ndarray, prior_sigma: np.ndarray):
+        self.sur = sur
+        self.data = data
+        self.ratio = var_ratio
+        self.mu = prior_mean
+        self.sig = np.where(prior_sigma > 0, prior_sigma, np.inf)
+        self.points = 0
+
+    def __call__(self, thetas: np.ndarray) -> np.ndarray:
+        thetas = np.atleast_2d(thetas)
+        pred = self.sur.predict_sums(thetas)
+        stat = barlow_llh(self.data, pred, pred * self.ratio)
+        pen = (((thetas - self.mu) / self.sig) ** 2).sum(1)
+        self.points += thetas.shape[0]
+        return -0.5 * (stat + pen)
+
+
+def laplace_scales(exact: ExactTarget, mode: np.ndarray, h: float = 1e-2) -> np.ndarray:
+    """Per-parameter posterior widths from the diagonal curvature of the exact
+    log target at the mode (one batched stencil of 2P + 1 points)."""
+    P = mode.size
+    pts = np.repeat(mode[None, :], 2 * P + 1, 0)
+    for j in range(P):
+        pts[1 + 2 * j, j] += h
+        pts[2 + 2 * j, j] -= h
+    L = exact(pts)
+    curv = -(L[1::2] + L[2::2] - 2 * L[0]) / (h * h)
+    return 1.0 / np.sqrt(np.maximum(curv, 1e-12))
+
+
+@dataclass
+class ChainResult:
+    samples: np.ndarray          # [steps, chains, P]
+    accepted: int                # accepted moves (all chains)
+    proposals: int               # proposals (all chains)
+    exact_evals: int             # exact likelihood evaluations
+    stage1_passed: int
+    seconds: float
+
+    @property
+    def accepted_per_s(self) -> float:
+        return self.accepted / self.seconds
+
+
+def adapt_cov(samples: np.ndarray, scale0: np.ndarray) -> np.ndarray:
+    d = samples.shape[-1]
+    flat = samples.reshape(-1, d)
+    cov = np.cov(flat.T) if flat.shape[0] > 2 * d else np.diag(scale0 ** 2)
+    return (2.38 ** 2 / d) * cov + 1e-10 * np.eye(d)
+
+
+def run_plain(exact: ExactTarget, theta0: np.ndarray, cov: np.ndarray, steps: int,
+              rng: np.random.Generator) -> ChainResult:
+    """One chain, one exact evaluation per proposal (the plain GPU chain)."""
+    P = theta0.size
+    Lc = np.linalg.cholesky(cov)
+    th = theta0.copy()
+    L = exact(th[None, :])[0]
+    out = np.empty((steps, 1, P))
+    acc = 0
+    t0 = time.perf_counter()
+    for s in range(steps):
+        prop = th + Lc @ rng.standard_normal(P)
+        Lp = exact(prop[None, :])[0]
+        if np.log(rng.uniform()) < Lp - L:
+            th, L = prop, Lp
+            acc += 1
+        out[s, 0] = th
+    return ChainResult(out, acc, steps, steps, steps, time.perf_counter() - t0)
+
+
+def run_delayed(exact: ExactTarget, surr: SurrogateTarget, theta0: np.ndarray,
+                cov: np.ndarray, steps: int, rng: np.random.Generator) -> ChainResult:
+    """C chains in lockstep with surrogate screening (theta0: [C, P])."""
+    C_, P = theta0.shape
+    Lc = np.linalg.cholesky(cov)
+    th = theta0.copy()
+    L = exact(th)
+    S = surr(th)
+    out = np.empty((steps, C_, P))
+    acc = passed = n_exact = 0
+    t0 = time.perf_counter()
+    for s in range(steps):
+        prop = th + rng.standard_normal((C_, P)) @ Lc.T
+        Sp = surr(prop)
+        stage1 = np.log(rng.uniform(size=C_)) < (Sp - S)
+        idx = np.nonzero(stage1)[0]
+        passed += idx.size
+        if idx.size:
+            Lp = exact(prop[idx])
+            n_exact += idx.size
+            a2 = np.log(rng.uniform(size=idx.size)) < (Lp - L[idx]) - (Sp[idx] - S[idx])
+            take = idx[a2]
+            th[take] = prop[take]
+            L[take] = Lp[a2]
+            S[take] = Sp[take]
+            acc += take.size
+        out[s] = th
+    return ChainResult(out, acc, steps * C_, n_exact, passed, time.perf_counter() - t0)
+
+
+def run_batched_exact(exact: ExactTarget, theta0: np.ndarray, cov: np.ndarray,
+                      steps: int, rng: np.random.Generator) -> ChainResult:
+    """C chains in lockstep, every proposal evaluated exactly (one batched
+    launch per step) -- the reference for the surrogate's contribution."""
+    C_, P = theta0.shape
+    Lc = np.linalg.cholesky(cov)
+    th = theta0.copy()
+    L = exact(th)
+    out = np.empty((steps, C_, P))
+    acc = 0
+    t0 = time.perf_counter()
+    for s in range(steps):
+        prop = th + rng.standard_normal((C_, P)) @ Lc.T
+        Lp = exact(prop)
+        a = np.log(rng.uniform(size=C_)) < (Lp - L)
+        th[a] = prop[a]
+        L[a] = Lp[a]
+        acc += int(a.sum())
+        out[s] = th
+    return ChainResult(out, acc, steps * C_, steps * C_, steps * C_,
+                       time.perf_counter() - t0)

@@ -226,6 +226,29 @@ int gcm_bin_events(int device, int64_t n_events, int32_t n_vars,
                    const double* values, int32_t n_bins, const double* edges,
                    int32_t* out_bin);
 
+/* ---- learned surrogate (no reference counterpart: the north star's MLP) --
+ * An MLP n_in -> n_hidden -> n_hidden -> n_out with GELU, BF16 weights and
+ * FP32 accumulation on MFMA, trained with Adam on the mean squared error.
+ * Parameter layout (FP32 master copy): W1[h][n_in] b1[h] W2[h][h] b2[h]
+ * W3[n_out][h] b3[n_out].  Inputs and targets are normalised by the caller. */
+typedef struct gcm_mlp gcm_mlp;
+int gcm_mlp_create(int device, int n_in, int n_hidden, int n_out, uint64_t seed,
+                   gcm_mlp** out);
+int gcm_mlp_destroy(gcm_mlp* m);
+int gcm_mlp_param_count(gcm_mlp* m, int64_t* count);
+int gcm_mlp_get_params(gcm_mlp* m, float* out);
+int gcm_mlp_set_params(gcm_mlp* m, const float* in);
+/* y[rows][n_out] = MLP(x[rows][n_in]) (host arrays). */
+int gcm_mlp_predict(gcm_mlp* m, const float* x, int64_t rows, float* y);
+/* Mean squared error and its gradient w.r.t. every parameter, no update. */
+int gcm_mlp_gradients(gcm_mlp* m, const float* x, const float* t, int rows,
+                      float* grad, double* loss);
+/* Resident training set and Adam epochs over it (shuffled minibatches);
+ * loss_out[epoch] (optional) is the mean of sampled minibatch losses. */
+int gcm_mlp_set_data(gcm_mlp* m, const float* x, const float* t, int64_t rows);
+int gcm_mlp_train(gcm_mlp* m, int epochs, int batch, float lr, uint64_t seed,
+                  double* loss_out);
+
 #ifdef __cplusplus
 }
 #endif
