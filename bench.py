@@ -133,6 +133,117 @@ def batched_workload3(args, log):
             "launch": cm.launch_config()}
 
 
+def surrogate_da(args, cm, wl, data, err, log):
+    """Delayed-acceptance MCMC with the learned surrogate on the 5M-event
+    model, against the plain exact GPU chain.
+
+    Training pairs come from the exact path during burn-in: 32 batched exact
+    chains started from Laplace draws (every evaluated point is kept), topped
+    up with draws from the Laplace approximation.  The network is
+    n_par -> 512 -> 512 -> n_bins (GELU, BF16 MFMA).  Each method then runs
+    for the same wall time from the burn-in end states with the adapted
+    proposal covariance.  "Exact-equivalent accepted samples/s" is the
+    squared-jump rate (Mahalanobis in the Laplace scales) divided by the
+    plain chain's squared jump per accepted move: the number of plain-chain
+    accepted moves that cover the same distance.  Raw accepted moves/s are
+    reported beside it."""
+    from gundam_amd.mcmc import (ExactTarget, SurrogateTarget, adapt_cov, laplace_scales,
+                                 run_batched_exact, run_delayed, run_plain, var_ratio)
+    from gundam_amd.surrogate import Surrogate
+
+    tv = wl.table_values if wl.tables.table_len else None
+    mode = wl.nominal()
+    d = mode.size
+    ex = ExactTarget(cm, tv)
+    sc = laplace_scales(ex, mode)
+    rng = np.random.default_rng(args.seed + 41)
+    C_ = 32
+    t0 = time.perf_counter()
+    ex.record = True
+    burn = run_batched_exact(ex, mode + sc * rng.standard_normal((C_, d)),
+                             (2.38 ** 2 / d) * np.diag(sc ** 2), args.c5_burn, rng)
+    extra = args.c5_pairs - sum(a.shape[0] for a in ex.rec_theta)
+    if extra > 0:
+        ex(mode + sc * rng.standard_normal((extra, d)))
+    ex.record = False
+    X, Y = ex.recorded()
+    gen_s = time.perf_counter() - t0
+    cov = adapt_cov(burn.samples[burn.samples.shape[0] // 2:], sc)
+    log(f"c5: {X.shape[0]} training pairs in {gen_s:.1f}s (burn-in acceptance "
+        f"{burn.accepted / burn.proposals:.3f})")
+    sur = Surrogate(d, wl.tables.n_bins, hidden=512, seed=args.seed + 2)
+    t0 = time.perf_counter()
+    losses = sur.fit(X, Y, epochs=args.c5_epochs, batch=256, lr=1e-3)
+    train_s = time.perf_counter() - t0
+    del X, Y
+    ex.rec_theta, ex.rec_sums = [], []
+    surr = SurrogateTarget(sur, data, var_ratio(data, err), wl.prior_mean, wl.prior_sigma)
+    test = mode + sc * rng.standard_normal((512, d))
+    dchi2 = 2.0 * (surr(test) - ex(test))
+    cen = np.abs(dchi2 - np.median(dchi2))
+    log(f"c5: trained {args.c5_epochs} epochs in {train_s:.1f}s, loss {losses[0]:.3f} -> "
+        f"{losses[-1]:.4f}; surrogate - exact chi2: offset {np.median(dchi2):.2f}, "
+        f"|centred| p50 {np.median(cen):.2f} p90 {np.quantile(cen, 0.9):.2f}")
+    start = burn.samples[-1]
+    T = args.c5_seconds
+    big = 10 ** 7
+    res = {
+        "plain": run_plain(ex, start[0].copy(), cov, big, rng, max_seconds=T),
+        "batched_exact": run_batched_exact(ex, start.copy(), cov, big, rng, max_seconds=T),
+        "da": run_delayed(ex, surr, start.copy(), cov, big, rng, n_sub=1, max_seconds=T),
+        "da_sub": run_delayed(ex, surr, start.copy(), cov, big, rng, n_sub=args.c5_sub,
+                              max_seconds=T),
+    }
+    plain = res["plain"]
+    jump_per_acc = plain.sq_jumps(sc) / max(1, plain.accepted)
+    out = {}
+    for k, r in res.items():
+        sj = r.sq_jumps(sc) / r.seconds
+        out[k] = {"accepted_per_s": r.accepted_per_s,
+                  "exact_equivalent_accepted_per_s": sj / jump_per_acc,
+                  "exact_evals_per_s": r.exact_evals / r.seconds,
+                  "exact_evals_per_accepted": r.exact_evals / max(1, r.accepted),
+                  "stage2_acceptance": r.accepted / max(1, r.stage1_passed),
+                  "iterations": int(r.samples.shape[0]), "chains": int(r.samples.shape[1]),
+                  "seconds": r.seconds}
+        log(f"c5 {k}: {r.accepted_per_s:.1f} accepted/s, "
+            f"{sj / jump_per_acc:.1f} exact-equivalent/s, "
+            f"{r.exact_evals / max(1, r.accepted):.2f} exact evals per accepted move")
+    out["da"]["n_sub"] = 1
+    out["da_sub"]["n_sub"] = args.c5_sub
+    base = out["plain"]["exact_equivalent_accepted_per_s"]
+    # posterior means of the delayed-acceptance chains against the batched
+    # exact chains (second halves), in Laplace sigmas
+    def mean2(r):
+        return r.samples[r.samples.shape[0] // 2:].reshape(-1, d).mean(0)
+    ref = mean2(res["batched_exact"])
+    agree = {k: {"median": float(np.median(np.abs(mean2(res[k]) - ref) / sc)),
+                 "max": float(np.max(np.abs(mean2(res[k]) - ref) / sc))}
+             for k in ("da", "da_sub")}
+    b = res["batched_exact"].samples
+    h = b.shape[1] // 2
+    split = np.abs(b[b.shape[0] // 2:, :h].reshape(-1, d).mean(0) -
+                   b[b.shape[0] // 2:, h:].reshape(-1, d).mean(0)) / sc
+    return {
+        "model": f"{d} -> 512 -> 512 -> {wl.tables.n_bins} (GELU, BF16 MFMA, FP32 accumulate)",
+        "training_pairs": int(args.c5_pairs), "pair_generation_s": gen_s,
+        "epochs": args.c5_epochs, "train_s": train_s,
+        "loss_first_last": [float(losses[0]), float(losses[-1])],
+        "surrogate_minus_exact_chi2": {"offset": float(np.median(dchi2)),
+                                       "p50_abs_centred": float(np.median(cen)),
+                                       "p90_abs_centred": float(np.quantile(cen, 0.9))},
+        "methods": out,
+        "gain_exact_equivalent": {k: out[k]["exact_equivalent_accepted_per_s"] / base
+                                  for k in ("batched_exact", "da", "da_sub")},
+        "gain_accepted_per_s": {k: out[k]["accepted_per_s"] / out["plain"]["accepted_per_s"]
+                                for k in ("batched_exact", "da", "da_sub")},
+        "posterior_mean_diff_sigma": agree,
+        "exact_split_half_diff_sigma": {"median": float(np.median(split)),
+                                        "max": float(np.max(split))},
+        "seconds_per_method": T,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,6 +265,13 @@ def main():
     ap.add_argument("--no-c3", action="store_true",
                     help="skip the 100k-event K=64 batched measurement")
     ap.add_argument("--tune-events", type=int, default=500000)
+    ap.add_argument("--no-c5", action="store_true",
+                    help="skip the surrogate delayed-acceptance MCMC leg")
+    ap.add_argument("--c5-pairs", type=int, default=16000)
+    ap.add_argument("--c5-burn", type=int, default=120)
+    ap.add_argument("--c5-epochs", type=int, default=100)
+    ap.add_argument("--c5-sub", type=int, default=8)
+    ap.add_argument("--c5-seconds", type=float, default=10.0)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -203,7 +321,7 @@ def main():
         f"{cm.resident_memory()/2**30:.2f} GiB, launch {cm.launch_config()}")
 
     cm.propagate_parameters(nominal, wl.table_values(nominal))
-    data = cm.hist(0)[0].copy()  # Asimov data at nominal
+    data, data_err = (a.copy() for a in cm.hist(0))  # Asimov data at nominal
     cm.set_likelihood("barlow", data, wl.prior_mean, wl.prior_sigma)
 
     n_pts = args.warmup + args.steps
@@ -263,6 +381,10 @@ def main():
 
     batched_c3 = None if args.no_c3 else batched_workload3(args, log)
 
+    c5 = None
+    if not args.no_c5 and args.batch >= 32:
+        c5 = surrogate_da(args, cm, wl, data, data_err, log)
+
     cpu = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
@@ -314,6 +436,7 @@ def main():
         },
         "batched": batched,
         "batched_c3": batched_c3,
+        "surrogate_da": c5,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -128,6 +128,8 @@ GCM_SIGNATURES = {
     "gcm_mlp_gradients": (C.c_int, [_P, _P, _P, C.c_int, _P, C.POINTER(_D)]),
     "gcm_mlp_set_data": (C.c_int, [_P, _P, _P, _I64]),
     "gcm_mlp_train": (C.c_int, [_P, C.c_int, C.c_int, C.c_float, C.c_uint64, _P]),
+    "gcm_mlp_set_head": (C.c_int, [_P, _P, _P, _P, _P]),
+    "gcm_mlp_predict_llh": (C.c_int, [_P, _P, _I64, _P]),
 }
 
 SYNTH_SIGNATURES = {

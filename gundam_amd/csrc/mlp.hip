@@ -210,6 +210,58 @@ __global__ void to_bf16_kernel(const float* __restrict__ p, __bf16* __restrict__
 
 int grid_for(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 4096); }
 
+// Surrogate likelihood head: one block per row.  Bin b's predicted content is
+// pred = exp(mean[b] + scale[b] * y[b]) (mean = -inf marks a bin with no
+// events: pred = 0), its MC variance pred * ratio[b], and the row's chi2 is
+// the Barlow-Beeston OA2021 sum over bins (BarlowBeestonBanff2022.h:150-294
+// with the default options: frac limit 1e150, expected minimum -1, penalty
+// included), reduced in a fixed tree order.
+constexpr int kHeadBlock = 256;
+
+__device__ double bb_chi2(double data, double pred, double mcvar) {
+  const double inf = __builtin_huge_val();
+  if (!isfinite(mcvar) || mcvar < 0.0) return inf;
+  if (pred <= -1.0) return (pred <= 0.0 && data > 0.0) ? inf : 0.0;
+  const double dmin = 2.2250738585072014e-308;
+  const double deps = 2.220446049250313e-16;
+  double newmc = pred, penalty = 0.0;
+  if (mcvar > 0.0) {
+    const double f = fmin(sqrt(mcvar) / pred, 1e150);
+    const double temp = pred * f * f - 1;
+    const double temp2 = temp * temp + 4 * data * f * f;
+    if (temp2 < 0) return inf;
+    const double beta = (-temp + sqrt(temp2)) / 2.;
+    newmc = fmax(pred * beta, dmin);
+    penalty = (beta - 1) * (beta - 1) / (2 * f * f);
+  }
+  double stat;
+  if (data <= deps) stat = newmc;
+  else if (newmc > dmin) stat = newmc - data + data * (log(data) - log(newmc));
+  else stat = dmin - data + data * (log(data) - log(dmin));
+  return 2.0 * stat + 2.0 * penalty;
+}
+
+__global__ void __launch_bounds__(kHeadBlock)
+head_llh_kernel(const float* __restrict__ y, int n_out, const double* __restrict__ mean,
+                const double* __restrict__ scale, const double* __restrict__ data,
+                const double* __restrict__ ratio, double* __restrict__ out) {
+  __shared__ double s[kHeadBlock];
+  const int row = blockIdx.x;
+  const float* yr = y + (int64_t)row * n_out;
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < n_out; b += kHeadBlock) {
+    const double pred = exp(mean[b] + scale[b] * (double)yr[b]);
+    acc += bb_chi2(data[b], pred, pred * ratio[b]);
+  }
+  s[threadIdx.x] = acc;
+  __syncthreads();
+  for (int st = kHeadBlock / 2; st > 0; st >>= 1) {
+    if (threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[row] = s[0];
+}
+
 }  // namespace
 
 struct gcm_mlp {
@@ -230,6 +282,10 @@ struct gcm_mlp {
   // resident training set
   float *dx = nullptr, *dt = nullptr;
   int64_t n_data = 0;
+  // likelihood head: mean, scale, data, ratio (n_out each) and per-row chi2
+  double* head = nullptr;
+  double* chi2 = nullptr;
+  int chi2_cap = 0;
   std::vector<void*> allocs;
 
   template <typename T>
@@ -437,6 +493,56 @@ int gcm_mlp_predict(gcm_mlp* m, const float* x, int64_t rows, float* y) {
     MLP_HIP(hipMemcpyAsync(m->x, x + r0 * m->n_in, (size_t)n * m->n_in * 4, hipMemcpyHostToDevice, m->stream));
     if ((rc = forward(m, n))) return rc;
     MLP_HIP(hipMemcpyAsync(y + r0 * m->n_out, m->y, (size_t)n * m->n_out * 4, hipMemcpyDeviceToHost, m->stream));
+  }
+  MLP_HIP(hipStreamSynchronize(m->stream));
+  return GCM_OK;
+}
+
+// Likelihood head over the network outputs (see head_llh_kernel).
+int gcm_mlp_set_head(gcm_mlp* m, const double* mean, const double* scale,
+                     const double* data, const double* var_ratio) {
+  if (!m || !mean || !scale || !data || !var_ratio) return fail(GCM_ERR_ARG, "NULL argument");
+  MLP_HIP(hipSetDevice(m->device));
+  MLP_HIP(hipStreamSynchronize(m->stream));
+  if (!m->head) {
+    int rc = m->alloc(&m->head, (size_t)4 * m->n_out);
+    if (rc) return rc;
+  }
+  const size_t n = (size_t)m->n_out * sizeof(double);
+  MLP_HIP(hipMemcpy(m->head, mean, n, hipMemcpyHostToDevice));
+  MLP_HIP(hipMemcpy(m->head + m->n_out, scale, n, hipMemcpyHostToDevice));
+  MLP_HIP(hipMemcpy(m->head + 2 * (size_t)m->n_out, data, n, hipMemcpyHostToDevice));
+  MLP_HIP(hipMemcpy(m->head + 3 * (size_t)m->n_out, var_ratio, n, hipMemcpyHostToDevice));
+  return GCM_OK;
+}
+
+int gcm_mlp_predict_llh(gcm_mlp* m, const float* x, int64_t rows, double* chi2) {
+  if (!m || (rows && (!x || !chi2)) || rows < 0) return fail(GCM_ERR_ARG, "invalid arguments");
+  if (!m->head) return fail(GCM_ERR_STATE, "gcm_mlp_set_head first");
+  MLP_HIP(hipSetDevice(m->device));
+  const int chunk = 8192;
+  int rc = m->ensure_batch((int)std::min<int64_t>(rows, chunk));
+  if (rc) return rc;
+  if (m->chi2_cap < std::min<int64_t>(rows, chunk)) {
+    if (m->chi2) {
+      MLP_HIP(hipStreamSynchronize(m->stream));
+      (void)hipFree(m->chi2);
+      m->allocs.erase(std::remove(m->allocs.begin(), m->allocs.end(), (void*)m->chi2), m->allocs.end());
+      m->chi2 = nullptr;
+    }
+    m->chi2_cap = (int)std::min<int64_t>(rows, chunk);
+    if ((rc = m->alloc(&m->chi2, m->chi2_cap))) return rc;
+  }
+  const double* h = m->head;
+  const int64_t no = m->n_out;
+  for (int64_t r0 = 0; r0 < rows; r0 += chunk) {
+    const int n = (int)std::min<int64_t>(chunk, rows - r0);
+    MLP_HIP(hipMemcpyAsync(m->x, x + r0 * m->n_in, (size_t)n * m->n_in * 4, hipMemcpyHostToDevice, m->stream));
+    if ((rc = forward(m, n))) return rc;
+    head_llh_kernel<<<n, kHeadBlock, 0, m->stream>>>(m->y, m->n_out, h, h + no, h + 2 * no,
+                                                     h + 3 * no, m->chi2);
+    MLP_HIP(hipGetLastError());
+    MLP_HIP(hipMemcpyAsync(chi2 + r0, m->chi2, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, m->stream));
   }
   MLP_HIP(hipStreamSynchronize(m->stream));
   return GCM_OK;

@@ -14,6 +14,12 @@ min(1, exp((L(theta') - L(theta)) - (s(theta') - s(theta)))).  The chain's
 stationary distribution is the exact target; the surrogate only decides
 which proposals are worth an exact evaluation.  Many chains run in lockstep,
 so each iteration is one batched MLP forward and one batched exact launch.
+
+With n_sub > 1 the first stage is n_sub Metropolis steps on the surrogate
+target (the surrogate transition method, Liu 2001, Monte Carlo Strategies in
+Scientific Computing, Springer, sec. 9.4.3): that kernel is reversible with
+respect to the surrogate, so its endpoint is a valid proposal with the same
+second-stage ratio, and an exact evaluation buys up to n_sub surrogate moves.
 """
 from __future__ import annotations
 
@@ -30,11 +36,15 @@ from .surrogate import Surrogate, barlow_llh
 class ExactTarget:
     """-LLH/2 from the device path, K points per launch."""
 
-    def __init__(self, cm: CacheManager, table_values: Optional[Callable] = None):
+    def __init__(self, cm: CacheManager, table_values: Optional[Callable] = None,
+                 record: bool = False):
         self.cm = cm
         self.table_values = table_values
         self.calls = 0
         self.points = 0
+        self.record = record      # keep (theta, bin contents) of every point
+        self.rec_theta: list = []
+        self.rec_sums: list = []
 
     def __call__(self, thetas: np.ndarray) -> np.ndarray:
         thetas = np.atleast_2d(thetas)
@@ -47,9 +57,17 @@ class ExactTarget:
                 self.cm.set_table_values(self.table_values(th))
             self.cm.fill()
             out[i:i + th.shape[0]] = -0.5 * self.cm.llh()[0][:th.shape[0]]
+            if self.record:
+                self.rec_theta.append(th.copy())
+                self.rec_sums.append(np.stack([self.cm.hist(k)[0]
+                                               for k in range(th.shape[0])]))
             self.calls += 1
             self.points += th.shape[0]
         return out
+
+    def recorded(self):
+        """(theta, bin contents) of every recorded point, stacked."""
+        return np.concatenate(self.rec_theta), np.concatenate(self.rec_sums)
 
     def sums(self, thetas: np.ndarray) -> np.ndarray:
         """Bin contents for each point (training pairs for the surrogate)."""
@@ -70,24 +88,30 @@ class ExactTarget:
 class SurrogateTarget:
     """-LLH/2 with the surrogate's predicted bin contents: Barlow-Beeston
     against the data with MC variance = predicted content x the nominal
-    (sum w^2 / sum w) ratio of each bin, plus the diagonal prior penalty."""
+    (sum w^2 / sum w) ratio of each bin (evaluated on the device), plus the
+    diagonal prior penalty."""
 
     def __init__(self, sur: Surrogate, data: np.ndarray, var_ratio: np.ndarray,
                  prior_mean: np.ndarray, prior_sigma: np.ndarray):
         self.sur = sur
-        self.data = data
-        self.ratio = var_ratio
+        sur.set_head(data, var_ratio)
         self.mu = prior_mean
         self.sig = np.where(prior_sigma > 0, prior_sigma, np.inf)
         self.points = 0
+        self.calls = 0
 
     def __call__(self, thetas: np.ndarray) -> np.ndarray:
         thetas = np.atleast_2d(thetas)
-        pred = self.sur.predict_sums(thetas)
-        stat = barlow_llh(self.data, pred, pred * self.ratio)
+        stat = self.sur.chi2(thetas)
         pen = (((thetas - self.mu) / self.sig) ** 2).sum(1)
         self.points += thetas.shape[0]
+        self.calls += 1
         return -0.5 * (stat + pen)
+
+
+def var_ratio(sums: np.ndarray, sqrt_sum_w2: np.ndarray) -> np.ndarray:
+    """Per-bin (sum w^2 / sum w) at one point, 0 for empty bins."""
+    return np.where(sums > 0, sqrt_sum_w2 ** 2 / np.where(sums > 0, sums, 1.0), 0.0)
 
 
 def laplace_scales(exact: ExactTarget, mode: np.ndarray, h: float = 1e-2) -> np.ndarray:
@@ -116,6 +140,13 @@ class ChainResult:
     def accepted_per_s(self) -> float:
         return self.accepted / self.seconds
 
+    def sq_jumps(self, scales: np.ndarray) -> float:
+        """Sum over chains and steps of the squared jump ||d theta / scales||^2
+        (the expected-squared-jumping-distance numerator, Pasarica & Gelman
+        2010, Stat. Sinica 20:343)."""
+        d = np.diff(self.samples, axis=0) / scales
+        return float((d * d).sum())
+
 
 def adapt_cov(samples: np.ndarray, scale0: np.ndarray) -> np.ndarray:
     d = samples.shape[-1]
@@ -125,65 +156,83 @@ def adapt_cov(samples: np.ndarray, scale0: np.ndarray) -> np.ndarray:
 
 
 def run_plain(exact: ExactTarget, theta0: np.ndarray, cov: np.ndarray, steps: int,
-              rng: np.random.Generator) -> ChainResult:
-    """One chain, one exact evaluation per proposal (the plain GPU chain)."""
+              rng: np.random.Generator, max_seconds: Optional[float] = None) -> ChainResult:
+    """One chain, one exact evaluation per proposal (the plain GPU chain).
+    Stops after `steps` or once max_seconds have passed."""
     P = theta0.size
     Lc = np.linalg.cholesky(cov)
     th = theta0.copy()
     L = exact(th[None, :])[0]
-    out = np.empty((steps, 1, P))
+    out = []
     acc = 0
     t0 = time.perf_counter()
+    n = 0
     for s in range(steps):
         prop = th + Lc @ rng.standard_normal(P)
         Lp = exact(prop[None, :])[0]
         if np.log(rng.uniform()) < Lp - L:
             th, L = prop, Lp
             acc += 1
-        out[s, 0] = th
-    return ChainResult(out, acc, steps, steps, steps, time.perf_counter() - t0)
+        out.append(th[None, :].copy())
+        n = s + 1
+        if max_seconds is not None and time.perf_counter() - t0 > max_seconds:
+            break
+    return ChainResult(np.stack(out), acc, n, n, n, time.perf_counter() - t0)
 
 
 def run_delayed(exact: ExactTarget, surr: SurrogateTarget, theta0: np.ndarray,
-                cov: np.ndarray, steps: int, rng: np.random.Generator) -> ChainResult:
-    """C chains in lockstep with surrogate screening (theta0: [C, P])."""
+                cov: np.ndarray, steps: int, rng: np.random.Generator,
+                n_sub: int = 1, max_seconds: Optional[float] = None) -> ChainResult:
+    """C chains in lockstep with surrogate screening (theta0: [C, P]).  Each
+    iteration runs n_sub surrogate Metropolis steps per chain (one batched
+    surrogate call each) and sends the chains that moved to one batched exact
+    launch; samples are recorded once per iteration."""
     C_, P = theta0.shape
     Lc = np.linalg.cholesky(cov)
     th = theta0.copy()
     L = exact(th)
     S = surr(th)
-    out = np.empty((steps, C_, P))
-    acc = passed = n_exact = 0
+    out = []
+    acc = passed = n_exact = n = 0
     t0 = time.perf_counter()
     for s in range(steps):
-        prop = th + rng.standard_normal((C_, P)) @ Lc.T
-        Sp = surr(prop)
-        stage1 = np.log(rng.uniform(size=C_)) < (Sp - S)
-        idx = np.nonzero(stage1)[0]
+        y = th.copy()
+        Sy = S.copy()
+        for _ in range(n_sub):
+            prop = y + rng.standard_normal((C_, P)) @ Lc.T
+            Sp = surr(prop)
+            a1 = np.log(rng.uniform(size=C_)) < (Sp - Sy)
+            y[a1] = prop[a1]
+            Sy[a1] = Sp[a1]
+        idx = np.nonzero(np.any(y != th, axis=1))[0]
         passed += idx.size
         if idx.size:
-            Lp = exact(prop[idx])
+            Lp = exact(y[idx])
             n_exact += idx.size
-            a2 = np.log(rng.uniform(size=idx.size)) < (Lp - L[idx]) - (Sp[idx] - S[idx])
+            a2 = np.log(rng.uniform(size=idx.size)) < (Lp - L[idx]) - (Sy[idx] - S[idx])
             take = idx[a2]
-            th[take] = prop[take]
+            th[take] = y[take]
             L[take] = Lp[a2]
-            S[take] = Sp[take]
+            S[take] = Sy[take]
             acc += take.size
-        out[s] = th
-    return ChainResult(out, acc, steps * C_, n_exact, passed, time.perf_counter() - t0)
+        out.append(th.copy())
+        n = s + 1
+        if max_seconds is not None and time.perf_counter() - t0 > max_seconds:
+            break
+    return ChainResult(np.stack(out), acc, n * C_, n_exact, passed, time.perf_counter() - t0)
 
 
 def run_batched_exact(exact: ExactTarget, theta0: np.ndarray, cov: np.ndarray,
-                      steps: int, rng: np.random.Generator) -> ChainResult:
+                      steps: int, rng: np.random.Generator,
+                      max_seconds: Optional[float] = None) -> ChainResult:
     """C chains in lockstep, every proposal evaluated exactly (one batched
     launch per step) -- the reference for the surrogate's contribution."""
     C_, P = theta0.shape
     Lc = np.linalg.cholesky(cov)
     th = theta0.copy()
     L = exact(th)
-    out = np.empty((steps, C_, P))
-    acc = 0
+    out = []
+    acc = n = 0
     t0 = time.perf_counter()
     for s in range(steps):
         prop = th + rng.standard_normal((C_, P)) @ Lc.T
@@ -192,6 +241,8 @@ def run_batched_exact(exact: ExactTarget, theta0: np.ndarray, cov: np.ndarray,
         th[a] = prop[a]
         L[a] = Lp[a]
         acc += int(a.sum())
-        out[s] = th
-    return ChainResult(out, acc, steps * C_, steps * C_, steps * C_,
-                       time.perf_counter() - t0)
+        out.append(th.copy())
+        n = s + 1
+        if max_seconds is not None and time.perf_counter() - t0 > max_seconds:
+            break
+    return ChainResult(np.stack(out), acc, n * C_, n * C_, n * C_, time.perf_counter() - t0)

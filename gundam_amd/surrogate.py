@@ -91,34 +91,89 @@ class MLP:
 
 class Surrogate:
     """Parameter point -> predicted bin contents (sum of weights), via an MLP
-    on standardised inputs ((theta - prior mean) / prior sigma) and
-    standardised log contents."""
+    on standardised inputs and chi2-weighted log contents.
 
-    def __init__(self, n_par: int, n_bins: int, prior_mean: np.ndarray,
-                 prior_sigma: np.ndarray, hidden: int = 512, seed: int = 0,
-                 floor: float = 1e-3, device: int = 0):
+    Targets: for a bin with mean content cbar_b over the training set, the
+    network learns t_b = (log c_b - m_b) * sqrt(cbar_b) / G, where m_b is the
+    mean log content and G one global scale.  An error e in t_b is an error
+    e G / sqrt(cbar_b) in log c_b, which moves the Poisson chi2 of the bin by
+    about cbar_b (e G / sqrt(cbar_b))^2 = (e G)^2: the mean squared error is
+    proportional to the chi2 error, so the network spends its capacity on the
+    bins the likelihood is sensitive to.  Bins without events in the training
+    set are predicted empty."""
+
+    def __init__(self, n_par: int, n_bins: int, hidden: int = 512, seed: int = 0,
+                 device: int = 0):
         self.mlp = MLP(n_par, hidden, n_bins, seed=seed, device=device)
-        self.mu = np.asarray(prior_mean, np.float64)
-        self.sig = np.where(np.asarray(prior_sigma) > 0, prior_sigma, 1.0)
-        self.floor = floor
-        self.y_mean = np.zeros(n_bins)
-        self.y_scale = np.ones(n_bins)
+        self.n_par, self.n_bins = n_par, n_bins
+        self.x_mu = np.zeros(n_par)
+        self.x_sd = np.ones(n_par)
+        self.mean = np.zeros(n_bins)
+        self.scale = np.ones(n_bins)
+        self.head_ready = False
 
     def _x(self, theta):
-        return ((np.atleast_2d(theta) - self.mu) / self.sig).astype(np.float32)
+        return ((np.atleast_2d(theta) - self.x_mu) / self.x_sd).astype(np.float32)
+
+    def targets(self, sums: np.ndarray) -> np.ndarray:
+        """Fix the output normalisation from a training set of bin contents
+        and return the normalised targets."""
+        sums = np.asarray(sums, np.float64)
+        cbar = sums.mean(0)
+        live = cbar > 0
+        floor = np.where(live, 1e-6 * cbar, 1.0)
+        logc = np.log(np.maximum(sums, floor))
+        m = logc.mean(0)
+        u = np.sqrt(np.where(live, cbar, 0.0))
+        dev = (logc - m) * u
+        G = float(np.sqrt(np.mean(dev[:, live] ** 2))) if live.any() else 1.0
+        G = G if G > 0 else 1.0
+        self.mean = np.where(live, m, -np.inf)
+        self.scale = np.where(live, G / np.where(live, u, 1.0), 0.0)
+        return (dev / G).astype(np.float32)
 
     def fit(self, theta: np.ndarray, sums: np.ndarray, epochs: int = 30,
-            batch: int = 256, lr: float = 1e-3, seed: int = 0) -> np.ndarray:
-        logy = np.log(np.maximum(sums, self.floor))
-        self.y_mean = logy.mean(0)
-        self.y_scale = np.maximum(logy.std(0), 1e-3)
-        t = ((logy - self.y_mean) / self.y_scale).astype(np.float32)
+            batch: int = 256, lr: float = 1e-3, lr_final: Optional[float] = None,
+            seed: int = 0) -> np.ndarray:
+        """Adam epochs on (theta, bin contents) pairs; the learning rate decays
+        geometrically from lr to lr_final (default lr / 30).  Returns the
+        sampled training loss of each epoch."""
+        theta = np.atleast_2d(np.asarray(theta, np.float64))
+        self.x_mu = theta.mean(0)
+        sd = theta.std(0)
+        self.x_sd = np.where(sd > 0, sd, 1.0)
+        t = self.targets(sums)
         self.mlp.set_data(self._x(theta), t)
-        return self.mlp.train(epochs, batch, lr, seed)
+        lr_final = lr / 30.0 if lr_final is None else lr_final
+        losses = np.empty(epochs)
+        for e in range(epochs):
+            le = lr * (lr_final / lr) ** (e / max(1, epochs - 1))
+            losses[e] = self.mlp.train(1, batch, le, seed * 100003 + e)[0]
+        self.head_ready = False
+        return losses
 
     def predict_sums(self, theta: np.ndarray) -> np.ndarray:
         y = self.mlp.predict(self._x(theta)).astype(np.float64)
-        return np.exp(y * self.y_scale + self.y_mean)
+        with np.errstate(invalid="ignore"):
+            out = np.exp(self.mean + self.scale * y)
+        return np.where(np.isfinite(self.mean), out, 0.0)
+
+    def set_head(self, data: np.ndarray, var_ratio: np.ndarray) -> None:
+        """Data histogram and per-bin (sum w^2 / sum w) ratio for chi2()."""
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        self._head = [f(self.mean), f(self.scale), f(data), f(var_ratio)]
+        N.check(self.mlp._lib.gcm_mlp_set_head(self.mlp._h, *[a.ctypes.data for a in self._head]))
+        self.head_ready = True
+
+    def chi2(self, theta: np.ndarray) -> np.ndarray:
+        """Barlow-Beeston chi2 of the predicted contents, on the device."""
+        if not self.head_ready:
+            raise RuntimeError("Surrogate.set_head after fit")
+        x = self._x(theta)
+        out = np.empty(x.shape[0])
+        N.check(self.mlp._lib.gcm_mlp_predict_llh(self.mlp._h, x.ctypes.data, x.shape[0],
+                                                 out.ctypes.data))
+        return out
 
 
 def barlow_llh(data: np.ndarray, pred: np.ndarray, mcvar: np.ndarray) -> np.ndarray:

@@ -248,6 +248,14 @@ int gcm_mlp_gradients(gcm_mlp* m, const float* x, const float* t, int rows,
 int gcm_mlp_set_data(gcm_mlp* m, const float* x, const float* t, int64_t rows);
 int gcm_mlp_train(gcm_mlp* m, int epochs, int batch, float lr, uint64_t seed,
                   double* loss_out);
+/* Likelihood head: for output b, pred = exp(mean[b] + scale[b] * y[b])
+ * (mean[b] = -inf: no events, pred = 0) with MC variance pred * var_ratio[b];
+ * gcm_mlp_predict_llh returns each row's Barlow-Beeston OA2021 chi2 against
+ * data[] (the same per-bin formula as GCM_LLH_BARLOW_OA2021), summed over the
+ * n_out bins in FP64 on the device. */
+int gcm_mlp_set_head(gcm_mlp* m, const double* mean, const double* scale,
+                     const double* data, const double* var_ratio);
+int gcm_mlp_predict_llh(gcm_mlp* m, const float* x, int64_t rows, double* chi2);
 
 #ifdef __cplusplus
 }

@@ -9,9 +9,9 @@ import numpy as np
 import pytest
 
 from gundam_amd import CacheManager, Workload
-from gundam_amd.mcmc import (ExactTarget, SurrogateTarget, laplace_scales,
-                             run_batched_exact, run_delayed)
-from gundam_amd.surrogate import MLP, Surrogate
+from gundam_amd.mcmc import (ExactTarget, SurrogateTarget, adapt_cov, laplace_scales,
+                             run_batched_exact, run_delayed, var_ratio)
+from gundam_amd.surrogate import MLP, Surrogate, barlow_llh
 from oracle import mlp_oracle
 
 pytestmark = pytest.mark.gpu
@@ -58,9 +58,11 @@ def test_mlp_training_reduces_loss():
 
 
 def test_surrogate_delayed_acceptance_gain():
-    """On the 100k-event workload: delayed acceptance with the trained
-    surrogate samples the same posterior as the exact chains while spending
-    fewer exact evaluations per accepted move."""
+    """On the 100k-event workload: the surrogate, trained on pairs that the
+    exact path generates during burn-in, screens proposals for delayed
+    acceptance.  The chains sample the same posterior as the exact chains
+    while spending fewer exact evaluations per accepted move, and surrogate
+    sub-chains move farther per exact evaluation."""
     print("surrogate DA-MCMC workload=1", flush=True)
     wl = Workload(1, seed=1)
     t = wl.tables
@@ -68,30 +70,45 @@ def test_surrogate_delayed_acceptance_gain():
     C_ = 32
     cm = CacheManager(0)
     cm.build(t, max_points=64)
-    cm.propagate_parameters(wl.nominal())
-    data, s2 = cm.hist(0)
-    cm.set_likelihood("barlow", data.copy(), wl.prior_mean, wl.prior_sigma)
-    exact = ExactTarget(cm)
     mode = wl.nominal()
+    cm.propagate_parameters(mode)
+    data, err = cm.hist(0)
+    data = data.copy()
+    ratio = var_ratio(data, err)
+    cm.set_likelihood("barlow", data, wl.prior_mean, wl.prior_sigma)
+    exact = ExactTarget(cm)
     sc = laplace_scales(exact, mode)
-    rng = np.random.default_rng(7)
-    train = mode + 3.0 * sc * rng.standard_normal((12000, mode.size))
-    sums = exact.sums(train)
-    sur = Surrogate(mode.size, t.n_bins, mode, sc, hidden=256, seed=2)
-    losses = sur.fit(train, sums, epochs=40, batch=256, lr=2e-3)
-    assert losses[-1] < 0.05 * losses[0]
-    ratio = np.where(data > 0, s2 / np.maximum(data, 1e-300), 1.0)
-    surr = SurrogateTarget(sur, data, ratio, wl.prior_mean, wl.prior_sigma)
     d = mode.size
-    cov = (2.38 ** 2 / d) * np.diag(sc ** 2)
-    th0 = mode + sc * rng.standard_normal((C_, d))
-    da = run_delayed(exact, surr, th0, cov, 400, rng)
-    ex = run_batched_exact(exact, th0, cov, 400, rng)
-    a = da.samples[100:].reshape(-1, d)
-    b = ex.samples[100:].reshape(-1, d)
-    # same posterior: means within 0.25 posterior sigma at these short lengths
-    assert np.all(np.abs(a.mean(0) - b.mean(0)) < 0.25 * sc)
+    rng = np.random.default_rng(7)
+    # burn-in: exact chains from Laplace draws; every evaluated point is kept
+    exact.record = True
+    burn = run_batched_exact(exact, mode + sc * rng.standard_normal((C_, d)),
+                             (2.38 ** 2 / d) * np.diag(sc ** 2), 150, rng)
+    exact(mode + sc * rng.standard_normal((12000 - exact.points, d)))
+    exact.record = False
+    X, Y = exact.recorded()
+    cov = adapt_cov(burn.samples[75:], sc)
+    sur = Surrogate(d, t.n_bins, hidden=256, seed=2)
+    losses = sur.fit(X, Y, epochs=150, batch=256, lr=2e-3)
+    assert losses[-1] < 0.1 * losses[0], (losses[0], losses[-1])
+    surr = SurrogateTarget(sur, data, ratio, wl.prior_mean, wl.prior_sigma)
+    # the device likelihood head against the host Barlow-Beeston restatement
+    test = mode + sc * rng.standard_normal((256, d))
+    pred = sur.predict_sums(test)
+    host = barlow_llh(data, pred, pred * ratio)
+    np.testing.assert_allclose(sur.chi2(test), host, rtol=1e-9)
+    start = burn.samples[-1]
+    da = run_delayed(exact, surr, start.copy(), cov, 1000, rng)
+    ex = run_batched_exact(exact, start.copy(), cov, 1000, rng)
+    sub = run_delayed(exact, surr, start.copy(), cov, 250, rng, n_sub=4)
+    a = da.samples[200:].reshape(-1, d)
+    b = ex.samples[200:].reshape(-1, d)
+    # same posterior: means within 0.3 posterior sigma at these lengths
+    assert np.all(np.abs(a.mean(0) - b.mean(0)) < 0.3 * sc), np.abs(a.mean(0) - b.mean(0)) / sc
     gain = (ex.exact_evals / ex.accepted) / (da.exact_evals / da.accepted)
+    jump_gain = (sub.sq_jumps(sc) / sub.exact_evals) / (ex.sq_jumps(sc) / ex.exact_evals)
     print("exact evaluations per accepted move: exact", ex.exact_evals / ex.accepted,
-          "delayed", da.exact_evals / da.accepted, "gain", gain)
+          "delayed", da.exact_evals / da.accepted, "gain", gain,
+          "; squared jump per exact evaluation, 4-step sub-chains vs exact", jump_gain)
     assert gain > 1.5
+    assert jump_gain > 2.0

@@ -3,6 +3,7 @@ finite differences, the vectorised Barlow-Beeston used by the surrogate
 stage pinned against the oracle's per-bin restatement, and the delayed
 acceptance sampler's exactness on a Gaussian target with a biased screen."""
 import numpy as np
+import pytest
 
 from gundam_amd.mcmc import run_delayed, run_plain
 from gundam_amd.surrogate import barlow_llh
@@ -51,9 +52,13 @@ def test_vectorised_barlow_matches_oracle():
         assert abs(got[r] - want) <= 1e-9 * abs(want)
 
 
-def test_delayed_acceptance_targets_the_exact_distribution():
-    """Exact target N(m, S); the screen is a deliberately wrong Gaussian.  The
-    delayed-acceptance chains must still sample N(m, S)."""
+@pytest.mark.parametrize("n_sub", [1, 5])
+def test_delayed_acceptance_targets_the_exact_distribution(n_sub):
+    """Exact target N(m, S); the screen is a deliberately wrong Gaussian
+    (shifted by 0.2, 1.3x wider).  The delayed-acceptance chains, with one
+    screening step or a surrogate sub-chain of five, must still sample
+    N(m, S): a missing second-stage correction would leave the screen's
+    mean and width, which the tolerances below exclude."""
     rng = np.random.default_rng(2)
     P = 3
     m = np.array([0.5, -1.0, 2.0])
@@ -69,7 +74,7 @@ def test_delayed_acceptance_targets_the_exact_distribution():
 
     cov = np.diag((2.38 * s) ** 2 / P)
     th0 = m + s * rng.standard_normal((64, P))
-    r = run_delayed(exact, screen, th0, cov, 1500, rng)
+    r = run_delayed(exact, screen, th0, cov, 1500, rng, n_sub=n_sub)
     smp = r.samples[300:].reshape(-1, P)
     np.testing.assert_allclose(smp.mean(0), m, atol=0.05 * s.max() * 3)
     np.testing.assert_allclose(smp.std(0), s, rtol=0.06)
