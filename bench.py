@@ -244,6 +244,105 @@ def surrogate_da(args, cm, wl, data, err, log):
     }
 
 
+def precision_c4(args, log):
+    """FP32 dial mode against the FP64 path on the 20M-event workload (6
+    samples of 60x50 bins, 500 parameters, 15 dials/event): max per-bin
+    relative error of the bin sums and |dLLH| (Barlow-Beeston against the
+    FP64 Asimov histogram) at points drawn from N(0,1) around the prior
+    means, K points per launch; plus the plain fill kernel of each mode.
+    The FP64 path is the reference here: it matches the CPU oracle to 1e-10
+    on the bin sums (tests/test_gpu_parity.py)."""
+    from gundam_amd import CacheManager, Workload
+
+    t0 = time.time()
+    wl = Workload(4, seed=args.seed)
+    t = wl.tables
+    wl.bin_events_on_device(0)
+    log(f"c4: generated {t.n_events} events, {t.n_dials} dials in {time.time()-t0:.1f}s")
+    K = 64
+    cms = {}
+    shapes = {}
+    from gundam_amd.autotune import autotune
+    tp = wl.random_points(1, 1.0, seed=args.seed + 5)
+    for prec in ("fp64", "fp32"):
+        tuned = autotune(t, tp, wl.table_values(tp), precision=prec,
+                         n_events=args.tune_events, reps=3)
+        shapes[prec] = {k: tuned[k] for k in ("tile_words", "block_threads",
+                                              "blocks_per_cu", "ring_buffers")}
+        t0 = time.time()
+        cms[prec] = CacheManager(0, precision=prec, **shapes[prec])
+        cms[prec].build(t, max_points=K)
+        log(f"c4: {prec} shape {shapes[prec]} built in {time.time()-t0:.1f}s, "
+            f"{cms[prec].counts()}, resident {cms[prec].resident_memory()/2**30:.2f} GiB")
+    nominal = wl.nominal()
+    cms["fp64"].propagate_parameters(nominal, wl.table_values(nominal))
+    data = cms["fp64"].hist(0)[0].copy()
+    for cm in cms.values():
+        cm.set_likelihood("barlow", data, wl.prior_mean, wl.prior_sigma)
+    fill = {}
+    for prec, cm in cms.items():
+        p1 = wl.random_points(6, 1.0, seed=args.seed + 7)
+        tb = wl.table_values(p1)
+        cm.enable_timing(True)
+        for i in range(6):
+            cm.set_parameters(p1[i])
+            cm.set_table_values(tb[i])
+            cm.fill()
+            cm.llh()
+        tm = cm.timing()
+        cm.enable_timing(False)
+        ms = tm["fill_ms"] / max(1, tm["count"])
+        gbs = t.algorithmic_bytes(8 if prec == "fp64" else 4) / (ms * 1e-3) / 1e9
+        fill[prec] = {"fill_kernel_ms": ms, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
+        log(f"c4: {prec} plain fill {ms:.3f} ms ({gbs:.0f} GB/s)")
+    n = args.c4_points
+    pts = wl.random_points(n, 1.0, seed=args.seed + 4)
+    rel_max = np.zeros(n)
+    rel2_max = np.zeros(n)
+    dllh = np.zeros(n)
+    llh64 = np.zeros(n)
+    zero_mismatch = 0
+    t0 = time.time()
+    for i in range(0, n, K):
+        th = pts[i:i + K]
+        tb = wl.table_values(th)
+        res = {}
+        for prec, cm in cms.items():
+            cm.set_parameters(th)
+            cm.set_table_values(tb)
+            cm.fill()
+            res[prec] = ([cm.hist(k) for k in range(th.shape[0])], cm.llh()[0][:th.shape[0]].copy())
+        for k in range(th.shape[0]):
+            s64, e64 = res["fp64"][0][k]
+            s32, e32 = res["fp32"][0][k]
+            nz = s64 != 0
+            zero_mismatch += int(np.count_nonzero(s32[~nz]))
+            rel_max[i + k] = np.max(np.abs(s32[nz] - s64[nz]) / np.abs(s64[nz]))
+            rel2_max[i + k] = np.max(np.abs(e32[nz] - e64[nz]) / np.abs(e64[nz]))
+        l64, l32 = res["fp64"][1], res["fp32"][1]
+        dllh[i:i + th.shape[0]] = np.abs(l32 - l64)
+        llh64[i:i + th.shape[0]] = l64
+    log(f"c4: {n} points in {time.time()-t0:.1f}s; max per-bin rel error "
+        f"{rel_max.max():.3e}, max |dLLH| {dllh.max():.3e}")
+    for cm in cms.values():
+        cm.close()
+    return {
+        "workload": (f"{t.n_events} events in 6 samples (8M/5M/3M/2M/1M/1M) x 3000 bins, "
+                     f"{wl.n_parameters} parameters, {t.n_dials} dials"),
+        "points": n, "points_per_launch": K,
+        "max_rel_bin_sum_error": float(rel_max.max()),
+        "median_of_per_point_max_rel_error": float(np.median(rel_max)),
+        "max_rel_sqrt_sumw2_error": float(rel2_max.max()),
+        "max_abs_dllh": float(dllh.max()),
+        "median_abs_dllh": float(np.median(dllh)),
+        "max_rel_dllh": float(np.max(dllh / np.abs(llh64))),
+        "llh_range": [float(llh64.min()), float(llh64.max())],
+        "nonzero_where_fp64_zero": zero_mismatch,
+        "plain_fill": fill,
+        "launch": shapes,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -272,6 +371,11 @@ def main():
     ap.add_argument("--c5-epochs", type=int, default=100)
     ap.add_argument("--c5-sub", type=int, default=8)
     ap.add_argument("--c5-seconds", type=float, default=10.0)
+    ap.add_argument("--c4", action="store_true",
+                    help="add the FP32-vs-FP64 precision study on the 20M-event workload")
+    ap.add_argument("--c4-points", type=int, default=1000)
+    ap.add_argument("--only-c4", action="store_true",
+                    help="run only the precision study and print its JSON")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -281,6 +385,10 @@ def main():
         sys.exit(2)
 
     from gundam_amd import CacheManager, Workload
+
+    if args.only_c4:
+        print(json.dumps({"precision_c4": precision_c4(args, log)}), flush=True)
+        return
 
     t0 = time.time()
     wl = Workload(args.workload, n_events=args.events, seed=args.seed)
@@ -381,6 +489,8 @@ def main():
 
     batched_c3 = None if args.no_c3 else batched_workload3(args, log)
 
+    c4 = precision_c4(args, log) if args.c4 else None
+
     c5 = None
     if not args.no_c5 and args.batch >= 32:
         c5 = surrogate_da(args, cm, wl, data, data_err, log)
@@ -437,6 +547,7 @@ def main():
         "batched": batched,
         "batched_c3": batched_c3,
         "surrogate_da": c5,
+        "precision_c4": c4,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

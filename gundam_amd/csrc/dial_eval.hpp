@@ -98,14 +98,46 @@ __device__ __forceinline__ R spline_uniform(R x, const R* d, int dim) {
   return clamp_value<R>(hermite<R>(p1, p2, m1, m2, fx), R(-1E20), R(1E20));
 }
 
-// CalculateGeneralSpline.h:51-197, branchless binary search (dim = 3n + 2).
-template <typename R>
+// Internal descriptor type codes (never seen by callers): general splines
+// and graphs whose knot abscissae are not non-decreasing (or hold a NaN) keep
+// the reference's binary search; sorted ones use the counting search below,
+// which returns the same segment index.
+constexpr int kTypeGeneralUnsorted = 9;
+constexpr int kTypeGraphUnsorted = 10;
+
+// #{ j in [first, end) : x > X[stride j] }, with the loads of a batch issued
+// together (independent LDS reads instead of a chain of dependent ones).
+template <typename R, int kBatch>
+__device__ __forceinline__ int count_below(R x, const R* X, int stride, int first, int end) {
+  int c = 0;
+  for (int j0 = first; j0 < end; j0 += kBatch) {
+    R v[kBatch];
+#pragma unroll
+    for (int q = 0; q < kBatch; ++q) {
+      const int j = (j0 + q < end) ? j0 + q : end - 1;
+      v[q] = X[stride * j];
+    }
+#pragma unroll
+    for (int q = 0; q < kBatch; ++q) c += (j0 + q < end && x > v[q]) ? 1 : 0;
+  }
+  return c;
+}
+
+// CalculateGeneralSpline.h:51-197.  The reference's BINARY_IF search (offsets
+// 16 .. 1) finds the largest j <= 31, j < last, with x > X[j] (or 0); for
+// non-decreasing X that is min(#{j in [1, last) : x > X[j]}, 31).
+template <typename R, bool kSorted>
 __device__ __forceinline__ R spline_general(R x, const R* d, int dim) {
   const int last = (dim - 2) / 3 - 1;
   int ix = 0;
+  if (kSorted) {
+    ix = count_below<R, 16>(x, d + 4, 3, 1, last);
+    if (ix > 31) ix = 31;
+  } else {
 #pragma unroll
-  for (int off = 16; off > 0; off >>= 1) {
-    if ((ix + off < last) && (x > d[2 + 3 * (ix + off) + 2])) ix += off;
+    for (int off = 16; off > 0; off >>= 1) {
+      if ((ix + off < last) && (x > d[2 + 3 * (ix + off) + 2])) ix += off;
+    }
   }
   const R x1 = d[2 + 3 * ix + 2];
   const R x2 = d[2 + 3 * (ix + 1) + 2];
@@ -118,14 +150,19 @@ __device__ __forceinline__ R spline_general(R x, const R* d, int dim) {
   return clamp_value<R>(hermite<R>(p1, p2, m1, m2, fx), R(-1E20), R(1E20));
 }
 
-// CalculateGraph.h:43-88 (dim = 2n words).
-template <typename R>
+// CalculateGraph.h:43-88 (dim = 2n words).  The binary search covers every
+// knot, so for non-decreasing X it returns #{j in [1, knots) : x > X[j]}.
+template <typename R, bool kSorted>
 __device__ __forceinline__ R graph_linear(R x, const R* d, int dim) {
   if (dim < 4) return d[0];
   const int knots = dim / 2;
   int ix = 0;
-  for (int off = 1 << (31 - __builtin_clz(knots)); off > 0; off >>= 1) {
-    if ((ix + off < knots) && (x > d[2 * (ix + off) + 1])) ix += off;
+  if (kSorted) {
+    ix = count_below<R, 8>(x, d + 1, 2, 1, knots);
+  } else {
+    for (int off = 1 << (31 - __builtin_clz(knots)); off > 0; off >>= 1) {
+      if ((ix + off < knots) && (x > d[2 * (ix + off) + 1])) ix += off;
+    }
   }
   if (ix + 1 >= knots) ix--;
   const R p1 = d[2 * ix];
@@ -194,20 +231,24 @@ __device__ __forceinline__ double eval_dial(int type, int n, const R* d,
       }
       return (double)spline_uniform<R>(x, d, 2 * n + 2);
     }
-    case GCM_DIAL_GENERAL: {
+    case GCM_DIAL_GENERAL:
+    case kTypeGeneralUnsorted: {
       if (no_extrap) {
         const R lo = d[0], hi = d[2 + 3 * (n - 1) + 2];
         if (x <= lo) x = lo;
         else if (x >= hi) x = hi;
       }
-      return (double)spline_general<R>(x, d, 3 * n + 2);
+      if (type == GCM_DIAL_GENERAL) return (double)spline_general<R, true>(x, d, 3 * n + 2);
+      return (double)spline_general<R, false>(x, d, 3 * n + 2);
     }
-    case GCM_DIAL_GRAPH: {
+    case GCM_DIAL_GRAPH:
+    case kTypeGraphUnsorted: {
       if (no_extrap) {  // Graph.cpp:48-51
         if (x <= d[1]) return (double)d[0];
         else if (x >= d[2 * n - 1]) return (double)d[2 * n - 2];
       }
-      return (double)graph_linear<R>(x, d, 2 * n);
+      if (type == GCM_DIAL_GRAPH) return (double)graph_linear<R, true>(x, d, 2 * n);
+      return (double)graph_linear<R, false>(x, d, 2 * n);
     }
     case GCM_DIAL_TABULATED: {  // Tabulated.h:27-28
       uint32_t slot;

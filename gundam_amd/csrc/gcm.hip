@@ -629,6 +629,20 @@ __global__ void bin_kernel(int64_t n, int nv, const double* __restrict__ vals,
 // ---------------------------------------------------------------------------
 
 // Words of device record per dial (per precision).
+// Descriptor type code of a dial: the public type, or an internal code for a
+// general spline / graph whose knot abscissae are not non-decreasing (NaN
+// included), which keeps the reference's binary segment search on the device.
+int device_type_code(int type, int n, const double* p) {
+  if (type == GCM_DIAL_GENERAL) {
+    for (int j = 0; j + 1 < n; ++j)
+      if (!(p[2 + 3 * j + 2] <= p[2 + 3 * (j + 1) + 2])) return kTypeGeneralUnsorted;
+  } else if (type == GCM_DIAL_GRAPH) {
+    for (int j = 0; j + 1 < n; ++j)
+      if (!(p[2 * j + 1] <= p[2 * (j + 1) + 1])) return kTypeGraphUnsorted;
+  }
+  return type;
+}
+
 int record_words(int type, int n, bool fp32) {
   switch (type) {
     case GCM_DIAL_NORM: return 0;
@@ -958,7 +972,8 @@ int build_layout(gcm_context* c, const gcm_tables* t) {
             }
           }
           const uint64_t in = (uint64_t)t->dial_input[d];
-          desc[h.first_dial + q] = in | ((uint64_t)type << 16) | ((uint64_t)n << 20) |
+          const int code = device_type_code(type, n, p);
+          desc[h.first_dial + q] = in | ((uint64_t)code << 16) | ((uint64_t)n << 20) |
                                    ((uint64_t)slot[idx[q]] << 32) | ((uint64_t)off << 48);
           off += record_words(type, n, fp32);
         }

@@ -159,3 +159,78 @@ def test_graph_replays_match_eager():
         eager.fill().get()
         np.testing.assert_array_equal(graph.hist(0)[0], eager.hist(0)[0])
         graph.enable_graph(True)
+
+
+def _search_tables(rng):
+    """One event per dial (each in its own bin), all dials on parameter 0:
+    general splines and graphs with sorted knots (including more than 33
+    knots, where the reference's binary search stops at segment 31, and
+    repeated abscissae) and unsorted knots.  (A NaN knot also takes the
+    binary search, but its NaN weights are an error on both sides.)"""
+    from gundam_amd import _native as N
+    from gundam_amd.tables import INPUT_DTYPE, Tables
+
+    dials = []
+    for n in (2, 3, 5, 9, 15, 33, 34, 40, 60):
+        xs = np.sort(rng.uniform(-3, 3, n))
+        dials.append((N.DIAL_GENERAL, xs))
+        dials.append((N.DIAL_GRAPH, xs))
+    dup = np.array([-2.0, -1.0, -1.0, 0.0, 0.0, 0.0, 1.5, 2.5])
+    dials += [(N.DIAL_GENERAL, dup), (N.DIAL_GRAPH, dup)]
+    uns = np.array([-2.0, 1.0, -0.5, 0.5, 2.0, 1.8, 3.0])
+    dials += [(N.DIAL_GENERAL, uns), (N.DIAL_GRAPH, uns)]
+    dials.append((N.DIAL_GRAPH, np.array([0.3])))
+    data, begin, types = [], [0], []
+    for typ, xs in dials:
+        n = xs.size
+        y = rng.uniform(0.5, 1.5, n)
+        if typ == N.DIAL_GENERAL:
+            rec = [xs[0], 1.0]
+            for j in range(n):
+                rec += [y[j], rng.uniform(-0.5, 0.5), xs[j]]
+        else:
+            rec = []
+            for j in range(n):
+                rec += [y[j], xs[j]]
+        data += rec
+        begin.append(len(data))
+        types.append(typ)
+    nd = len(types)
+    inputs = np.zeros(1, dtype=INPUT_DTYPE)
+    inputs["mirror_lower"] = np.nan
+    inputs["mirror_range"] = np.nan
+    inputs["min_response"] = 0.0
+    inputs["max_response"] = np.nan
+    t = Tables(base_weight=rng.uniform(0.5, 2.0, nd), bin=np.arange(nd, dtype=np.int32),
+               dial_begin=np.arange(nd + 1, dtype=np.int64),
+               dial_type=np.array(types, np.uint8), dial_input=np.zeros(nd, np.int32),
+               data_begin=np.array(begin, np.int64), dial_data=np.array(data),
+               inputs=inputs, table_begin=np.zeros(1, np.int64), n_parameters=1,
+               n_bins=nd)
+    knots = np.concatenate([xs for _, xs in dials])
+    return t, knots[np.isfinite(knots)]
+
+
+def test_segment_search_edge_cases():
+    """The device's counting segment search (sorted knots) and its binary
+    search (unsorted or NaN knots) return the reference's segment: weights
+    bit-exact against the oracle at parameter values on, between and beyond
+    the knots."""
+    print("segment search edge cases", flush=True)
+    rng = np.random.default_rng(12)
+    t, knots = _search_tables(rng)
+    xs = np.concatenate([knots, np.nextafter(knots, np.inf), np.nextafter(knots, -np.inf),
+                         rng.uniform(-4, 4, 40), [-1e30, 1e30, 0.0]])
+    orc = pyoracle.Oracle(t)
+    for prec in ("fp64",):
+        cm = CacheManager(0, precision=prec)
+        cm.build(t)
+        cm.set_event_weight_copy(True)
+        for x in xs:
+            theta = np.array([x])
+            w_ref, s_ref, _ = orc.propagate(theta)
+            cm.propagate_parameters(theta)
+            w = cm.copy_event_weights(np.full(t.n_events, np.nan))
+            np.testing.assert_array_equal(w, w_ref, err_msg=f"x={x!r}")
+        cm.close()
+    orc.close()
