@@ -368,9 +368,9 @@ def main():
                     help="skip the surrogate delayed-acceptance MCMC leg")
     ap.add_argument("--c5-pairs", type=int, default=16000)
     ap.add_argument("--c5-burn", type=int, default=120)
-    ap.add_argument("--c5-epochs", type=int, default=100)
-    ap.add_argument("--c5-sub", type=int, default=8)
-    ap.add_argument("--c5-seconds", type=float, default=10.0)
+    ap.add_argument("--c5-epochs", type=int, default=300)
+    ap.add_argument("--c5-sub", type=int, default=16)
+    ap.add_argument("--c5-seconds", type=float, default=8.0)
     ap.add_argument("--c4", action="store_true",
                     help="add the FP32-vs-FP64 precision study on the 20M-event workload")
     ap.add_argument("--c4-points", type=int, default=1000)

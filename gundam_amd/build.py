@@ -57,6 +57,17 @@ def build_hbm_copy() -> str:
     return out
 
 
+def build_lds_dma_stream() -> str:
+    """The LDS-DMA streaming calibration program (profiles/ uses its output)."""
+    out = os.path.join(HERE, "lds_dma_stream")
+    tmp = out + ".tmp"
+    _run([HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang",
+          "-target-feature", "-Xclang", "-packed-fp32-ops", "-o", tmp,
+          os.path.join(CSRC, "lds_dma_stream.hip")])
+    os.replace(tmp, out)
+    return out
+
+
 def build_facade_demo() -> str:
     """The C++ facade example, linked against the in-tree libraries."""
     out = os.path.join(HERE, "facade_demo")
@@ -72,6 +83,7 @@ def build_all() -> None:
     build_synth()
     build_gcm()
     build_hbm_copy()
+    build_lds_dma_stream()
     build_facade_demo()
 
 
