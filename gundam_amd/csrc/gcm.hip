@@ -832,6 +832,38 @@ int build_layout(gcm_context* c, const gcm_tables* t) {
     for (int64_t e = 0; e < N; ++e)
       if (t->bin[e] >= 0) order[pos[t->bin[e]]++] = e;
   }
+  // Inside each bin, events are ordered (stably) by the lookup table of their
+  // first kriged dial, then of their first tabulated dial: neighbouring events
+  // then read the same tables, so the table loads of a wave share cache lines.
+  // The per-bin sums keep a fixed order; only which fixed order changes.
+  {
+    std::vector<uint32_t> key(NB);
+    for (int64_t i = 0; i < NB; ++i) {
+      const int64_t e = order[i];
+      uint32_t kr = 0xffff, tb = 0xffff;
+      for (int64_t d = t->dial_begin[e]; d < t->dial_begin[e + 1]; ++d) {
+        const int ty = t->dial_type[d];
+        if (ty == GCM_DIAL_KRIGED && kr == 0xffff)
+          kr = (uint32_t)std::min<int64_t>((int64_t)t->dial_data[t->data_begin[d]], 0xfffe);
+        else if (ty == GCM_DIAL_TABULATED && tb == 0xffff)
+          tb = (uint32_t)std::min<int64_t>((int64_t)t->dial_data[t->data_begin[d]], 0xfffe);
+      }
+      key[i] = kr << 16 | tb;
+    }
+    std::vector<int64_t> idx;
+    std::vector<int64_t> tmp;
+    for (int b = 0; b < t->n_bins; ++b) {
+      const int64_t b0 = count[b], b1 = count[b + 1];
+      if (b1 - b0 < 2) continue;
+      idx.resize(b1 - b0);
+      for (int64_t i = b0; i < b1; ++i) idx[i - b0] = i;
+      std::stable_sort(idx.begin(), idx.end(),
+                       [&](int64_t x, int64_t y) { return key[x] < key[y]; });
+      tmp.resize(b1 - b0);
+      for (int64_t i = b0; i < b1; ++i) tmp[i - b0] = order[idx[i - b0]];
+      std::copy(tmp.begin(), tmp.end(), order.begin() + b0);
+    }
+  }
   // tiles: greedy over sorted events
   const int words_per16 = 16 / (int)sizeof(R);
   const int64_t cap_words = (int64_t)c->launch.tile_words * 8 / (int64_t)sizeof(R);
@@ -930,12 +962,19 @@ int build_layout(gcm_context* c, const gcm_tables* t) {
             slot.push_back(local++);
           }
         }
-        // type-major order, stable inside a type
+        // type-major order (table dials also by table), stable otherwise
         std::vector<int> idx(dial_ids.size());
         for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
-        std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) {
-          return t->dial_type[dial_ids[x]] < t->dial_type[dial_ids[y]];
-        });
+        auto sort_key = [&](int q) -> int64_t {
+          const int64_t d = dial_ids[q];
+          const int ty = t->dial_type[d];
+          int64_t tb = 0;
+          if (ty == GCM_DIAL_TABULATED || ty == GCM_DIAL_KRIGED)
+            tb = (int64_t)t->dial_data[t->data_begin[d]];
+          return (int64_t)ty << 40 | tb;
+        };
+        std::stable_sort(idx.begin(), idx.end(),
+                         [&](int x, int y) { return sort_key(x) < sort_key(y); });
         int64_t off = 0;
         for (size_t q = 0; q < idx.size(); ++q) {
           const int64_t d = dial_ids[idx[q]];
