@@ -40,6 +40,16 @@ PRECISION_FP64 = 0
 PRECISION_FP32 = 1
 LLH_POISSON = 0
 LLH_BARLOW_OA2021 = 1
+LLH_BARLOW_BEESTON = 2
+LLH_BARLOW_OA2020 = 3
+LLH_BARLOW_OA2021_SFGD = 4
+LLH_CHI_SQUARED = 5
+LLH_LEAST_SQUARES = 6
+# names of the reference's JointProbability plugins (JointProbability.cpp)
+LLH_TYPES = {"poisson": LLH_POISSON, "barlow": LLH_BARLOW_OA2021,
+             "barlow_oa2021": LLH_BARLOW_OA2021, "barlow_classic": LLH_BARLOW_BEESTON,
+             "barlow_oa2020": LLH_BARLOW_OA2020, "barlow_oa2021_sfgd": LLH_BARLOW_OA2021_SFGD,
+             "chi2": LLH_CHI_SQUARED, "least_squares": LLH_LEAST_SQUARES}
 
 
 class GcmInput(C.Structure):
@@ -64,7 +74,11 @@ class GcmTables(C.Structure):
 class GcmLlhConfig(C.Structure):
     _fields_ = [("type", C.c_int32), ("use_poisson_likelihood", C.c_int32),
                 ("fractional_error_limit", C.c_double),
-                ("expected_value_minimum", C.c_double)]
+                ("expected_value_minimum", C.c_double),
+                ("lsq_poissonian_approximation", C.c_int32),
+                ("bb_no_update_weights", C.c_int32),
+                ("nominal_err", C.c_void_p), ("extra_fraction", C.c_void_p),
+                ("ignore_bins", C.c_void_p)]
 
 
 class GcmLaunchConfig(C.Structure):

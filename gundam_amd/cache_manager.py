@@ -104,11 +104,31 @@ class CacheManager:
                        prior_sigma: Optional[np.ndarray] = None,
                        use_poisson_likelihood: bool = False,
                        fractional_error_limit: float = 1.0e150,
-                       expected_value_minimum: float = -1.0) -> None:
+                       expected_value_minimum: float = -1.0,
+                       lsq_poissonian_approximation: bool = False,
+                       nominal_err: Optional[np.ndarray] = None,
+                       extra_fraction: Optional[np.ndarray] = None,
+                       ignore_bins: Optional[np.ndarray] = None) -> None:
+        """kind: one of _native.LLH_TYPES ("poisson", "barlow" (OA2021),
+        "barlow_classic", "barlow_oa2020", "barlow_oa2021_sfgd", "chi2",
+        "least_squares").  nominal_err switches on BBNoUpdateWeights (OA2021);
+        ignore_bins is the ignoreBinsWithZeroPredictionAtPrior mask
+        (zero_prediction_mask() computes it)."""
+        if kind not in N.LLH_TYPES:
+            raise ValueError(f"unknown likelihood {kind!r}")
+        f64 = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        nominal_err, extra_fraction = f64(nominal_err), f64(extra_fraction)
+        if ignore_bins is not None:
+            ignore_bins = np.ascontiguousarray(ignore_bins, dtype=np.uint8)
+        for a in (nominal_err, extra_fraction, ignore_bins):
+            if a is not None and a.shape != (self.n_bins,):
+                raise ValueError("per-bin array has the wrong size")
+        ptr = lambda a: None if a is None else a.ctypes.data
         cfg = N.GcmLlhConfig(
-            {"poisson": N.LLH_POISSON, "barlow": N.LLH_BARLOW_OA2021}[kind],
-            int(use_poisson_likelihood), float(fractional_error_limit),
-            float(expected_value_minimum))
+            N.LLH_TYPES[kind], int(use_poisson_likelihood), float(fractional_error_limit),
+            float(expected_value_minimum), int(lsq_poissonian_approximation),
+            int(nominal_err is not None), ptr(nominal_err), ptr(extra_fraction),
+            ptr(ignore_bins))
         data = np.ascontiguousarray(data, dtype=np.float64)
         if data.shape != (self.n_bins,):
             raise ValueError("data histogram has the wrong size")
@@ -116,11 +136,17 @@ class CacheManager:
         if prior_mean is not None:
             pm = np.ascontiguousarray(prior_mean, dtype=np.float64)
             ps = np.ascontiguousarray(prior_sigma, dtype=np.float64)
-        self._keep_llh = (data, pm, ps)
+        self._keep_llh = (data, pm, ps, nominal_err, extra_fraction, ignore_bins)
         N.check(self._lib.gcm_set_likelihood(
             self._h, C.byref(cfg), data.ctypes.data,
             None if pm is None else pm.ctypes.data,
             None if ps is None else ps.ctypes.data))
+
+    def zero_prediction_mask(self, k: int = 0) -> np.ndarray:
+        """Bins whose current prediction (point k) is exactly zero: the
+        ignoreBinsWithZeroPredictionAtPrior flags when called at the prior
+        (JointProbabilityBase.cpp:20-45)."""
+        return (self.hist(k)[0] == 0.0).astype(np.uint8)
 
     # -- per evaluation -------------------------------------------------------
     def set_parameters(self, theta: np.ndarray) -> None:
@@ -149,6 +175,7 @@ class CacheManager:
 
     # -- fetch ----------------------------------------------------------------
     def hist(self, k: int = 0):
+        """(sum w, sum w^2) per bin for point k."""
         s = np.empty(self.n_bins)
         s2 = np.empty(self.n_bins)
         N.check(self._lib.gcm_get_hist(self._h, int(k), s.ctypes.data,

@@ -444,49 +444,120 @@ struct LlhParams {
   int32_t use_poisson;
   double frac_limit;
   double expected_min;
+  int32_t lsq_poisson;
+  int32_t no_update_weights;
+  const double* nominal_err;     // [n_bins] or null
+  const double* extra_fraction;  // [n_bins] or null
+  const uint8_t* ignore;         // [n_bins] or null
 };
 
-// PoissonLogLikelihood.h:19-53 and BarlowBeestonBanff2022.h:150-294
-// (BBNoUpdateWeights false; err = sqrt(sum w^2), squared as at :125-126).
-__device__ double llh_bin(const LlhParams& c, double data, double pred,
+// Per-bin -2 ln L of bin b from (data, sum w, sqrt(sum w^2)) as
+// JointProbabilityBase::eval(SamplePair, bin) hands them over
+// (JointProbabilityBase.h:37-43).  Where the reference throws (LogThrowIf)
+// the result is NaN.
+__device__ double llh_bin(const LlhParams& c, int b, double data, double pred,
                           double err) {
   const double inf = __builtin_huge_val();
-  if (c.type == GCM_LLH_POISSON) {
-    if (pred <= 0) return inf;
-    if (data <= 0) return 2.0 * pred;
-    return 2.0 * (pred - data + data * log(data / pred));
-  }
-  double mcuncert = err;
-  mcuncert *= mcuncert;
-  if (!isfinite(mcuncert) || mcuncert < 0.0) return inf;
-  if (pred <= c.expected_min) {
-    if (pred <= 0.0 && data > 0.0) return inf;
-    return 0.0;
-  }
+  const double nan = __builtin_nan("");
   const double dmin = 2.2250738585072014e-308;  // numeric_limits<double>::min
   const double deps = 2.220446049250313e-16;    // numeric_limits<double>::epsilon
-  double newmc = pred;
-  double penalty = 0.0;
-  if (mcuncert > 0.0) {
-    const double fractional = fmin(sqrt(mcuncert) / pred, c.frac_limit);
-    const double temp = pred * fractional * fractional - 1;
-    const double temp2 = temp * temp + 4 * data * fractional * fractional;
-    if (temp2 < 0) return inf;
-    const double beta = (-1 * temp + sqrt(temp2)) / 2.;
-    newmc = fmax(pred * beta, dmin);
-    penalty = (beta - 1) * (beta - 1) / (2 * fractional * fractional);
+  // JointProbabilityBase.h:74-77: bins flagged zero at the prior
+  if (c.ignore && c.ignore[b]) return 0.0;
+  switch (c.type) {
+    case GCM_LLH_POISSON:  // PoissonLogLikelihood.h:19-53
+      if (pred <= 0) return inf;
+      if (data <= 0) return 2.0 * pred;
+      return 2.0 * (pred - data + data * log(data / pred));
+    case GCM_LLH_BARLOW_OA2021: {  // BarlowBeestonBanff2022.h:92-294
+      double mcuncert = (c.no_update_weights && c.nominal_err) ? c.nominal_err[b] : err;
+      mcuncert *= mcuncert;
+      if (!isfinite(mcuncert) || mcuncert < 0.0) return inf;
+      if (pred <= c.expected_min) {
+        if (pred <= 0.0 && data > 0.0) return inf;
+        return 0.0;
+      }
+      double newmc = pred;
+      double penalty = 0.0;
+      if (mcuncert > 0.0) {
+        const double fractional = fmin(sqrt(mcuncert) / pred, c.frac_limit);
+        const double temp = pred * fractional * fractional - 1;
+        const double temp2 = temp * temp + 4 * data * fractional * fractional;
+        if (temp2 < 0) return inf;
+        const double beta = (-1 * temp + sqrt(temp2)) / 2.;
+        newmc = fmax(pred * beta, dmin);
+        penalty = (beta - 1) * (beta - 1) / (2 * fractional * fractional);
+      }
+      double stat = 0;
+      if (data <= deps) {
+        stat = newmc;
+      } else if (newmc > dmin) {
+        stat = newmc - data + data * (log(data) - log(newmc));
+      } else {
+        stat = dmin - data + data * (log(data) - log(dmin));
+      }
+      double chisq = 2.0 * stat;
+      if (!c.use_poisson) chisq += 2.0 * penalty;
+      return chisq;
+    }
+    case GCM_LLH_BARLOW_BEESTON: {  // BarlowBeeston.h:33-62 (err as given)
+      const double dataVal = fmax(data, 0.0);
+      const double predVal = fmax(pred, sqrt(dmin));
+      const double mcUncert = fmax(err, dmin);
+      const double rel_var = mcUncert / (predVal * predVal);
+      const double bq = (predVal * rel_var) - 1;
+      const double cq = 4 * dataVal * rel_var;
+      const double beta = (-bq + sqrt(bq * bq + cq)) / 2.0;
+      const double mc_hat = predVal * beta;
+      double chi2 = 0.0;
+      if (dataVal <= 0.0) {
+        chi2 = 2 * mc_hat;
+        chi2 += (beta - 1) * (beta - 1) / rel_var;
+      } else {
+        chi2 = 2 * (mc_hat - dataVal);
+        if (dataVal > 0.0) chi2 += 2 * dataVal * (log(dataVal) - log(mc_hat));
+        chi2 += (beta - 1) * (beta - 1) / rel_var;
+      }
+      return chi2;
+    }
+    case GCM_LLH_BARLOW_OA2020:      // BarlowBeestonBanff2020.h:18-85
+    case GCM_LLH_BARLOW_OA2021_SFGD: {  // BarlowBeestonBanff2022Sfgd.h:20-134
+      double fractional;
+      if (c.type == GCM_LLH_BARLOW_OA2020) {
+        fractional = sqrt(err) / pred;
+      } else {
+        const double sfg = c.extra_fraction ? err + pred * c.extra_fraction[b] : err;
+        fractional = sfg / pred;
+      }
+      const double temp = pred * fractional * fractional - 1;
+      const double temp2 = temp * temp + 4 * data * fractional * fractional;
+      if (temp2 < 0) return nan;
+      const double beta = (-1 * temp + sqrt(temp2)) / 2.;
+      const double newmc = pred * beta;
+      const double penalty = (beta - 1) * (beta - 1) / (2 * fractional * fractional);
+      double stat = 0;
+      if (data == 0) stat = newmc;
+      else if (newmc > 0) stat = newmc - data + data * log(data / newmc);
+      double chisq = 0.;
+      if (c.type == GCM_LLH_BARLOW_OA2020 && isnan(penalty)) return inf;
+      if (pred > 0.0 && data > 0.0) chisq += 2.0 * (stat + penalty);
+      else if (pred > 0.0) chisq += 2.0 * (stat + penalty);
+      if (c.type == GCM_LLH_BARLOW_OA2020 && isnan(chisq)) return nan;
+      return chisq;
+    }
+    case GCM_LLH_CHI_SQUARED: {  // ChiSquared.h:19-29
+      if (pred == 0) return inf;
+      const double d = pred - data;
+      return d * d / pred;
+    }
+    case GCM_LLH_LEAST_SQUARES: {  // LeastSquares.h:45-52
+      double v = data - pred;
+      v = v * v;
+      if (c.lsq_poisson && data > 1.0) v /= 0.5 * data;
+      return v;
+    }
+    default:
+      return nan;
   }
-  double stat = 0;
-  if (data <= deps) {
-    stat = newmc;
-  } else if (newmc > dmin) {
-    stat = newmc - data + data * (log(data) - log(newmc));
-  } else {
-    stat = dmin - data + data * (log(data) - log(dmin));
-  }
-  double chisq = 2.0 * stat;
-  if (!c.use_poisson) chisq += 2.0 * penalty;
-  return chisq;
 }
 
 // Wave-level sum of (s, c) pairs with a fixed butterfly shape: the result is
@@ -559,7 +630,7 @@ __global__ void __launch_bounds__(kFinalizeBlock)
     if (lane == 0) {
       hist[(size_t)k * n_bins + b] = s;
       hist2[(size_t)k * n_bins + b] = s2;
-      if (data) wave_llh += llh_bin(cfg, data[b], s, sqrt(s2));
+      if (data) wave_llh += llh_bin(cfg, b, data[b], s, sqrt(s2));
     }
   }
   if (lane == 0) s_red[wave] = wave_llh;
@@ -686,7 +757,10 @@ struct gcm_context {
   int max_dials = 1024;
   int fin_blocks = 1;
   double cap = __builtin_huge_val();
-  LlhParams llh{GCM_LLH_POISSON, 0, 1e150, -1.0};
+  LlhParams llh{GCM_LLH_POISSON, 0, 1e150, -1.0, 0, 0, nullptr, nullptr, nullptr};
+  double* d_llh_nominal = nullptr;   // [n_bins], BBNoUpdateWeights
+  double* d_llh_extra = nullptr;     // [n_bins], SFGD/WAGASCI fractions
+  uint8_t* d_llh_ignore = nullptr;   // [n_bins], zero-prediction-at-prior mask
   std::vector<void*> allocations;
   int64_t resident = 0;
   // device
@@ -1298,8 +1372,10 @@ int gcm_set_likelihood(gcm_context* c, const gcm_llh_config* cfg,
                        const double* prior_sigma) {
   if (!c || !cfg || !data_hist) return fail(GCM_ERR_ARG, "NULL argument");
   if (!c->built) return fail(GCM_ERR_STATE, "gcm_build first");
-  if (cfg->type != GCM_LLH_POISSON && cfg->type != GCM_LLH_BARLOW_OA2021)
+  if (cfg->type < 0 || cfg->type >= GCM_LLH_NTYPES)
     return fail(GCM_ERR_ARG, "unknown likelihood type");
+  if (cfg->bb_no_update_weights && !cfg->nominal_err)
+    return fail(GCM_ERR_ARG, "bb_no_update_weights needs nominal_err");
   GCM_HIP(hipSetDevice(c->device));
   GCM_HIP(hipStreamSynchronize(c->stream));
   c->llh.type = cfg->type;
@@ -1307,6 +1383,27 @@ int gcm_set_likelihood(gcm_context* c, const gcm_llh_config* cfg,
   // BarlowBeestonBanff2022.h:85-88: hard limit on the fractional error
   c->llh.frac_limit = std::min(cfg->fractional_error_limit, 1.0E+152);
   c->llh.expected_min = cfg->expected_value_minimum;
+  c->llh.lsq_poisson = cfg->lsq_poissonian_approximation;
+  c->llh.no_update_weights = cfg->bb_no_update_weights;
+  const size_t nb = (size_t)c->n_bins;
+  c->llh.nominal_err = nullptr;
+  c->llh.extra_fraction = nullptr;
+  c->llh.ignore = nullptr;
+  if (cfg->nominal_err) {
+    if (!c->d_llh_nominal) GCM_TRY(c->alloc(&c->d_llh_nominal, nb));
+    GCM_HIP(hipMemcpy(c->d_llh_nominal, cfg->nominal_err, nb * sizeof(double), hipMemcpyHostToDevice));
+    c->llh.nominal_err = c->d_llh_nominal;
+  }
+  if (cfg->extra_fraction) {
+    if (!c->d_llh_extra) GCM_TRY(c->alloc(&c->d_llh_extra, nb));
+    GCM_HIP(hipMemcpy(c->d_llh_extra, cfg->extra_fraction, nb * sizeof(double), hipMemcpyHostToDevice));
+    c->llh.extra_fraction = c->d_llh_extra;
+  }
+  if (cfg->ignore_bins) {
+    if (!c->d_llh_ignore) GCM_TRY(c->alloc(&c->d_llh_ignore, nb));
+    GCM_HIP(hipMemcpy(c->d_llh_ignore, cfg->ignore_bins, nb, hipMemcpyHostToDevice));
+    c->llh.ignore = c->d_llh_ignore;
+  }
   GCM_HIP(hipMemcpy(c->d_data, data_hist, c->n_bins * sizeof(double), hipMemcpyHostToDevice));
   std::vector<double> mean(c->n_par, 0.0), sig(c->n_par, 0.0);
   if (prior_mean && prior_sigma) {

@@ -109,9 +109,10 @@ class SurrogateTarget:
         return -0.5 * (stat + pen)
 
 
-def var_ratio(sums: np.ndarray, sqrt_sum_w2: np.ndarray) -> np.ndarray:
-    """Per-bin (sum w^2 / sum w) at one point, 0 for empty bins."""
-    return np.where(sums > 0, sqrt_sum_w2 ** 2 / np.where(sums > 0, sums, 1.0), 0.0)
+def var_ratio(sums: np.ndarray, sum_w2: np.ndarray) -> np.ndarray:
+    """Per-bin (sum w^2 / sum w) at one point, 0 for empty bins; the inputs
+    are the two outputs of CacheManager.hist (sum w and sum w^2)."""
+    return np.where(sums > 0, sum_w2 / np.where(sums > 0, sums, 1.0), 0.0)
 
 
 def laplace_scales(exact: ExactTarget, mode: np.ndarray, h: float = 1e-2) -> np.ndarray:

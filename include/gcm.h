@@ -74,6 +74,12 @@ extern "C" {
 /* ---- likelihood types (JointProbability.cpp:61-94) ---------------------- */
 #define GCM_LLH_POISSON 0           /* PoissonLogLikelihood.h:19-53          */
 #define GCM_LLH_BARLOW_OA2021 1     /* BarlowBeestonBanff2022.h:92-294      */
+#define GCM_LLH_BARLOW_BEESTON 2    /* classic, BarlowBeeston.h:25-62       */
+#define GCM_LLH_BARLOW_OA2020 3     /* BarlowBeestonBanff2020.h:18-85       */
+#define GCM_LLH_BARLOW_OA2021_SFGD 4 /* BarlowBeestonBanff2022Sfgd.h:20-134 */
+#define GCM_LLH_CHI_SQUARED 5       /* ChiSquared.h:19-29                    */
+#define GCM_LLH_LEAST_SQUARES 6     /* LeastSquares.h:45-52                  */
+#define GCM_LLH_NTYPES 7
 
 /* One dial input: the parameter reference of a DialInputBuffer with its
  * mirror edges (DialInputBuffer.cpp:44-89), and the DialResponseSupervisor of
@@ -115,13 +121,30 @@ typedef struct gcm_tables {
                                  NaN or +inf: disabled                     */
 } gcm_tables;
 
-/* Likelihood configuration: the BarlowLLH_BANFF_OA2021 options
- * (BarlowBeestonBanff2022.h:29-53) that change arithmetic. */
+/* Likelihood configuration: the options of the reference's joint
+ * probabilities that change arithmetic.  The per-bin arrays are [n_bins] in
+ * global bin order, copied by gcm_set_likelihood; NULL means "not used".
+ * Every type receives, per bin, (data, sum w, sqrt(sum w^2)) as
+ * JointProbabilityBase::eval(SamplePair, bin) passes them
+ * (JointProbabilityBase.h:37-43); the OA2021 types square the uncertainty
+ * themselves (BarlowBeestonBanff2022.h:125-126), the classic and OA2020 ones
+ * use it as given. */
 typedef struct gcm_llh_config {
   int32_t type;                   /* GCM_LLH_*                              */
-  int32_t use_poisson_likelihood; /* BB: drop the beta penalty              */
-  double fractional_error_limit;  /* BB default 1e150                        */
-  double expected_value_minimum;  /* BB default -1                           */
+  int32_t use_poisson_likelihood; /* OA2021: drop the beta penalty          */
+  double fractional_error_limit;  /* OA2021, default 1e150 (capped 1e152)   */
+  double expected_value_minimum;  /* OA2021, default -1                     */
+  int32_t lsq_poissonian_approximation; /* LeastSquares.h:28, 50          */
+  int32_t bb_no_update_weights;   /* OA2021 BBNoUpdateWeights (:36, 109-123):
+                                     uncertainty from nominal_err          */
+  const double* nominal_err;      /* sqrt(sum w^2) of the nominal MC        */
+  const double* extra_fraction;   /* OA2021_SFGD: the sample's detector
+                                     fraction, added as pred x fraction to
+                                     sqrt(sum w^2) (Sfgd.h:28-69; give the one
+                                     nonzero SFGD or WAGASCI value)         */
+  const uint8_t* ignore_bins;     /* nonzero: the bin contributes 0
+                                     (ignoreBinsWithZeroPredictionAtPrior,
+                                     JointProbabilityBase.h:56-77)          */
 } gcm_llh_config;
 
 /* Kernel launch shape.  Zero fields mean "use the built-in default".  The

@@ -14,6 +14,11 @@
 //     Histogram::refillHistogram                   Histogram.cpp:107-135
 //   LLH: PoissonLogLikelihood::eval                PoissonLogLikelihood.h:19-53
 //        BarlowBeestonBanff2022::eval              BarlowBeestonBanff2022.h:92-294
+//        BarlowBeeston::eval (classic)             BarlowBeeston.h:25-62
+//        BarlowBeestonBanff2020::eval              BarlowBeestonBanff2020.h:18-85
+//        BarlowBeestonBanff2022Sfgd::eval          BarlowBeestonBanff2022Sfgd.h:20-134
+//        ChiSquared::eval, LeastSquares::eval      ChiSquared.h:19-29, LeastSquares.h:45-52
+//        zero-prediction-at-prior bin mask         JointProbabilityBase.h:56-77
 //        LikelihoodInterface penalty + total       LikelihoodInterface.cpp:165-215
 //   Event -> bin: LoaderUtils::fillBinIndex        LoaderUtils.cpp:32-40
 //                 Bin::isBetweenEdges              Bin.cpp:167-176
@@ -498,6 +503,99 @@ double oracle_llh_barlow_bin(double data, double pred, double err,
   return chisq;
 }
 
+// BarlowBeeston.h:33-62 ("classic"; err is sqrtSumSqWeights as the base
+// class passes it, used as a variance).
+double oracle_llh_barlow_classic_bin(double data, double pred, double err) {
+  const double dataVal = std::max(data, 0.0);
+  const double predVal = std::max(pred, std::sqrt(std::numeric_limits<double>::min()));
+  const double mcUncert = std::max(err, std::numeric_limits<double>::min());
+  const double rel_var = mcUncert / (predVal * predVal);
+  const double b = (predVal * rel_var) - 1;
+  const double c = 4 * dataVal * rel_var;
+  const double beta = (-b + std::sqrt(b * b + c)) / 2.0;
+  const double mc_hat = predVal * beta;
+  double chi2 = 0.0;
+  if (dataVal <= 0.0) {
+    chi2 = 2 * mc_hat;
+    chi2 += (beta - 1) * (beta - 1) / rel_var;
+  } else {
+    chi2 = 2 * (mc_hat - dataVal);
+    if (dataVal > 0.0) chi2 += 2 * dataVal * (std::log(dataVal) - std::log(mc_hat));
+    chi2 += (beta - 1) * (beta - 1) / rel_var;
+  }
+  return chi2;
+}
+
+// BarlowBeestonBanff2020.h:18-85 (fractional = sqrt(err) / pred, err as the
+// base class passes it) and BarlowBeestonBanff2022Sfgd.h:20-134 (fractional
+// = (err + pred x detector fraction) / pred).  The reference throws where
+// this returns NaN.
+double oracle_llh_barlow_banff_bin(int type, double data, double pred, double err,
+                                   double extra_fraction) {
+  double fractional;
+  if (type == GCM_LLH_BARLOW_OA2020) {
+    fractional = std::sqrt(err) / pred;
+  } else {
+    const double sfgUncert = err + pred * extra_fraction;
+    fractional = sfgUncert / pred;
+  }
+  const double temp = pred * fractional * fractional - 1;
+  const double temp2 = temp * temp + 4 * data * fractional * fractional;
+  if (temp2 < 0) return std::numeric_limits<double>::quiet_NaN();
+  const double beta = (-1 * temp + std::sqrt(temp2)) / 2.;
+  const double newmc = pred * beta;
+  const double penalty = (beta - 1) * (beta - 1) / (2 * fractional * fractional);
+  double stat = 0;
+  if (data == 0) stat = newmc;
+  else if (newmc > 0) stat = newmc - data + data * std::log(data / newmc);
+  if (type == GCM_LLH_BARLOW_OA2020 && std::isnan(penalty))
+    return std::numeric_limits<double>::infinity();
+  double chisq = 0.;
+  if ((pred > 0.0) && (data > 0.0)) chisq += 2.0 * (stat + penalty);
+  else if (pred > 0.0) chisq += 2.0 * (stat + penalty);
+  if (type == GCM_LLH_BARLOW_OA2020 && std::isnan(chisq))
+    return std::numeric_limits<double>::quiet_NaN();
+  return chisq;
+}
+
+// ChiSquared.h:19-29.
+double oracle_llh_chi2_bin(double data, double pred) {
+  if (pred == 0) return std::numeric_limits<double>::infinity();
+  return (pred - data) * (pred - data) / pred;
+}
+
+// LeastSquares.h:45-52.
+double oracle_llh_least_squares_bin(double data, double pred, int poissonian) {
+  double v = data - pred;
+  v = v * v;
+  if (poissonian && data > 1.0) v /= 0.5 * data;
+  return v;
+}
+
+// One bin of any type: JointProbabilityBase::eval(SamplePair, bin)
+// (JointProbabilityBase.h:37-43) with the bin mask of :74-77 and the OA2021
+// BBNoUpdateWeights switch (BarlowBeestonBanff2022.h:109-123).
+double oracle_llh_bin(const gcm_llh_config* cfg, int b, double data, double pred,
+                      double err) {
+  if (cfg->ignore_bins && cfg->ignore_bins[b]) return 0.0;
+  switch (cfg->type) {
+    case GCM_LLH_POISSON: return oracle_llh_poisson_bin(data, pred);
+    case GCM_LLH_BARLOW_OA2021:
+      return oracle_llh_barlow_bin(
+          data, pred, (cfg->bb_no_update_weights && cfg->nominal_err) ? cfg->nominal_err[b] : err,
+          cfg);
+    case GCM_LLH_BARLOW_BEESTON: return oracle_llh_barlow_classic_bin(data, pred, err);
+    case GCM_LLH_BARLOW_OA2020:
+    case GCM_LLH_BARLOW_OA2021_SFGD:
+      return oracle_llh_barlow_banff_bin(cfg->type, data, pred, err,
+                                         cfg->extra_fraction ? cfg->extra_fraction[b] : 0.0);
+    case GCM_LLH_CHI_SQUARED: return oracle_llh_chi2_bin(data, pred);
+    case GCM_LLH_LEAST_SQUARES:
+      return oracle_llh_least_squares_bin(data, pred, cfg->lsq_poissonian_approximation);
+    default: return std::numeric_limits<double>::quiet_NaN();
+  }
+}
+
 // JointProbabilityBase::eval(SamplePair) summed over bins in order, plus the
 // diagonal-prior penalty of LikelihoodInterface.cpp:179-215; total =
 // stat + penalty.
@@ -507,12 +605,8 @@ int oracle_llh(const gcm_llh_config* cfg, int n_bins, const double* data,
                const double* prior_sigma, double* total, double* stat_out,
                double* penalty_out) {
   double stat = 0;
-  for (int b = 0; b < n_bins; ++b) {
-    if (cfg->type == GCM_LLH_POISSON)
-      stat += oracle_llh_poisson_bin(data[b], sums[b]);
-    else
-      stat += oracle_llh_barlow_bin(data[b], sums[b], sqrt_sum_sq[b], cfg);
-  }
+  for (int b = 0; b < n_bins; ++b)
+    stat += oracle_llh_bin(cfg, b, data[b], sums[b], sqrt_sum_sq[b]);
   double pen = 0;
   for (int p = 0; p < n_par; ++p) {
     if (!prior_sigma || !(prior_sigma[p] > 0)) continue;

@@ -21,7 +21,15 @@ _P, _I32, _I64, _D = C.c_void_p, C.c_int32, C.c_int64, C.c_double
 class LlhConfig(C.Structure):
     _fields_ = [("type", C.c_int32), ("use_poisson_likelihood", C.c_int32),
                 ("fractional_error_limit", C.c_double),
-                ("expected_value_minimum", C.c_double)]
+                ("expected_value_minimum", C.c_double),
+                ("lsq_poissonian_approximation", C.c_int32),
+                ("bb_no_update_weights", C.c_int32),
+                ("nominal_err", C.c_void_p), ("extra_fraction", C.c_void_p),
+                ("ignore_bins", C.c_void_p)]
+
+
+LLH_KINDS = {"poisson": 0, "barlow": 1, "barlow_oa2021": 1, "barlow_classic": 2,
+             "barlow_oa2020": 3, "barlow_oa2021_sfgd": 4, "chi2": 5, "least_squares": 6}
 
 
 _lib = None
@@ -47,6 +55,8 @@ def lib() -> C.CDLL:
         l.oracle_llh_poisson_bin.argtypes = [_D, _D]
         l.oracle_llh_barlow_bin.restype = _D
         l.oracle_llh_barlow_bin.argtypes = [_D, _D, _D, C.POINTER(LlhConfig)]
+        l.oracle_llh_bin.restype = _D
+        l.oracle_llh_bin.argtypes = [C.POINTER(LlhConfig), C.c_int, _D, _D, _D]
         l.oracle_llh.argtypes = [C.POINTER(LlhConfig), C.c_int, _P, _P, _P,
                                  C.c_int, _P, _P, _P, _P, _P, _P]
         l.oracle_bin_events.argtypes = [_I64, C.c_int, _P, C.c_int, _P, _P]
@@ -57,9 +67,32 @@ def lib() -> C.CDLL:
 
 
 def llh_config(kind: str = "poisson", use_poisson_likelihood=False,
-               fractional_error_limit=1.0e150, expected_value_minimum=-1.0):
-    return LlhConfig(0 if kind == "poisson" else 1, int(use_poisson_likelihood),
-                     float(fractional_error_limit), float(expected_value_minimum))
+               fractional_error_limit=1.0e150, expected_value_minimum=-1.0,
+               lsq_poissonian_approximation=False, nominal_err=None,
+               extra_fraction=None, ignore_bins=None):
+    """The C configuration; per-bin arrays are kept alive on the returned
+    object (attribute _keep)."""
+    keep = []
+
+    def ptr(a, dt):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=dt)
+        keep.append(a)
+        return a.ctypes.data
+    cfg = LlhConfig(LLH_KINDS[kind], int(use_poisson_likelihood),
+                    float(fractional_error_limit), float(expected_value_minimum),
+                    int(lsq_poissonian_approximation), int(nominal_err is not None),
+                    ptr(nominal_err, np.float64), ptr(extra_fraction, np.float64),
+                    ptr(ignore_bins, np.uint8))
+    cfg._keep = keep
+    return cfg
+
+
+def llh_bin(kind, b, data, pred, err, **opts):
+    """One bin of the likelihood `kind` (see LLH_KINDS)."""
+    cfg = llh_config(kind, **opts)
+    return lib().oracle_llh_bin(C.byref(cfg), int(b), float(data), float(pred), float(err))
 
 
 class Oracle:
@@ -102,6 +135,7 @@ class Oracle:
 
 def llh(kind, data, sums, sqrt_sum_sq, theta=None, prior_mean=None,
         prior_sigma=None, **opts):
+    """(total, stat, penalty) of the likelihood `kind` (see LLH_KINDS)."""
     cfg = llh_config(kind, **opts)
     data = np.ascontiguousarray(data, dtype=np.float64)
     sums = np.ascontiguousarray(sums, dtype=np.float64)

@@ -4,6 +4,7 @@ deterministic.  No device compute here."""
 import ctypes as C
 import os
 import re
+import subprocess
 
 import numpy as np
 import pytest
@@ -95,3 +96,27 @@ def test_cpp_facade_compiles_and_links():
                             "-L" + os.path.join(ROOT, "gundam_amd"), "-lgcm", "-lgcm_synth",
                             "-lpthread"], capture_output=True, text=True)
         assert r.returncode == 0, r.stderr
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    """The ctypes mirrors in _native.py have the sizes and field offsets the
+    C compiler gives the structs of include/gcm.h."""
+    structs = {"gcm_input": N.GcmInput, "gcm_tables": N.GcmTables,
+               "gcm_llh_config": N.GcmLlhConfig, "gcm_launch_config": N.GcmLaunchConfig}
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "gcm.h"', "int main(void) {"]
+    for cname, py in structs.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'  printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.run(["gcc", "-std=c11", "-I", inc, "-o", str(exe), str(src)], check=True)
+    out = subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout
+    for line in out.splitlines():
+        cname, field, val = line.split()
+        py = structs[cname]
+        want = C.sizeof(py) if field == "size" else getattr(py, field).offset
+        assert int(val) == want, (cname, field, int(val), want)

@@ -234,3 +234,40 @@ def test_segment_search_edge_cases():
             np.testing.assert_array_equal(w, w_ref, err_msg=f"x={x!r}")
         cm.close()
     orc.close()
+
+
+@pytest.mark.parametrize("kind,opts", [
+    ("poisson", {}), ("barlow", {}), ("barlow", {"nominal": True}),
+    ("barlow", {"use_poisson_likelihood": True}), ("barlow_classic", {}),
+    ("barlow_oa2020", {}), ("barlow_oa2021_sfgd", {"extra": True}), ("chi2", {}),
+    ("least_squares", {}), ("least_squares", {"lsq_poissonian_approximation": True})])
+def test_llh_variants_match_oracle(kind, opts):
+    """Every likelihood type and option on the device against the oracle on
+    the same histogram (stat within 1e-10 relative), with bins that have no
+    prediction masked as ignoreBinsWithZeroPredictionAtPrior does."""
+    print("llh variant", kind, opts, flush=True)
+    wl, t = _setup(1, 0)
+    orc = pyoracle.Oracle(t)
+    theta = wl.random_points(1, 0.3, seed=9)[0]
+    _, s_ref, e_ref = orc.propagate(theta)
+    rng = np.random.default_rng(4)
+    data = rng.poisson(np.maximum(s_ref, 0) * 1.05 + 0.2).astype(float)
+    o = dict(opts)
+    kw = {"ignore_bins": (s_ref == 0).astype(np.uint8)}
+    if o.pop("nominal", False):
+        kw["nominal_err"] = 1.1 * e_ref + 0.01
+    if o.pop("extra", False):
+        kw["extra_fraction"] = np.where(np.arange(t.n_bins) % 3 == 0, 0.04, 0.0)
+    kw.update(o)
+    cm = CacheManager(0)
+    cm.build(t)
+    cm.set_likelihood(kind, data, wl.prior_mean, wl.prior_sigma, **kw)
+    cm.propagate_parameters(theta)
+    _, st, pen = cm.llh()
+    _, st_ref, pen_ref = pyoracle.llh(kind, data, s_ref, e_ref, theta, wl.prior_mean,
+                                      wl.prior_sigma, **kw)
+    assert np.isfinite(st_ref)
+    assert abs(st[0] - st_ref) <= RTOL_LLH * abs(st_ref), (st[0], st_ref)
+    assert pen[0] == pen_ref
+    cm.close()
+    orc.close()

@@ -75,6 +75,13 @@ def test_surrogate_delayed_acceptance_gain():
     data, err = cm.hist(0)
     data = data.copy()
     ratio = var_ratio(data, err)
+    # the surrogate's MC-variance model (content x nominal sum w^2 / sum w)
+    # reproduces the device's Barlow-Beeston statistic at the nominal point
+    off = 1.1 * data + 0.5
+    cm.set_likelihood("barlow", off, wl.prior_mean, wl.prior_sigma)
+    cm.propagate_parameters(mode)
+    np.testing.assert_allclose(barlow_llh(off, data[None, :], (data * ratio)[None, :])[0],
+                               cm.llh()[1][0], rtol=1e-10)
     cm.set_likelihood("barlow", data, wl.prior_mean, wl.prior_sigma)
     exact = ExactTarget(cm)
     sc = laplace_scales(exact, mode)
