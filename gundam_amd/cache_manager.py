@@ -142,6 +142,30 @@ class CacheManager:
             None if pm is None else pm.ctypes.data,
             None if ps is None else ps.ctypes.data))
 
+    def gradient(self, theta: np.ndarray, h: float = 1e-3, params=None,
+                 table_values=None):
+        """Batched central-difference gradient of the total -2 ln L for a
+        minimiser: theta and theta +- h e_j for every j in `params` (default:
+        all) go out in launches of up to max_points points (the reference's
+        RootMinimizer gets Minuit's numerical gradient, one evalFit per point,
+        RootMinimizer.cpp:125-126).  table_values(points) -> [k, table_len]
+        when the lookup tables depend on the parameters.  Returns
+        (llh(theta), grad)."""
+        theta = np.asarray(theta, dtype=np.float64)
+        params = np.arange(theta.size) if params is None else np.asarray(params, dtype=int)
+        pts = np.repeat(theta[None, :], 2 * params.size + 1, axis=0)
+        pts[1 + 2 * np.arange(params.size), params] += h
+        pts[2 + 2 * np.arange(params.size), params] -= h
+        llh = np.empty(pts.shape[0])
+        for i in range(0, pts.shape[0], self.max_points):
+            chunk = pts[i:i + self.max_points]
+            self.set_parameters(chunk)
+            if table_values is not None and self.table_len:
+                self.set_table_values(table_values(chunk))
+            self.fill().get()
+            llh[i:i + chunk.shape[0]] = self.llh()[0][:chunk.shape[0]]
+        return llh[0], (llh[1::2] - llh[2::2]) / (2 * h)
+
     def zero_prediction_mask(self, k: int = 0) -> np.ndarray:
         """Bins whose current prediction (point k) is exactly zero: the
         ignoreBinsWithZeroPredictionAtPrior flags when called at the prior

@@ -3,6 +3,8 @@
 // takes the Asimov histogram at nominal as data, and evaluates the Poisson
 // LLH at a few parameter points, the way GUNDAM's Propagator +
 // LikelihoodInterface would drive the cache.
+#include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <random>
 #include <vector>
@@ -39,7 +41,7 @@ int main() {
   t.reweight_cap = v.reweight_cap;
 
   gcm::CacheManager cm(0);
-  cm.Build(t);
+  cm.Build(t, 16);  // up to 16 points per launch (the gradient below)
   std::vector<double> nominal(v.prior_mean, v.prior_mean + v.n_parameters);
   cm.PropagateParameters(nominal);
   std::vector<double> data = cm.SumWeights();  // Asimov data
@@ -59,6 +61,31 @@ int main() {
     const double l = cm.Likelihood()[0];
     total += l;
     std::printf("point %d: LLH %.10g\n", i, l);
+  }
+  // a Minuit-style gradient over the first five parameters in one batched
+  // launch of 11 points, checked against single-point evaluations
+  std::vector<int> params{0, 1, 2, 3, 4};
+  std::vector<double> grad;
+  std::vector<double> th0(nominal);
+  for (int p = 0; p < v.n_parameters; ++p) th0[p] += 0.2 * v.prior_sigma[p];
+  const double h = 1e-3;
+  const double l0 = cm.Gradient(th0, h, params, &grad);
+  double worst = 0;
+  for (int j = 0; j < 5; ++j) {
+    std::vector<double> tp(th0), tm(th0);
+    tp[params[j]] += h;
+    tm[params[j]] -= h;
+    cm.Fill(tp).get();
+    const double lp = cm.Likelihood()[0];
+    cm.Fill(tm).get();
+    const double lm = cm.Likelihood()[0];
+    worst = std::max(worst, std::fabs((lp - lm) / (2 * h) - grad[j]));
+  }
+  std::printf("gradient at LLH %.10g: d/dp0 %.6g ... max |batched - single| %.3g\n",
+              l0, grad[0], worst);
+  if (worst != 0.0) {
+    std::printf("batched gradient differs from single-point evaluations\n");
+    return 1;
   }
   std::printf("resident %.1f MiB, nominal LLH check %.3g\n",
               cm.GetResidentMemory() / 1048576.0, total);

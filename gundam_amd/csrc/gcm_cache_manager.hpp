@@ -7,8 +7,10 @@
 // passes are the EventDialCache content laid out as gcm_tables.
 #pragma once
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <functional>
 #include <future>
 #include <stdexcept>
 #include <string>
@@ -55,6 +57,7 @@ class CacheManager {
   // Cache::Manager::Build + Update (CacheManager.cpp:198-740)
   bool Build(const gcm_tables& tables, int max_points = 1) {
     check(gcm_build(ctx_, &tables, max_points), "gcm_build");
+    max_points_ = max_points;
     n_bins_ = tables.n_bins;
     n_events_ = tables.n_events;
     n_par_ = tables.n_parameters;
@@ -127,6 +130,48 @@ class CacheManager {
     return tot;
   }
 
+  // Batched central-difference gradient of the total -2 ln L for Minuit:
+  // the points theta and theta +- h e_j for every j in `params` go out in
+  // launches of up to max_points points (the reference's RootMinimizer asks
+  // Minuit for a numerical gradient, one evalFit per point,
+  // RootMinimizer.cpp:125-126).  table_fn(points, k, values) fills the
+  // lookup tables of k points (k x n_parameters, row-major) when the tables
+  // depend on the parameters.  Returns -2 ln L at theta; grad[i] is the
+  // derivative along params[i].
+  using TableFn = std::function<void(const std::vector<double>&, int, std::vector<double>&)>;
+  double Gradient(const std::vector<double>& theta, double h, const std::vector<int>& params,
+                  std::vector<double>* grad, const TableFn& table_fn = {}) {
+    const int m = (int)params.size();
+    const int n = 2 * m + 1;
+    std::vector<double> pts((size_t)n * n_par_), llh(n), tab;
+    for (int i = 0; i < n; ++i) std::copy(theta.begin(), theta.end(), pts.begin() + (size_t)i * n_par_);
+    for (int j = 0; j < m; ++j) {
+      pts[(size_t)(1 + 2 * j) * n_par_ + params[j]] += h;
+      pts[(size_t)(2 + 2 * j) * n_par_ + params[j]] -= h;
+    }
+    for (int i0 = 0; i0 < n; i0 += max_points_) {
+      const int k = std::min(max_points_, n - i0);
+      std::vector<double> chunk(pts.begin() + (size_t)i0 * n_par_,
+                                pts.begin() + (size_t)(i0 + k) * n_par_);
+      const std::vector<double>* tv = nullptr;
+      if (table_fn && table_len_) {
+        table_fn(chunk, k, tab);
+        tv = &tab;
+      }
+      const bool hc = hist_copy_, wc = weight_copy_;
+      hist_copy_ = weight_copy_ = false;
+      Fill(chunk, k, tv).get();
+      hist_copy_ = hc;
+      weight_copy_ = wc;
+      check(gcm_get_llh(ctx_, llh.data() + i0, nullptr, nullptr), "gcm_get_llh");
+    }
+    if (grad) {
+      grad->resize(m);
+      for (int j = 0; j < m; ++j) (*grad)[j] = (llh[1 + 2 * j] - llh[2 + 2 * j]) / (2 * h);
+    }
+    return llh[0];
+  }
+
   std::size_t GetResidentMemory() const {
     int64_t b = 0;
     gcm_resident_bytes(ctx_, &b);
@@ -143,6 +188,7 @@ class CacheManager {
   bool built_ = false;
   bool hist_copy_ = true;  // fIsHistContentCopyEnabled default true
   bool weight_copy_ = false;
+  int max_points_ = 1;
   int n_bins_ = 0, n_par_ = 0;
   int64_t n_events_ = 0, table_len_ = 0;
   std::vector<double> sum_w_, sqrt_sum_w2_, weights_;

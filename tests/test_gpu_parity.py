@@ -271,3 +271,42 @@ def test_llh_variants_match_oracle(kind, opts):
     assert pen[0] == pen_ref
     cm.close()
     orc.close()
+
+
+def test_batched_gradient_matches_single_points_and_oracle():
+    """The batched central-difference gradient (2m+1 points in launches of up
+    to max_points) equals the one formed from single-point evaluations bit
+    for bit, and the oracle's to the LLH tolerance."""
+    print("batched gradient workload=2", flush=True)
+    wl, t = _setup(2, 20000)
+    cm = CacheManager(0)
+    cm.build(t, max_points=8)
+    nominal = wl.nominal()
+    cm.propagate_parameters(nominal, wl.table_values(nominal[None])[0])
+    data = cm.hist(0)[0] * 1.02 + 0.5
+    cm.set_likelihood("barlow", data, wl.prior_mean, wl.prior_sigma)
+    theta = wl.random_points(1, 0.3, seed=11)[0]
+    params = [0, 5, 40, 120, 299]
+    h = 1e-3
+    l0, g = cm.gradient(theta, h, params, wl.table_values)
+    orc = pyoracle.Oracle(t)
+    single, ref = [], []
+    for p in params:
+        vals, rvals = [], []
+        for sgn in (1, -1):
+            th = theta.copy()
+            th[p] += sgn * h
+            tab = wl.table_values(th[None])[0]
+            cm.propagate_parameters(th, tab)
+            vals.append(cm.llh()[0][0])
+            orc.set_table_values(tab)
+            _, s, e = orc.propagate(th, want_weights=False)
+            rvals.append(pyoracle.llh("barlow", data, s, e, th, wl.prior_mean,
+                                      wl.prior_sigma)[0])
+        single.append((vals[0] - vals[1]) / (2 * h))
+        ref.append((rvals[0] - rvals[1]) / (2 * h))
+    np.testing.assert_array_equal(g, np.array(single))
+    tol = 2 * RTOL_LLH * abs(l0) / (2 * h)
+    assert np.all(np.abs(g - np.array(ref)) <= tol), (g, ref, tol)
+    cm.close()
+    orc.close()
