@@ -32,8 +32,10 @@ DIAL_GRAPH = 5
 DIAL_TABULATED = 6
 DIAL_KRIGED = 7
 DIAL_SHIFT = 8
+DIAL_BILINEAR = 9
+DIAL_BICUBIC = 10
 DIAL_NAMES = ["norm", "monotonic", "uniform", "general", "compact", "graph",
-              "tabulated", "kriged", "shift"]
+              "tabulated", "kriged", "shift", "bilinear", "bicubic"]
 
 INPUT_NO_EXTRAPOLATION = 1
 PRECISION_FP64 = 0
@@ -68,7 +70,7 @@ class GcmTables(C.Structure):
                 ("n_inputs", C.c_int32), ("inputs", C.c_void_p),
                 ("n_tables", C.c_int32), ("table_begin", C.c_void_p),
                 ("n_parameters", C.c_int32), ("n_bins", C.c_int32),
-                ("reweight_cap", C.c_double)]
+                ("reweight_cap", C.c_double), ("dial_input2", C.c_void_p)]
 
 
 class GcmLlhConfig(C.Structure):
@@ -99,7 +101,8 @@ class GcmSynthView(C.Structure):
                 ("dial_input", C.c_void_p), ("data_begin", C.c_void_p),
                 ("dial_data", C.c_void_p), ("inputs", C.c_void_p),
                 ("table_begin", C.c_void_p), ("prior_mean", C.c_void_p),
-                ("prior_sigma", C.c_void_p), ("par_type", C.c_void_p)]
+                ("prior_sigma", C.c_void_p), ("par_type", C.c_void_p),
+                ("dial_input2", C.c_void_p)]
 
 
 _P = C.c_void_p

@@ -102,8 +102,8 @@ __device__ __forceinline__ R spline_uniform(R x, const R* d, int dim) {
 // and graphs whose knot abscissae are not non-decreasing (or hold a NaN) keep
 // the reference's binary search; sorted ones use the counting search below,
 // which returns the same segment index.
-constexpr int kTypeGeneralUnsorted = 9;
-constexpr int kTypeGraphUnsorted = 10;
+constexpr int kTypeGeneralUnsorted = 11;
+constexpr int kTypeGraphUnsorted = 12;
 
 // #{ j in [first, end) : x > X[stride j] }, with the loads of a batch issued
 // together (independent LDS reads instead of a chain of dependent ones).
@@ -173,6 +173,121 @@ __device__ __forceinline__ R graph_linear(R x, const R* d, int dim) {
   const R fx = (x - x1) / step;
   const R m = p2 - p1;
   return clamp_value<R>(p1 + fx * m, R(-1E20), R(1E20));
+}
+
+// 2-D dials on a grid: d = [nx, ny, x knots (nx), y knots (ny), values
+// (nx x ny, x-major)], the payload of Bilinear.cpp:5-32 / Bicubic.cpp:5-32.
+// The four-step index search of CalculateBilinearInterpolation.h:36-45 (the
+// same in CalculateBicubicSpline.h): offsets 8, 4, 2, 1, never past 15; the
+// comparison is skipped when it cannot count (the reference multiplies it by
+// zero after reading a[i + off]), which gives the same index without reading
+// past the knots.
+template <typename R>
+__device__ __forceinline__ int grid_index(R v, const R* a, int na) {
+  int i = (8 < na && !(v < a[8])) ? 8 : 0;
+  i += (i + 4 < na && !(v < a[i + 4])) ? 4 : 0;
+  i += (i + 2 < na && !(v < a[i + 2])) ? 2 : 0;
+  i += (i + 1 < na && !(v < a[i + 1])) ? 1 : 0;
+  return i;
+}
+
+template <typename R>
+__device__ __forceinline__ R lerp_cint(R x, R x0, R p0, R x1, R p1) {
+  return p0 + (p1 - p0) * (x - x0) / (x1 - x0);
+}
+
+// CalculateBilinearInterpolation.h:47-152.
+template <typename R>
+__device__ __forceinline__ R grid_bilinear(R x, R y, const R* d) {
+  const int nx = (int)d[0], ny = (int)d[1];
+  const R* xx = d + 2;
+  const R* yy = xx + nx;
+  const R* k = yy + ny;
+  const int ix = grid_index<R>(x, xx, nx - 1);
+  const int iy = grid_index<R>(y, yy, ny - 1);
+  const R v0 = lerp_cint<R>(x, xx[ix], k[ix * ny + iy], xx[ix + 1], k[(ix + 1) * ny + iy]);
+  const R v1 = lerp_cint<R>(x, xx[ix], k[ix * ny + iy + 1], xx[ix + 1], k[(ix + 1) * ny + iy + 1]);
+  return clamp_value<R>(lerp_cint<R>(y, yy[iy], v0, yy[iy + 1], v1), R(-1E20), R(1E20));
+}
+
+// The Catmull-Rom Hermite through four points, Horner form of
+// CalculateBicubicSpline.h (BICUBIC_USE_HORNER_FACTORIZATION).
+template <typename R>
+__device__ __forceinline__ R cubic_cint(R x, R x0, R p0, R x1, R p1, R x2, R p2, R x3, R p3) {
+  const R step = x2 - x1;
+  const R fx = (x - x1) / step;
+  R m1 = step * (p2 - p0) / (x2 - x0);
+  R m2 = step * (p3 - p1) / (x3 - x1);
+  if (fx < R(0)) m2 = m1;
+  if (fx > R(1)) m1 = m2;
+  return ((((R(-2) * p2 + R(2) * p1 + m2 + m1) * fx + R(3) * p2 - R(3) * p1 - m2 - R(2) * m1) * fx +
+           m1) * fx + p1);
+}
+
+// CalculateBicubicSpline.h: the 4 x 4 neighbourhood of (ix, iy); rows past
+// the grid are extrapolated linearly in y from the two nearest rows, columns
+// past it linearly in x from the two nearest columns (of those values).
+template <typename R>
+__device__ __forceinline__ R grid_value(const R* k, int ny, int nx, int ix, int iy, int c, int r) {
+  auto row = [&](int cc) -> R {
+    if (r < 0) return R(2) * k[cc * ny + iy] - k[cc * ny + iy + 1];
+    if (r > ny - 1) return R(2) * k[cc * ny + iy + 1] - k[cc * ny + iy];
+    return k[cc * ny + r];
+  };
+  if (c < 0) return R(2) * row(ix) - row(ix + 1);
+  if (c > nx - 1) return R(2) * row(ix + 1) - row(ix);
+  return row(c);
+}
+
+template <typename R>
+__device__ __forceinline__ R grid_bicubic(R x, R y, const R* d) {
+  const int nx = (int)d[0], ny = (int)d[1];
+  const R* xx = d + 2;
+  const R* yy = xx + nx;
+  const R* k = yy + ny;
+  const int ix = grid_index<R>(x, xx, nx - 1);
+  const int iy = grid_index<R>(y, yy, ny - 1);
+  const R a0 = (ix > 0) ? xx[ix - 1] : R(2) * xx[ix] - xx[ix + 1];
+  const R a1 = xx[ix], a2 = xx[ix + 1];
+  const R a3 = (ix < nx - 2) ? xx[ix + 2] : R(2) * xx[ix + 1] - xx[ix];
+  // one row at a time (not unrolled: fewer values live at once)
+  R v0 = 0, v1 = 0, v2 = 0, v3 = 0;
+#pragma unroll 1
+  for (int j = 0; j < 4; ++j) {
+    const int r = iy - 1 + j;
+    const R vj = cubic_cint<R>(x, a0, grid_value<R>(k, ny, nx, ix, iy, ix - 1, r), a1,
+                               grid_value<R>(k, ny, nx, ix, iy, ix, r), a2,
+                               grid_value<R>(k, ny, nx, ix, iy, ix + 1, r), a3,
+                               grid_value<R>(k, ny, nx, ix, iy, ix + 2, r));
+    v0 = (j == 0) ? vj : v0;
+    v1 = (j == 1) ? vj : v1;
+    v2 = (j == 2) ? vj : v2;
+    v3 = (j == 3) ? vj : v3;
+  }
+  const R v[4] = {v0, v1, v2, v3};
+  const R u0 = (iy > 0) ? yy[iy - 1] : R(2) * yy[iy] - yy[iy + 1];
+  const R u3 = (iy < ny - 2) ? yy[iy + 2] : R(2) * yy[iy + 1] - yy[iy];
+  return clamp_value<R>(cubic_cint<R>(y, u0, v[0], yy[iy], v[1], yy[iy + 1], v[2], u3, v[3]),
+                        R(-1E20), R(1E20));
+}
+
+// A 2-D dial (Bilinear.cpp:40-64, Bicubic.cpp:40-64): x, y are the mirrored
+// values of its two inputs; both are clamped to the knot range when
+// extrapolation is off.
+template <typename R>
+__device__ __forceinline__ double eval_dial2d(int type, const R* d, double x_in, double y_in,
+                                              int flags) {
+  R x = (R)x_in, y = (R)y_in;
+  if (flags & GCM_INPUT_NO_EXTRAPOLATION) {
+    const int nx = (int)d[0], ny = (int)d[1];
+    const R* xx = d + 2;
+    const R* yy = xx + nx;
+    if (x < xx[0]) x = xx[0];
+    if (x > xx[nx - 1]) x = xx[nx - 1];
+    if (y < yy[0]) y = yy[0];
+    if (y > yy[ny - 1]) y = yy[ny - 1];
+  }
+  return (double)(type == GCM_DIAL_BICUBIC ? grid_bicubic<R>(x, y, d) : grid_bilinear<R>(x, y, d));
 }
 
 // Packed table entry {u32 slot, f32 weight}.

@@ -70,7 +70,7 @@ struct TileHdr {
   uint16_t n_events;
   uint16_t n_dials;
   uint32_t first_partial;
-  uint32_t pad;
+  uint32_t grid_begin;  // first 2-D dial in the tile's type-sorted order
 };
 static_assert(sizeof(TileHdr) == 32, "tile header is 32 bytes");
 
@@ -259,7 +259,7 @@ __device__ __forceinline__ void dma_header(const TileHdr* hdr, unsigned char* bu
 //                  is p mod (NB-1) HBM -> LDS with DMA loads, NB-1 tiles
 //                  ahead, into a ring of NB buffers.
 // A step is one (tile, point); every wave meets one barrier per step.
-template <typename R, bool kComp, bool kInputsLds, int NB>
+template <typename R, bool kComp, bool kInputsLds, int NB, bool kGrid>
 __global__ void __launch_bounds__(1024, 6) fill_kernel(FillArgs a) {
   constexpr int P = NB - 1;  // producer waves
   extern __shared__ __align__(16) unsigned char lds[];
@@ -419,7 +419,10 @@ __global__ void __launch_bounds__(1024, 6) fill_kernel(FillArgs a) {
         const InputRec* inp = kInputsLds ? s_in + (size_t)k * a.n_inputs
                                          : a.xin + (size_t)k * a.n_inputs;
         const double* tk = a.tab + (size_t)k * a.table_len;
-        for (int d0 = threadIdx.x; d0 < h.n_dials; d0 += eval_threads) {
+        // 1-D dials; the 2-D dials (sorted last) follow in their own loop, so
+        // that their register-hungry grid code stays out of this one
+        const int grid_begin = kGrid ? (int)h.grid_begin : (int)h.n_dials;
+        for (int d0 = threadIdx.x; d0 < grid_begin; d0 += eval_threads) {
           const uint64_t d = s_desc[d0];
           const int in = (int)(d & 0xffff);
           const int type = (int)((d >> 16) & 0xf);
@@ -433,10 +436,40 @@ __global__ void __launch_bounds__(1024, 6) fill_kernel(FillArgs a) {
           else if (r > ir.hi) r = ir.hi;
           resp[dest] = r;
         }
+        for (int d0 = grid_begin + (int)threadIdx.x; kGrid && d0 < h.n_dials;
+             d0 += eval_threads) {
+          const uint64_t d = s_desc[d0];
+          const int type = (int)((d >> 16) & 0xf);
+          const int dest = (int)((d >> 32) & 0xffff);
+          const int off = (int)(d >> 48);
+          const InputRec ir = inp[(int)(d & 0xffff)];
+          // record word 0: the second input (y axis); the payload follows
+          const InputRec ir2 = inp[(int)s_rec[off]];
+          double r = eval_dial2d<R>(type, s_rec + off + 1, ir.x, ir2.x, ir.flags);
+          if (r < ir.lo) r = ir.lo;
+          else if (r > ir.hi) r = ir.hi;
+          resp[dest] = r;
+        }
       }
       lds_barrier();
     }
   }
+}
+
+// The fill kernel instance for a precision, input staging, ring depth and
+// whether the tables hold 2-D dials (kGrid: their loop is compiled in only
+// where needed, so the other workloads keep the leaner kernel).
+template <bool kGrid>
+static const void* fill_fn(bool fp32, bool inputs_lds, int nb) {
+  if (fp32)
+    return nb == 3 ? (inputs_lds ? (const void*)fill_kernel<float, true, true, 3, kGrid>
+                                 : (const void*)fill_kernel<float, true, false, 3, kGrid>)
+                   : (inputs_lds ? (const void*)fill_kernel<float, true, true, 2, kGrid>
+                                 : (const void*)fill_kernel<float, true, false, 2, kGrid>);
+  return nb == 3 ? (inputs_lds ? (const void*)fill_kernel<double, false, true, 3, kGrid>
+                               : (const void*)fill_kernel<double, false, false, 3, kGrid>)
+                 : (inputs_lds ? (const void*)fill_kernel<double, false, true, 2, kGrid>
+                               : (const void*)fill_kernel<double, false, false, 2, kGrid>);
 }
 
 struct LlhParams {
@@ -714,8 +747,10 @@ int device_type_code(int type, int n, const double* p) {
   return type;
 }
 
-int record_words(int type, int n, bool fp32) {
+int record_words(int type, int n, bool fp32, int64_t len) {
   switch (type) {
+    case GCM_DIAL_BILINEAR:
+    case GCM_DIAL_BICUBIC: return (int)(1 + len);  // second input index + payload
     case GCM_DIAL_NORM: return 0;
     case GCM_DIAL_SHIFT: return 1;
     case GCM_DIAL_MONOTONIC:
@@ -758,6 +793,7 @@ struct gcm_context {
   int fin_blocks = 1;
   double cap = __builtin_huge_val();
   LlhParams llh{GCM_LLH_POISSON, 0, 1e150, -1.0, 0, 0, nullptr, nullptr, nullptr};
+  bool has_grid = false;             // the tables hold 2-D dials
   double* d_llh_nominal = nullptr;   // [n_bins], BBNoUpdateWeights
   double* d_llh_extra = nullptr;     // [n_bins], SFGD/WAGASCI fractions
   uint8_t* d_llh_ignore = nullptr;   // [n_bins], zero-prediction-at-prior mask
@@ -854,6 +890,18 @@ int validate_dial(const gcm_tables* t, int64_t d, int* n_out) {
     case GCM_DIAL_GRAPH: n = (int)(len / 2); if (len % 2 || n < 1) return fail(GCM_ERR_ARG, "graph needs 2n values"); break;
     case GCM_DIAL_TABULATED: n = 1; if (len != 3) return fail(GCM_ERR_ARG, "tabulated dial needs table, index, fraction"); break;
     case GCM_DIAL_KRIGED: n = (int)((len - 1) / 2); if (len < 1 || (len - 1) % 2) return fail(GCM_ERR_ARG, "kriged dial needs table + pairs"); break;
+    case GCM_DIAL_BILINEAR:
+    case GCM_DIAL_BICUBIC: {
+      const double* p = t->dial_data + t->data_begin[d];
+      const int64_t nx = len >= 2 ? (int64_t)p[0] : 0, ny = len >= 2 ? (int64_t)p[1] : 0;
+      if (nx < 2 || ny < 2 || len != 2 + nx + ny + nx * ny)
+        return fail(GCM_ERR_ARG, "2-D dial needs nx, ny >= 2 and 2 + nx + ny + nx*ny values");
+      if (!t->dial_input2) return fail(GCM_ERR_ARG, "2-D dial without dial_input2");
+      const int in2 = t->dial_input2[d];
+      if (in2 < 0 || in2 >= t->n_inputs) return fail(GCM_ERR_ARG, "2-D dial second input out of range");
+      n = 0;  // the grid shape is read from the record
+      break;
+    }
     default: return fail(GCM_ERR_UNSUPPORTED, "unsupported dial type " + std::to_string(type));
   }
   if (n > 255) return fail(GCM_ERR_UNSUPPORTED, "more than 255 knots/entries in one dial");
@@ -889,6 +937,8 @@ int build_layout(gcm_context* c, const gcm_tables* t) {
     int n;
     GCM_TRY(validate_dial(t, d, &n));
     knots[d] = (uint8_t)n;
+    if (t->dial_type[d] == GCM_DIAL_BILINEAR || t->dial_type[d] == GCM_DIAL_BICUBIC)
+      c->has_grid = true;
   }
   // stable counting sort of binned events by bin
   std::vector<int64_t> count(t->n_bins + 1, 0);
@@ -960,7 +1010,7 @@ int build_layout(gcm_context* c, const gcm_tables* t) {
       int64_t w = 0;
       const int64_t nd = t->dial_begin[e + 1] - t->dial_begin[e];
       for (int64_t d = t->dial_begin[e]; d < t->dial_begin[e + 1]; ++d)
-        w += record_words(t->dial_type[d], knots[d], fp32);
+        w += record_words(t->dial_type[d], knots[d], fp32, t->data_begin[d + 1] - t->data_begin[d]);
       if (w > cap_words || nd > c->max_dials)
         return fail(GCM_ERR_UNSUPPORTED, "one event's dials exceed the tile capacity; raise tile_words");
       if (cur_events > 0 && (cur_events + 1 > kMaxTileEvents ||
@@ -1019,7 +1069,7 @@ int build_layout(gcm_context* c, const gcm_tables* t) {
       std::vector<int64_t> dial_ids;
       std::vector<int> slot;
       for (int64_t ti = th; ti < T; ti += nthreads) {
-        const TileHdr& h = tiles[ti];
+        TileHdr& h = tiles[ti];
         dial_ids.clear();
         slot.clear();
         int local = 0;
@@ -1050,10 +1100,13 @@ int build_layout(gcm_context* c, const gcm_tables* t) {
         std::stable_sort(idx.begin(), idx.end(),
                          [&](int x, int y) { return sort_key(x) < sort_key(y); });
         int64_t off = 0;
+        uint32_t grid0 = (uint32_t)idx.size();
         for (size_t q = 0; q < idx.size(); ++q) {
           const int64_t d = dial_ids[idx[q]];
           const int type = t->dial_type[d];
           const int n = knots[d];
+          if ((type == GCM_DIAL_BILINEAR || type == GCM_DIAL_BICUBIC) && grid0 > q)
+            grid0 = (uint32_t)q;
           const double* p = t->dial_data + t->data_begin[d];
           R* out = rec.data() + h.rec_begin + off;
           switch (type) {
@@ -1079,6 +1132,13 @@ int build_layout(gcm_context* c, const gcm_tables* t) {
               }
               break;
             }
+            case GCM_DIAL_BILINEAR:
+            case GCM_DIAL_BICUBIC: {
+              const int64_t len = t->data_begin[d + 1] - t->data_begin[d];
+              out[0] = (R)t->dial_input2[d];
+              for (int64_t j = 0; j < len; ++j) out[1 + j] = (R)p[j];
+              break;
+            }
             default: {
               const int64_t len = t->data_begin[d + 1] - t->data_begin[d];
               for (int64_t j = 0; j < len; ++j) out[j] = (R)p[j];
@@ -1088,8 +1148,9 @@ int build_layout(gcm_context* c, const gcm_tables* t) {
           const int code = device_type_code(type, n, p);
           desc[h.first_dial + q] = in | ((uint64_t)code << 16) | ((uint64_t)n << 20) |
                                    ((uint64_t)slot[idx[q]] << 32) | ((uint64_t)off << 48);
-          off += record_words(type, n, fp32);
+          off += record_words(type, n, fp32, t->data_begin[d + 1] - t->data_begin[d]);
         }
+        h.grid_begin = grid0;  // the 2-D types sort last
       }
     });
   }
@@ -1435,15 +1496,12 @@ int gcm_enable_weights(gcm_context* c, int enable) {
 // Per-kernel attributes that must be set outside stream capture.
 static int prepare_launch(gcm_context* c) {
   if (c->lds_attr_set) return GCM_OK;
-  for (const void* f : {(const void*)fill_kernel<float, true, true, 3>,
-                        (const void*)fill_kernel<float, true, false, 3>,
-                        (const void*)fill_kernel<float, true, true, 2>,
-                        (const void*)fill_kernel<float, true, false, 2>,
-                        (const void*)fill_kernel<double, false, true, 3>,
-                        (const void*)fill_kernel<double, false, false, 3>,
-                        (const void*)fill_kernel<double, false, true, 2>,
-                        (const void*)fill_kernel<double, false, false, 2>})
+  for (int v = 0; v < 16; ++v) {
+    const bool fp32 = v & 1, inl = v & 2, grid = v & 8;
+    const int nb = (v & 4) ? 3 : 2;
+    const void* f = grid ? fill_fn<true>(fp32, inl, nb) : fill_fn<false>(fp32, inl, nb);
     GCM_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxDynamicLds));
+  }
   c->lds_attr_set = true;
   return GCM_OK;
 }
@@ -1503,16 +1561,8 @@ static int enqueue_fill(gcm_context* c, hipStream_t s, int K, bool copies,
     const int nb = c->launch.ring_buffers;
     // evaluators + the event wave + nb - 1 producer waves
     const int threads = c->launch.block_threads + nb * kWave;
-    const void* fn = nullptr;
-#define GCM_FILL_FN(RT, CMP, INL, NBV) (const void*)fill_kernel<RT, CMP, INL, NBV>
-    if (fp32)
-      fn = nb == 3 ? (a.inputs_lds ? GCM_FILL_FN(float, true, true, 3) : GCM_FILL_FN(float, true, false, 3))
-                   : (a.inputs_lds ? GCM_FILL_FN(float, true, true, 2) : GCM_FILL_FN(float, true, false, 2));
-    else
-      fn = nb == 3 ? (a.inputs_lds ? GCM_FILL_FN(double, false, true, 3) : GCM_FILL_FN(double, false, false, 3))
-                   : (a.inputs_lds ? GCM_FILL_FN(double, false, true, 2) : GCM_FILL_FN(double, false, false, 2));
-#undef GCM_FILL_FN
-    (void)fn;
+    const void* fn = c->has_grid ? fill_fn<true>(fp32, a.inputs_lds, nb)
+                                 : fill_fn<false>(fp32, a.inputs_lds, nb);
     void* args[] = {&a};
     GCM_HIP(hipLaunchKernel(fn, dim3(grid), dim3(threads), args, lds, s));
     GCM_HIP(hipGetLastError());

@@ -92,6 +92,7 @@ struct Chunk {
   std::vector<int64_t> n_dials;
   std::vector<uint8_t> type;
   std::vector<int32_t> input;
+  std::vector<int32_t> input2;  // second input of 2-D dials, else -1
   std::vector<int64_t> data_len;
   std::vector<double> data;
 };
@@ -120,6 +121,7 @@ struct Synth {
   std::vector<int64_t> dial_begin;
   std::vector<uint8_t> type;
   std::vector<int32_t> input;
+  std::vector<int32_t> input2;
   std::vector<int64_t> data_begin;
   std::vector<double> data;
   std::vector<double> edges;         // [n_bins][3][2]
@@ -240,6 +242,31 @@ void emit_dial(Synth& S, Chunk& c, Xoshiro256ss& r, int kind, int par,
   }
   c.type.push_back(t);
   c.input.push_back(in);
+  c.input2.push_back(-1);
+  c.data_len.push_back((int64_t)c.data.size() - start);
+}
+
+// A 2-D dial (Bilinear/Bicubic) on the parameters behind inputs in (x) and
+// in2 (y): nx in [2, 18] (past the 16 knots of the reference's index search),
+// ny in [2, 6], sorted knots, smooth positive values.
+void emit_dial2d(Chunk& c, Xoshiro256ss& r, int in, int in2) {
+  const uint8_t t = r.uniform() < 0.5 ? GCM_DIAL_BILINEAR : GCM_DIAL_BICUBIC;
+  const int nx = 2 + r.integer(17), ny = 2 + r.integer(5);
+  std::vector<double> xk, yk;
+  random_knots(r, nx, xk);
+  random_knots(r, ny, yk);
+  for (double& v : yk) v *= 0.8;
+  const int64_t start = (int64_t)c.data.size();
+  c.data.push_back(nx);
+  c.data.push_back(ny);
+  for (double v : xk) c.data.push_back(v);
+  for (double v : yk) c.data.push_back(v);
+  for (int i = 0; i < nx; ++i)
+    for (int j = 0; j < ny; ++j)
+      c.data.push_back(std::max(1.0 + 0.1 * r.normal() * (xk[i] + 0.5 * yk[j]), 0.05));
+  c.type.push_back(t);
+  c.input.push_back(in);
+  c.input2.push_back(in2);
   c.data_len.push_back((int64_t)c.data.size() - start);
 }
 
@@ -297,8 +324,17 @@ void make_chunk(Synth& S, int64_t e0, int64_t e1, Chunk& c) {
         // weight; the CPU path multiplies them in dial-list order)
         c.type.push_back(GCM_DIAL_SHIFT);
         c.input.push_back(S.par_input[S.pool[P_NORM][0]]);
+        c.input2.push_back(-1);
         c.data.push_back(0.5 + r.uniform());
         c.data_len.push_back(1);
+      }
+      if (sp.features && r.uniform() < 0.3) {
+        // a 2-D dial on two spline parameters (Bilinear / Bicubic)
+        const auto& pool = S.pool[P_MONO];
+        const int a = pool[r.integer((int)pool.size())];
+        int b = pool[r.integer((int)pool.size())];
+        if (b == a) b = pool[(std::find(pool.begin(), pool.end(), a) - pool.begin() + 1) % pool.size()];
+        emit_dial2d(c, r, S.par_input[a], S.par_input[b]);
       }
     }
     c.n_dials.push_back((int64_t)c.type.size() - before);
@@ -465,6 +501,7 @@ void* gcm_synth_create(int workload, int64_t n_events, uint64_t seed,
     S->dial_begin.reserve(N + 1);
     S->type.reserve(nd);
     S->input.reserve(nd);
+    S->input2.reserve(nd);
     S->data_begin.reserve(nd + 1);
     S->data.reserve(ndata);
     S->dial_begin.push_back(0);
@@ -475,6 +512,7 @@ void* gcm_synth_create(int workload, int64_t n_events, uint64_t seed,
       for (auto n : c.n_dials) S->dial_begin.push_back(S->dial_begin.back() + n);
       S->type.insert(S->type.end(), c.type.begin(), c.type.end());
       S->input.insert(S->input.end(), c.input.begin(), c.input.end());
+      S->input2.insert(S->input2.end(), c.input2.begin(), c.input2.end());
       for (auto n : c.data_len) S->data_begin.push_back(S->data_begin.back() + n);
       S->data.insert(S->data.end(), c.data.begin(), c.data.end());
       c = Chunk();
@@ -516,6 +554,7 @@ int gcm_synth_view_get(void* h, gcm_synth_view* v) {
   v->prior_mean = S->prior_mean.data();
   v->prior_sigma = S->prior_sigma.data();
   v->par_type = S->par_type.data();
+  v->dial_input2 = S->input2.data();
   return 0;
 }
 

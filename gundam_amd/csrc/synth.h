@@ -38,6 +38,7 @@ typedef struct gcm_synth_view {
   const double* prior_mean;
   const double* prior_sigma;
   const int32_t* par_type;    /* generator's parameter kind               */
+  const int32_t* dial_input2; /* [n_dials] second input of 2-D dials, -1   */
 } gcm_synth_view;
 
 const char* gcm_synth_last_error(void);

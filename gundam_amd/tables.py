@@ -10,6 +10,7 @@ from __future__ import annotations
 
 import ctypes as C
 from dataclasses import dataclass, field
+from typing import Optional
 
 import numpy as np
 
@@ -40,6 +41,7 @@ class Tables:
     n_parameters: int
     n_bins: int
     reweight_cap: float = float("nan")
+    dial_input2: Optional[np.ndarray] = None  # [n_dials] y input of 2-D dials
     _keep: list = field(default_factory=list, repr=False)
 
     @property
@@ -68,6 +70,9 @@ class Tables:
                 raise TypeError(f"{name} must be contiguous {np.dtype(dt)}")
         if self.inputs.dtype != INPUT_DTYPE:
             raise TypeError("inputs must use INPUT_DTYPE")
+        if self.dial_input2 is not None and (self.dial_input2.dtype != np.int32
+                                             or not self.dial_input2.flags.c_contiguous):
+            raise TypeError("dial_input2 must be contiguous int32")
 
     def as_c(self) -> N.GcmTables:
         """The gcm_tables struct; valid while self is alive."""
@@ -89,6 +94,7 @@ class Tables:
         t.n_parameters = int(self.n_parameters)
         t.n_bins = int(self.n_bins)
         t.reweight_cap = float(self.reweight_cap)
+        t.dial_input2 = _ptr(self.dial_input2)
         return t
 
     # -- reference record sizes, for the algorithmic byte count ------------
@@ -97,13 +103,17 @@ class Tables:
         sum over dials of (w*L_d + 2) + 16 per binned event + 8 per table
         entry + 8 per parameter + 16 per bin, with L_d the reference record
         payload (monotonic/compact n+2, uniform 2n+2, general 3n+2, graph 2n,
-        norm 0, shift 1 word, tabulated 12 B (8 B in FP32), kriged 8k+4 B)."""
+        norm 0, shift 1 word, tabulated 12 B (8 B in FP32), kriged 8k+4 B,
+        bilinear/bicubic 2+nx+ny+nx*ny words plus a second 2-byte parameter
+        index)."""
         t = self.dial_type
         ln = np.diff(self.data_begin)
         b = np.zeros(t.shape[0], dtype=np.int64)
         spl = np.isin(t, [N.DIAL_MONOTONIC, N.DIAL_COMPACT, N.DIAL_UNIFORM,
                           N.DIAL_GENERAL, N.DIAL_GRAPH, N.DIAL_SHIFT])
         b[spl] = ln[spl] * word_bytes
+        grid = np.isin(t, [N.DIAL_BILINEAR, N.DIAL_BICUBIC])
+        b[grid] = ln[grid] * word_bytes + 2
         tab = t == N.DIAL_TABULATED
         b[tab] = 12 if word_bytes == 8 else 8
         kr = t == N.DIAL_KRIGED
@@ -158,7 +168,8 @@ class Workload:
             inputs=inputs,
             table_begin=arr(v.table_begin, C.c_int64, v.n_tables + 1),
             n_parameters=v.n_parameters, n_bins=v.n_bins,
-            reweight_cap=v.reweight_cap)
+            reweight_cap=v.reweight_cap,
+            dial_input2=arr(v.dial_input2, C.c_int32, nd))
         self.tables._keep.append(self)
 
     @property

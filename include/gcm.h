@@ -28,6 +28,12 @@
  *   KRIGED     : table, (index, weight)[0..k-1]  Kriged.cpp:3-21 (weights are
  *                rounded to float as the reference does)
  *   SHIFT      : value                        Shift.h:21
+ *   BILINEAR,  : nx, ny, x[0..nx-1], y[0..ny-1], v[0..nx*ny-1] (x-major)
+ *   BICUBIC      Bilinear.cpp:5-32, Bicubic.cpp:5-32; nx, ny >= 2.  The
+ *                dial reads dial_input (x) and dial_input2 (y); the response
+ *                supervisor is dial_input's; both coordinates are clamped to
+ *                the knot range when dial_input has GCM_INPUT_NO_EXTRAPOLATION
+ *                (the reference's default for these dials).
  * "table" is an index into gcm_tables.table_begin; "index" is relative to the
  * start of that table.
  */
@@ -61,7 +67,9 @@ extern "C" {
 #define GCM_DIAL_TABULATED 6
 #define GCM_DIAL_KRIGED 7
 #define GCM_DIAL_SHIFT 8
-#define GCM_DIAL_NTYPES 9
+#define GCM_DIAL_BILINEAR 9  /* 2-D: Bilinear.cpp, CalculateBilinearInterpolation.h */
+#define GCM_DIAL_BICUBIC 10  /* 2-D: Bicubic.cpp, CalculateBicubicSpline.h        */
+#define GCM_DIAL_NTYPES 11
 
 /* ---- input flags --------------------------------------------------------- */
 #define GCM_INPUT_NO_EXTRAPOLATION 1 /* DialBase::setAllowExtrapolation(false) */
@@ -119,6 +127,10 @@ typedef struct gcm_tables {
   int32_t n_bins;
   double reweight_cap;        /* GlobalEventReweightCap (EventDialCache.h:26-35);
                                  NaN or +inf: disabled                     */
+  const int32_t* dial_input2; /* [n_dials] second input of 2-D dials (the
+                                 y axis; DialInputBuffer slot 1), ignored
+                                 for 1-D dials; may be NULL without 2-D
+                                 dials                                      */
 } gcm_tables;
 
 /* Likelihood configuration: the options of the reference's joint

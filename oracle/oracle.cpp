@@ -177,6 +177,30 @@ struct KrigedDial : DialBase {  // Kriged.cpp:14-21
   }
 };
 
+// Bilinear.cpp:40-64 and Bicubic.cpp:40-64: two inputs (DialInputBuffer
+// slots 0 and 1), clamped to the knot range unless extrapolation is allowed.
+struct Dial2D : DialBase {
+  std::vector<double> data;  // nx, ny, x knots, y knots, values
+  const DialInputBuffer* second{nullptr};
+  bool allow_extrapolation{false};
+  bool bicubic{false};
+  double evalResponse(const DialInputBuffer& in) const override {
+    double input0 = in.buffer;
+    double input1 = second->buffer;
+    const int nx = int(data[0]), ny = int(data[1]);
+    const double* xx = data.data() + 2;
+    const double* yy = xx + nx;
+    if (!allow_extrapolation) {
+      if (input0 < xx[0]) input0 = xx[0];
+      if (input0 > xx[nx - 1]) input0 = xx[nx - 1];
+      if (input1 < yy[0]) input1 = yy[0];
+      if (input1 > yy[ny - 1]) input1 = yy[ny - 1];
+    }
+    return bicubic ? calc_bicubic(input0, input1, -1E20, 1E20, data.data())
+                   : calc_bilinear(input0, input1, -1E20, 1E20, data.data());
+  }
+};
+
 // DialInterface.h:57.
 struct DialInterface {
   std::unique_ptr<DialBase> dial;
@@ -227,6 +251,7 @@ struct Model {
   std::vector<DialInterface> interfaces;
   std::vector<CacheEntry> cache;
   std::vector<std::vector<Event*>> bin_events;  // Histogram BinContext lists
+  bool always{true};  // 2-D dials re-evaluate on every propagation
   std::vector<BinContent> hist;
 
   void reweightEntry(CacheEntry& entry) {
@@ -377,6 +402,26 @@ void* oracle_create(const gcm_tables* t, int n_threads) {
           dial = std::move(kr);
           break;
         }
+        case GCM_DIAL_BILINEAR:
+        case GCM_DIAL_BICUBIC: {
+          const int in2 = t->dial_input2 ? t->dial_input2[d] : -1;
+          if (in2 < 0 || in2 >= t->n_inputs) {
+            g_error = "2-D dial second input out of range";
+            return nullptr;
+          }
+          const int nx = len >= 2 ? int(p[0]) : 0, ny = len >= 2 ? int(p[1]) : 0;
+          if (nx < 2 || ny < 2 || len != 2 + nx + ny + (int64_t)nx * ny) {
+            g_error = "2-D dial needs nx, ny >= 2 and 2 + nx + ny + nx*ny values";
+            return nullptr;
+          }
+          auto g2 = std::make_unique<Dial2D>();
+          g2->data.assign(p, p + len);
+          g2->second = &m->inputs[in2];
+          g2->allow_extrapolation = extrap;
+          g2->bicubic = t->dial_type[d] == GCM_DIAL_BICUBIC;
+          dial = std::move(g2);
+          break;
+        }
         default:
           g_error = "unsupported dial type";
           return nullptr;
@@ -396,8 +441,10 @@ void* oracle_create(const gcm_tables* t, int n_threads) {
       ce.event = &ev;
       for (int64_t d = t->dial_begin[e]; d < t->dial_begin[e + 1]; ++d) {
         DialInterface* di = &m->interfaces[d];
+        const bool two_d = t->dial_type[d] == GCM_DIAL_BILINEAR ||
+                           t->dial_type[d] == GCM_DIAL_BICUBIC;
         ce.dials.push_back(DialResponseCache{di, std::nan("unset"),
-                                             &di->input->update_requested});
+                                             two_d ? &m->always : &di->input->update_requested});
       }
       if (ev.bin >= 0) {
         if (ev.bin >= t->n_bins) {
@@ -643,6 +690,15 @@ int oracle_bin_events(int64_t n_events, int n_vars, const double* values,
 }
 
 // Direct access to the restated Calculate* functions for the golden checks.
+// One 2-D dial evaluation (the golden-vector check): data is the payload
+// layout [nx, ny, x knots, y knots, values].
+double oracle_calc2d(int type, double x, double y, double lo, double hi,
+                     const double* data) {
+  if (type == GCM_DIAL_BILINEAR) return calc_bilinear(x, y, lo, hi, data);
+  if (type == GCM_DIAL_BICUBIC) return calc_bicubic(x, y, lo, hi, data);
+  return std::nan("");
+}
+
 double oracle_calc(int type, double x, double lo, double hi, const double* data,
                    int dim) {
   switch (type) {

@@ -150,4 +150,95 @@ inline double calc_graph(double x, double lo, double hi, const double* data,
   return clamp_value(p1 + fx * m, lo, hi);
 }
 
+// 2-D dials.  data = the reference's getDialData() layout
+// [nx, ny, x knots (nx), y knots (ny), values (nx x ny, x-major)]
+// (Bilinear.cpp:5-32, Bicubic.cpp:5-32).
+
+// BilinearIndex / BicubicIndex (CalculateBilinearInterpolation.h:36-45,
+// CalculateBicubicSpline.h): a four-step search over offsets 8, 4, 2, 1 (it
+// never returns more than 15).  The reference multiplies each comparison by
+// (i + off < na), reading a[i + off] even past the array; the comparison is
+// skipped here when it cannot count, which gives the same index.
+inline int grid_index(double v, const double* a, int na) {
+  int i = (8 < na && !(v < a[8])) ? 8 : 0;
+  i += (i + 4 < na && !(v < a[i + 4])) ? 4 : 0;
+  i += (i + 2 < na && !(v < a[i + 2])) ? 2 : 0;
+  i += (i + 1 < na && !(v < a[i + 1])) ? 1 : 0;
+  return i;
+}
+
+// CalculateBilinearInterpolation.h:47-152.
+inline double bilinear_cint(double x, double x0, double p0, double x1, double p1) {
+  return p0 + (p1 - p0) * (x - x0) / (x1 - x0);
+}
+
+inline double calc_bilinear(double x, double y, double lo, double hi, const double* data) {
+  const int nx = int(data[0]), ny = int(data[1]);
+  const double* xx = data + 2;
+  const double* yy = xx + nx;
+  const double* k = yy + ny;
+  const int ix = grid_index(x, xx, nx - 1);
+  const int iy = grid_index(y, yy, ny - 1);
+  const double v0 = bilinear_cint(x, xx[ix], k[ix * ny + iy], xx[ix + 1], k[(ix + 1) * ny + iy]);
+  const double v1 =
+      bilinear_cint(x, xx[ix], k[ix * ny + iy + 1], xx[ix + 1], k[(ix + 1) * ny + iy + 1]);
+  double val = bilinear_cint(y, yy[iy], v0, yy[iy + 1], v1);
+  if (val < lo) val = lo;
+  if (val > hi) val = hi;
+  return val;
+}
+
+// CalculateBicubicSpline.h: the Catmull-Rom Hermite through four points in
+// the Horner form the reference compiles (BICUBIC_USE_HORNER_FACTORIZATION).
+inline double bicubic_cint(double x, double x0, double p0, double x1, double p1, double x2,
+                           double p2, double x3, double p3) {
+  const double step = x2 - x1;
+  const double fx = (x - x1) / step;
+  double m1 = step * (p2 - p0) / (x2 - x0);
+  double m2 = step * (p3 - p1) / (x3 - x1);
+  if (fx < 0.0) m2 = m1;
+  if (fx > 1.0) m1 = m2;
+  return ((((-2.0 * p2 + 2.0 * p1 + m2 + m1) * fx + 3 * p2 - 3 * p1 - m2 - 2 * m1) * fx + m1) *
+              fx +
+          p1);
+}
+
+// CalculateBicubicSpline.h: a 4 x 4 neighbourhood of (ix, iy); missing rows
+// are extrapolated linearly in y from the two nearest, missing columns
+// linearly in x from the two nearest (of the y-extrapolated values at the
+// corners), then four x interpolations and one y interpolation.
+inline double calc_bicubic(double x, double y, double lo, double hi, const double* data) {
+  const int nx = int(data[0]), ny = int(data[1]);
+  const double* xx = data + 2;
+  const double* yy = xx + nx;
+  const double* k = yy + ny;
+  const int ix = grid_index(x, xx, nx - 1);
+  const int iy = grid_index(y, yy, ny - 1);
+  auto row_val = [&](int c, int r) -> double {  // column c in [0, nx)
+    if (r < 0) return 2 * k[c * ny + iy] - k[c * ny + iy + 1];
+    if (r > ny - 1) return 2 * k[c * ny + iy + 1] - k[c * ny + iy];
+    return k[c * ny + r];
+  };
+  auto val = [&](int c, int r) -> double {
+    if (c < 0) return 2 * row_val(ix, r) - row_val(ix + 1, r);
+    if (c > nx - 1) return 2 * row_val(ix + 1, r) - row_val(ix, r);
+    return row_val(c, r);
+  };
+  const double a0 = (ix > 0) ? xx[ix - 1] : 2 * xx[ix] - xx[ix + 1];
+  const double a1 = xx[ix], a2 = xx[ix + 1];
+  const double a3 = (ix < nx - 2) ? xx[ix + 2] : 2 * xx[ix + 1] - xx[ix];
+  double v[4];
+  for (int j = 0; j < 4; ++j) {
+    const int r = iy - 1 + j;
+    v[j] = bicubic_cint(x, a0, val(ix - 1, r), a1, val(ix, r), a2, val(ix + 1, r), a3,
+                        val(ix + 2, r));
+  }
+  const double u0 = (iy > 0) ? yy[iy - 1] : 2 * yy[iy] - yy[iy + 1];
+  const double u3 = (iy < ny - 2) ? yy[iy + 2] : 2 * yy[iy + 1] - yy[iy];
+  double out = bicubic_cint(y, u0, v[0], yy[iy], v[1], yy[iy + 1], v[2], u3, v[3]);
+  if (out < lo) out = lo;
+  if (out > hi) out = hi;
+  return out;
+}
+
 }  // namespace oracle

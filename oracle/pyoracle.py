@@ -62,6 +62,8 @@ def lib() -> C.CDLL:
         l.oracle_bin_events.argtypes = [_I64, C.c_int, _P, C.c_int, _P, _P]
         l.oracle_calc.restype = _D
         l.oracle_calc.argtypes = [C.c_int, _D, _D, _D, _P, C.c_int]
+        l.oracle_calc2d.restype = _D
+        l.oracle_calc2d.argtypes = [C.c_int, _D, _D, _D, _D, _P]
         _lib = l
     return _lib
 
@@ -184,5 +186,8 @@ def ref_lib():
         l = C.CDLL(REF_LIB)
         l.ref_calc.restype = _D
         l.ref_calc.argtypes = [C.c_int, _D, _D, _D, _P, C.c_int]
+        if hasattr(l, "ref_calc2d"):
+            l.ref_calc2d.restype = _D
+            l.ref_calc2d.argtypes = [C.c_int, _D, _D, _D, _D, _P]
         _ref = l
     return _ref

@@ -8,6 +8,8 @@
 #include "CalculateGeneralSpline.h"
 #include "CalculateCompactSpline.h"
 #include "CalculateGraph.h"
+#include "CalculateBilinearInterpolation.h"
+#include "CalculateBicubicSpline.h"
 
 #include "../include/gcm.h"
 
@@ -21,4 +23,19 @@ extern "C" double ref_calc(int type, double x, double lo, double hi,
     case GCM_DIAL_GRAPH: return CalculateGraph(x, lo, hi, data, dim);
     default: return __builtin_nan("");
   }
+}
+
+// 2-D dials: data is the getDialData() layout [nx, ny, x knots, y knots,
+// values], unpacked as Bilinear::evalResponse / Bicubic::evalResponse do.
+extern "C" double ref_calc2d(int type, double x, double y, double lo, double hi,
+                             const double* data) {
+  const int nx = int(data[0]), ny = int(data[1]);
+  const double* xx = data + 2;
+  const double* yy = xx + nx;
+  const double* knots = yy + ny;
+  if (type == GCM_DIAL_BILINEAR)
+    return CalculateBilinearInterpolation(x, y, lo, hi, knots, nx, ny, xx, nx, yy, ny);
+  if (type == GCM_DIAL_BICUBIC)
+    return CalculateBicubicSpline(x, y, lo, hi, knots, nx, ny, xx, nx, yy, ny);
+  return __builtin_nan("");
 }

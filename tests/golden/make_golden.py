@@ -4,6 +4,9 @@ per-dial math, computed by the reference's own header-only Calculate*.h
 (src/Utils/include) compiled in this container into oracle/_ref
 (`make -C oracle ref`).  The fixture holds only inputs and outputs.
 
+Also tests/golden/calc2d_golden.npz: the 2-D Bilinear/Bicubic math of
+CalculateBilinearInterpolation.h / CalculateBicubicSpline.h on seeded grids.
+
 Cases: the knot arrays and x sweeps of the reference's unit tests
 (tests/GTests/unitTest_Calculate{Monotonic,Compact,Graph}*.cpp), plus seeded
 random records of every 1-D type with x inside the knots, exactly on knots,
@@ -95,6 +98,61 @@ def random_records(rng):
     return out
 
 
+BILIN, BICUB = 9, 10
+
+
+def grid_records(rng):
+    """2-D grids (Bilinear/Bicubic payloads): nx, ny from 2 up to 20 (past
+    the 16 knots the reference's four-step index search reaches), sorted
+    knots with uneven spacing, smooth and oscillating values; points inside,
+    on knots, just outside and far outside, with wide and tight clamps."""
+    out = []
+    shapes = [(2, 2), (3, 2), (2, 5), (5, 4), (9, 3), (16, 4), (17, 3), (20, 7), (4, 17)]
+    for rep in range(6):
+        for t in (BILIN, BICUB):
+            for nx, ny in shapes:
+                xk = np.sort(np.concatenate([[-3.0, 3.0], rng.uniform(-3, 3, nx - 2)]))
+                yk = np.sort(np.concatenate([[-2.0, 2.0], rng.uniform(-2, 2, ny - 2)]))
+                if rep % 3 == 0:
+                    v = rng.standard_normal(nx * ny)
+                else:
+                    v = 1.0 + 0.2 * np.add.outer(xk, 0.5 * yk).ravel() \
+                        + 0.05 * rng.standard_normal(nx * ny)
+                d = np.concatenate([[nx, ny], xk, yk, v])
+                kx = rng.integers(0, nx, 6)
+                ky = rng.integers(0, ny, 6)
+                px = np.concatenate([rng.uniform(-3, 3, 20), xk[kx], [-3.5, 3.5, -9.0, 9.0,
+                                     xk[0] - 1e-9, xk[-1] + 1e-9]])
+                py = np.concatenate([rng.uniform(-2, 2, 20), yk[ky], [-2.5, 2.5, 6.0, -6.0,
+                                     yk[0] - 1e-9, yk[-1] + 1e-9]])
+                lo, hi = (-1e20, 1e20) if rep % 2 == 0 else (0.9, 1.1)
+                out.append((t, d, px, py, lo, hi))
+    return out
+
+
+def write_grid_golden(ref, rng):
+    recs = grid_records(rng)
+    types, lo_all, hi_all, data_all, doff, xs, ys, poff, exp = [], [], [], [], [0], [], [], [0], []
+    for t, d, px, py, lo, hi in recs:
+        padded = np.concatenate([d, np.zeros(16)])  # the reference's index reads past small grids
+        types.append(t)
+        lo_all.append(lo)
+        hi_all.append(hi)
+        data_all.append(d)
+        doff.append(doff[-1] + d.size)
+        for x, y in zip(px, py):
+            xs.append(float(x))
+            ys.append(float(y))
+            exp.append(ref.ref_calc2d(t, float(x), float(y), lo, hi, padded.ctypes.data))
+        poff.append(len(xs))
+    out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "calc2d_golden.npz")
+    np.savez_compressed(out, type=np.array(types, np.int32), lo=np.array(lo_all),
+                        hi=np.array(hi_all), data=np.concatenate(data_all),
+                        data_off=np.array(doff, np.int64), x=np.array(xs), y=np.array(ys),
+                        p_off=np.array(poff, np.int64), expected=np.array(exp))
+    print(f"wrote {out}: {len(recs)} grids, {len(xs)} evaluations")
+
+
 def main():
     ref = pyoracle.ref_lib()
     if ref is None:
@@ -124,6 +182,7 @@ def main():
         x=np.array(xs_all), x_off=np.array(xoff, np.int64),
         expected=np.array(exp))
     print(f"wrote {out}: {len(recs)} records, {len(xs_all)} evaluations")
+    write_grid_golden(ref, np.random.default_rng(20261204))
 
 
 if __name__ == "__main__":

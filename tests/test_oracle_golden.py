@@ -103,3 +103,68 @@ def test_uniform_and_general_agree_on_uniform_knots():
         a = pyoracle.calc(UNIF, x, -1e20, 1e20, du, du.size)
         b = pyoracle.calc(GEN, x, -1e20, 1e20, dg, dg.size)
         assert abs(a - b) < 1e-12
+
+
+GOLDEN2D = os.path.join(os.path.dirname(__file__), "golden", "calc2d_golden.npz")
+BILIN, BICUB = 9, 10
+
+
+def _grids():
+    g = np.load(GOLDEN2D)
+    for i in range(len(g["type"])):
+        d = np.ascontiguousarray(g["data"][g["data_off"][i]:g["data_off"][i + 1]])
+        sl = slice(g["p_off"][i], g["p_off"][i + 1])
+        yield (int(g["type"][i]), d, g["lo"][i], g["hi"][i], g["x"][sl], g["y"][sl],
+               g["expected"][sl])
+
+
+def _calc2d(t, d, lo, hi, x, y):
+    return pyoracle.lib().oracle_calc2d(t, float(x), float(y), lo, hi, d.ctypes.data)
+
+
+def test_calc2d_bit_exact_against_reference_headers():
+    """Bilinear and bicubic grids (2x2 up to 20x7, points inside, on knots and
+    outside, with clamps) against CalculateBilinearInterpolation.h and
+    CalculateBicubicSpline.h compiled in place."""
+    n, types = 0, set()
+    for t, d, lo, hi, xs, ys, exp in _grids():
+        got = np.array([_calc2d(t, d, lo, hi, x, y) for x, y in zip(xs, ys)])
+        same = (got == exp) | (np.isnan(got) & np.isnan(exp))
+        assert same.all(), (t, d[:2], xs[~same][:3], ys[~same][:3], got[~same][:3], exp[~same][:3])
+        n += xs.size
+        types.add(t)
+    assert types == {BILIN, BICUB} and n > 3000
+
+
+def test_calc2d_planted_mistakes_are_caught():
+    """A bicubic record read with the bilinear formula, or with its value
+    array transposed (y-major instead of x-major), differs from the fixture:
+    the vectors pin both."""
+    swapped = wrong = 0
+    for t, d, lo, hi, xs, ys, exp in _grids():
+        if t != BICUB or lo > -1:
+            continue
+        alt = np.array([_calc2d(BILIN, d, lo, hi, x, y) for x, y in zip(xs, ys)])
+        wrong += int((alt != exp).sum())
+        nx, ny = int(d[0]), int(d[1])
+        if nx == ny:
+            continue
+        v = d[2 + nx + ny:].reshape(nx, ny)
+        dt = np.concatenate([d[:2 + nx + ny], v.T.ravel()])
+        alt2 = np.array([_calc2d(t, dt, lo, hi, x, y) for x, y in zip(xs, ys)])
+        swapped += int((alt2 != exp).sum())
+    assert wrong > 100
+    assert swapped > 100
+
+
+def test_bilinear_reproduces_planes():
+    """A plane v = a + b x + c y is reproduced exactly (to rounding) inside
+    the grid by both methods (closed form)."""
+    rng = np.random.default_rng(5)
+    xk = np.sort(rng.uniform(-3, 3, 6))
+    yk = np.sort(rng.uniform(-2, 2, 5))
+    v = 0.7 + 0.3 * xk[:, None] - 0.2 * yk[None, :]
+    d = np.concatenate([[6, 5], xk, yk, v.ravel()])
+    for t in (BILIN, BICUB):
+        for x, y in rng.uniform([xk[0], yk[0]], [xk[-1], yk[-1]], (50, 2)):
+            assert abs(_calc2d(t, d, -1e20, 1e20, x, y) - (0.7 + 0.3 * x - 0.2 * y)) < 1e-12

@@ -66,6 +66,17 @@ def _python_weights(wl, theta, tab):
                 r = 0.0
                 for k in range(1, p.size, 2):
                     r += float(np.float32(p[k + 1])) * tab[base + int(p[k])]
+            elif typ in (N.DIAL_BILINEAR, N.DIAL_BICUBIC):
+                # Bilinear/Bicubic::evalResponse: both coordinates clamped to
+                # the knot range when extrapolation is off
+                y = xin[t.dial_input2[d]]
+                nx, ny = int(p[0]), int(p[1])
+                xk, yk = p[2:2 + nx], p[2 + nx:2 + nx + ny]
+                if noext:
+                    x = min(max(x, xk[0]), xk[-1])
+                    y = min(max(y, yk[0]), yk[-1])
+                pc = np.ascontiguousarray(p)
+                r = pyoracle.lib().oracle_calc2d(typ, x, y, -1e20, 1e20, pc.ctypes.data)
             elif typ == N.DIAL_GRAPH:
                 if noext and x <= p[1]:
                     r = p[0]
