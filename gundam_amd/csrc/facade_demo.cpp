@@ -39,6 +39,7 @@ int main() {
   t.n_parameters = v.n_parameters;
   t.n_bins = v.n_bins;
   t.reweight_cap = v.reweight_cap;
+  t.dial_input2 = v.dial_input2;
 
   gcm::CacheManager cm(0);
   cm.Build(t, 16);  // up to 16 points per launch (the gradient below)
