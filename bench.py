@@ -343,6 +343,80 @@ def precision_c4(args, log):
     }
 
 
+def posterior_agreement(args, log):
+    """Posterior means of the delayed-acceptance chains (surrogate sub-chains
+    of args.c5_sub steps) against batched exact chains on the 100k-event
+    model (20 parameters), each run for args.post_seconds from the same
+    burn-in states: per-parameter difference of the means in units of the
+    exact chains' posterior sigma, with its Monte Carlo standard error from
+    batch means (20 batches per chain set)."""
+    from gundam_amd import CacheManager, Workload
+    from gundam_amd.mcmc import (ExactTarget, SurrogateTarget, adapt_cov, laplace_scales,
+                                 run_batched_exact, run_delayed, var_ratio)
+    from gundam_amd.surrogate import Surrogate
+
+    wl = Workload(1, seed=args.seed + 1)
+    t = wl.tables
+    wl.bin_events_on_device(0)
+    cm = CacheManager(0)
+    cm.build(t, max_points=64)
+    mode = wl.nominal()
+    cm.propagate_parameters(mode)
+    data, w2 = (a.copy() for a in cm.hist(0))
+    cm.set_likelihood("barlow", data, wl.prior_mean, wl.prior_sigma)
+    ex = ExactTarget(cm)
+    sc = laplace_scales(ex, mode)
+    d = mode.size
+    rng = np.random.default_rng(args.seed + 77)
+    C_ = 32
+    ex.record = True
+    burn = run_batched_exact(ex, mode + sc * rng.standard_normal((C_, d)),
+                             (2.38 ** 2 / d) * np.diag(sc ** 2), 400, rng)
+    ex(mode + sc * rng.standard_normal((max(0, 16000 - ex.points), d)))
+    ex.record = False
+    X, Y = ex.recorded()
+    cov = adapt_cov(burn.samples[200:], sc)
+    sur = Surrogate(d, t.n_bins, hidden=512, seed=args.seed + 3)
+    sur.fit(X, Y, epochs=200, batch=256, lr=1e-3)
+    surr = SurrogateTarget(sur, data, var_ratio(data, w2), wl.prior_mean, wl.prior_sigma)
+    start = burn.samples[-1]
+    T = args.post_seconds
+    big = 10 ** 8
+    res = {"exact": run_batched_exact(ex, start.copy(), cov, big, rng, max_seconds=T),
+           "delayed": run_delayed(ex, surr, start.copy(), cov, big, rng, n_sub=args.c5_sub,
+                                  max_seconds=T)}
+
+    def batch_stats(smp, nb=20):
+        smp = smp[smp.shape[0] // 10:]  # drop 10% as further burn-in
+        n = smp.shape[0] // nb * nb
+        b = smp[:n].reshape(nb, -1, smp.shape[1], d).mean(axis=(1, 2))
+        return smp.reshape(-1, d).mean(0), b.std(0, ddof=1) / np.sqrt(nb)
+
+    m_ex, se_ex = batch_stats(res["exact"].samples)
+    m_da, se_da = batch_stats(res["delayed"].samples)
+    sigma = res["exact"].samples[res["exact"].samples.shape[0] // 10:].reshape(-1, d).std(0)
+    diff = (m_da - m_ex) / sigma
+    se = np.sqrt(se_ex ** 2 + se_da ** 2) / sigma
+    out = {
+        "workload": "100k events, 20 parameters, 400 bins, Barlow-Beeston OA2021 (Asimov)",
+        "chains": C_, "seconds_per_method": T, "n_sub": args.c5_sub,
+        "iterations": {k: int(r.samples.shape[0]) for k, r in res.items()},
+        "max_abs_mean_diff_sigma": float(np.max(np.abs(diff))),
+        "median_abs_mean_diff_sigma": float(np.median(np.abs(diff))),
+        "max_mc_standard_error_sigma": float(np.max(se)),
+        "params_within_0p05_sigma": int(np.sum(np.abs(diff) < 0.05)),
+        "max_abs_z": float(np.max(np.abs(diff / se))),
+        "std_ratio_delayed_over_exact": [
+            float(x) for x in np.round(res["delayed"].samples[res["delayed"].samples.shape[0] // 10:]
+                                       .reshape(-1, d).std(0) / sigma, 4)],
+    }
+    log(f"posterior: max |dmean| {out['max_abs_mean_diff_sigma']:.3f} sigma "
+        f"(MC s.e. <= {out['max_mc_standard_error_sigma']:.3f}), "
+        f"{out['params_within_0p05_sigma']}/{d} within 0.05 sigma, max |z| {out['max_abs_z']:.2f}")
+    cm.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -376,6 +450,9 @@ def main():
     ap.add_argument("--c4-points", type=int, default=1000)
     ap.add_argument("--only-c4", action="store_true",
                     help="run only the precision study and print its JSON")
+    ap.add_argument("--only-posterior", action="store_true",
+                    help="run only the posterior-agreement study (100k-event model)")
+    ap.add_argument("--post-seconds", type=float, default=60.0)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -388,6 +465,9 @@ def main():
 
     if args.only_c4:
         print(json.dumps({"precision_c4": precision_c4(args, log)}), flush=True)
+        return
+    if args.only_posterior:
+        print(json.dumps({"posterior_agreement": posterior_agreement(args, log)}), flush=True)
         return
 
     t0 = time.time()
