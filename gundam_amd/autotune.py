@@ -20,13 +20,17 @@ LDS_BYTES_PER_CU = 160 * 1024
 
 
 def lds_per_block(tile_words: int, n_inputs: int, k_points: int = 1,
-                  ring_buffers: int = 2) -> int:
-    """LDS of one fill block (mirrors the host computation in gcm.hip)."""
+                  ring_buffers: int = 2, blocks_per_cu: int = 1) -> int:
+    """LDS of one fill block (mirrors the host computation in gcm.hip: the
+    input records are staged only when they are small and still let
+    blocks_per_cu blocks share the CU)."""
     max_dials = min(1024, max(256, (tile_words // 8 + 127) // 128 * 128))
+    ring = ring_buffers * (tile_words * 8 + max_dials * 8 + 64 * 16 + 32) + 2 * max_dials * 8
     in_bytes = k_points * n_inputs * 32
-    in_pad = (in_bytes + 1023) // 1024 * 1024 if in_bytes <= 16384 else 0
-    return (in_pad + ring_buffers * (tile_words * 8 + max_dials * 8 + 64 * 16 + 32)
-            + 2 * max_dials * 8)
+    staged = (in_bytes + 1023) // 1024 * 1024
+    if in_bytes <= 16384 and (staged + ring) * blocks_per_cu <= LDS_BYTES_PER_CU:
+        return staged + ring
+    return ring
 
 
 def head(t: Tables, n_events: int) -> Tables:
@@ -39,15 +43,17 @@ def head(t: Tables, n_events: int) -> Tables:
                   dial_data=t.dial_data, inputs=t.inputs,
                   table_begin=t.table_begin, n_parameters=t.n_parameters,
                   n_bins=t.n_bins, reweight_cap=t.reweight_cap,
+                  dial_input2=None if t.dial_input2 is None else t.dial_input2[:nd],
                   _keep=[t])
 
 
 def candidates(n_inputs: int, precision: str = "fp64", k_points: int = 1) -> list:
     out = []
-    for nb, tile, block, bpc in itertools.product((2, 3), (1024, 1536, 2048, 2560, 3072, 4096),
-                                                  (128, 256, 384, 512, 768),
+    for nb, tile, block, bpc in itertools.product((2, 3), (1024, 1536, 2048, 2560, 3072, 3584,
+                                                         3840, 4096),
+                                                  (256, 384, 512, 768),
                                                   (1, 2, 3, 4, 5, 6)):
-        lds = lds_per_block(tile, n_inputs, k_points, ring_buffers=nb)
+        lds = lds_per_block(tile, n_inputs, k_points, ring_buffers=nb, blocks_per_cu=bpc)
         # fill the CU's LDS: only the largest block count that fits
         if lds * bpc > LDS_BYTES_PER_CU or lds * (bpc + 1) <= LDS_BYTES_PER_CU:
             continue

@@ -50,6 +50,7 @@ constexpr int kDefaultBlocksPerCU = 3;
 constexpr int kFinalizeBlock = 256;
 constexpr int kFinalizeMaxBlocks = 128;
 constexpr size_t kInputsLdsMax = 16384;  // stage input records in LDS up to this
+constexpr size_t kLdsPerCU = 160 * 1024;   // gfx950 LDS per compute unit
 constexpr size_t kHeaderBytes = 32;      // tile header copy in each LDS buffer
 constexpr int kMaxDynamicLds = 160 * 1024;  // gfx950 LDS per CU
 
@@ -1547,13 +1548,19 @@ static int enqueue_fill(gcm_context* c, hipStream_t s, int K, bool copies,
   a.flags = c->d_flags;
   a.rec_bytes = c->launch.tile_words * 8;
   a.max_dials = c->max_dials;
+  // The input records are staged in LDS when they are small and the staging
+  // still lets blocks_per_cu blocks share a CU; otherwise the evaluators read
+  // them from global memory (L1/L2) and the LDS goes to larger tiles.
   const size_t in_bytes = (size_t)K * c->n_inputs * sizeof(InputRec);
-  a.inputs_lds = in_bytes <= kInputsLdsMax;
-  const size_t in_pad = a.inputs_lds ? (in_bytes + 1023) / 1024 * 1024 : 0;
-  const size_t lds = in_pad +
-                     (size_t)c->launch.ring_buffers * ((size_t)a.rec_bytes + (size_t)a.max_dials * 8 +
-                          kMaxTileEvents * sizeof(EventRec) + kHeaderBytes) +
-                     2 * (size_t)a.max_dials * 8;
+  const size_t ring_lds =
+      (size_t)c->launch.ring_buffers * ((size_t)a.rec_bytes + (size_t)a.max_dials * 8 +
+                                        kMaxTileEvents * sizeof(EventRec) + kHeaderBytes) +
+      2 * (size_t)a.max_dials * 8;
+  const size_t staged_pad = (in_bytes + 1023) / 1024 * 1024;
+  a.inputs_lds = in_bytes <= kInputsLdsMax &&
+                 (staged_pad + ring_lds) * (size_t)c->launch.blocks_per_cu <= kLdsPerCU;
+  const size_t in_pad = a.inputs_lds ? staged_pad : 0;
+  const size_t lds = in_pad + ring_lds;
   if (tev) GCM_HIP(hipEventRecord(tev[0], s));
   if (c->n_tiles > 0) {
     const int64_t want = (int64_t)c->launch.blocks_per_cu * c->n_cu;
