@@ -133,7 +133,7 @@ def batched_workload3(args, log):
             "launch": cm.launch_config()}
 
 
-def surrogate_da(args, cm, wl, data, err, log):
+def surrogate_da(args, cm, wl, data, err, log, device_tables=False):
     """Delayed-acceptance MCMC with the learned surrogate on the 5M-event
     model, against the plain exact GPU chain.
 
@@ -151,7 +151,7 @@ def surrogate_da(args, cm, wl, data, err, log):
                                  run_batched_exact, run_delayed, run_plain, var_ratio)
     from gundam_amd.surrogate import Surrogate
 
-    tv = wl.table_values if wl.tables.table_len else None
+    tv = wl.table_values if wl.tables.table_len and not device_tables else None
     mode = wl.nominal()
     d = mode.size
     ex = ExactTarget(cm, tv)
@@ -445,6 +445,9 @@ def main():
     ap.add_argument("--c5-epochs", type=int, default=300)
     ap.add_argument("--c5-sub", type=int, default=16)
     ap.add_argument("--c5-seconds", type=float, default=8.0)
+    ap.add_argument("--host-tables", action="store_true",
+                    help="compute lookup tables with the host callback and upload them "
+                         "every step (default: device table refresh)")
     ap.add_argument("--c4", action="store_true",
                     help="add the FP32-vs-FP64 precision study on the 20M-event workload")
     ap.add_argument("--c4-points", type=int, default=1000)
@@ -508,13 +511,20 @@ def main():
     log(f"built in {time.time()-t0:.1f}s: {counts}, resident "
         f"{cm.resident_memory()/2**30:.2f} GiB, launch {cm.launch_config()}")
 
-    cm.propagate_parameters(nominal, wl.table_values(nominal))
+    # lookup tables: refreshed on the device at every fill from the mirrored
+    # parameters (gcm_add_table_update), or with --host-tables computed by the
+    # host callback and uploaded with the parameters, as the reference does
+    device_tables = bool(t.table_len) and not args.host_tables
+    if device_tables:
+        from gundam_amd import _native as N
+        cm.add_table_update(N.TABLE_UPDATE_COS, N.TABLE_UPDATE_COS_KERNEL, wl.table_jobs())
+    cm.propagate_parameters(nominal, None if device_tables else wl.table_values(nominal))
     data, data_err = (a.copy() for a in cm.hist(0))  # Asimov data at nominal
     cm.set_likelihood("barlow", data, wl.prior_mean, wl.prior_sigma)
 
     n_pts = args.warmup + args.steps
     pts = wl.random_points(n_pts, 0.5, seed=args.seed + 17)
-    tabs = wl.table_values(pts) if t.table_len else None
+    tabs = wl.table_values(pts) if t.table_len and not device_tables else None
 
     def one(i):
         cm.set_parameters(pts[i])
@@ -544,7 +554,7 @@ def main():
     batched = None
     if args.batch > 1:
         bp = wl.random_points(K, 0.5, seed=args.seed + 99)
-        btab = wl.table_values(bp) if t.table_len else None
+        btab = wl.table_values(bp) if t.table_len and not device_tables else None
         cm.set_parameters(bp)
         if btab is not None:
             cm.set_table_values(btab)
@@ -573,7 +583,7 @@ def main():
 
     c5 = None
     if not args.no_c5 and args.batch >= 32:
-        c5 = surrogate_da(args, cm, wl, data, data_err, log)
+        c5 = surrogate_da(args, cm, wl, data, data_err, log, device_tables)
 
     cpu = None
     if not args.no_cpu:
@@ -612,6 +622,8 @@ def main():
             "precision": args.precision,
             "launch": cm.launch_config(),
             "autotune": tuned,
+            "lookup_tables": ("device refresh (gcm_add_table_update)" if device_tables
+                              else "host callback + upload per step"),
         },
         "roofline": {
             "bound": "hbm",

@@ -11,6 +11,9 @@ import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libgcm.so")
+# device table function of the synthetic workloads (gcm_add_table_update)
+TABLE_UPDATE_COS = os.path.join(_HERE, "table_update_cos.hsaco")
+TABLE_UPDATE_COS_KERNEL = "gcm_synth_cos_table"
 SYNTH_PATH = os.path.join(_HERE, "libgcm_synth.so")
 
 # status codes
@@ -83,6 +86,16 @@ class GcmLlhConfig(C.Structure):
                 ("ignore_bins", C.c_void_p)]
 
 
+class GcmPenaltySet(C.Structure):
+    _fields_ = [("n", C.c_int32), ("parameters", C.c_void_p), ("prior", C.c_void_p),
+                ("inv_cov", C.c_void_p), ("lower", C.c_void_p), ("upper", C.c_void_p)]
+
+
+class GcmTableJob(C.Structure):
+    _fields_ = [("table", C.c_int32), ("n_inputs", C.c_int32), ("inputs", C.c_void_p),
+                ("user_id", C.c_int32)]
+
+
 class GcmLaunchConfig(C.Structure):
     _fields_ = [("block_threads", C.c_int32), ("tile_words", C.c_int32),
                 ("blocks_per_cu", C.c_int32), ("ring_buffers", C.c_int32)]
@@ -126,6 +139,10 @@ GCM_SIGNATURES = {
     "gcm_set_parameters": (C.c_int, [_P, _P, _I32, _I32]),
     "gcm_set_table_values": (C.c_int, [_P, _P, _I64, _I32]),
     "gcm_set_likelihood": (C.c_int, [_P, C.POINTER(GcmLlhConfig), _P, _P, _P]),
+    "gcm_set_penalty_sets": (C.c_int, [_P, _I32, C.POINTER(GcmPenaltySet)]),
+    "gcm_add_table_update": (C.c_int, [_P, C.c_char_p, C.c_char_p, _I32, C.POINTER(GcmTableJob)]),
+    "gcm_clear_table_updates": (C.c_int, [_P]),
+    "gcm_get_table_values": (C.c_int, [_P, _I32, _P]),
     "gcm_enable_weights": (C.c_int, [_P, C.c_int]),
     "gcm_fill": (C.c_int, [_P, _P]),
     "gcm_sync": (C.c_int, [_P]),

@@ -38,6 +38,17 @@ def build_gcm() -> str:
     return out
 
 
+def build_table_update() -> str:
+    """The synthetic workloads' device table function as a code object for
+    gcm_add_table_update."""
+    out = os.path.join(HERE, "table_update_cos.hsaco")
+    tmp = out + ".tmp"
+    flags = [f for f in HIP_FLAGS if f not in ("-shared", "-fPIC")]
+    _run([HIPCC, "--genco", *flags, "-o", tmp, os.path.join(CSRC, "table_update_cos.hip")])
+    os.replace(tmp, out)
+    return out
+
+
 def build_synth() -> str:
     out = os.path.join(HERE, "libgcm_synth.so")
     tmp = out + ".tmp"
@@ -82,6 +93,7 @@ def build_facade_demo() -> str:
 def build_all() -> None:
     build_synth()
     build_gcm()
+    build_table_update()
     build_hbm_copy()
     build_lds_dma_stream()
     build_facade_demo()

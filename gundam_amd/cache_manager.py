@@ -184,6 +184,52 @@ class CacheManager:
         N.check(self._lib.gcm_set_table_values(self._h, v.ctypes.data,
                                                v.shape[1], v.shape[0]))
 
+    def add_table_update(self, code_object: str, kernel: str, jobs) -> None:
+        """Refresh lookup tables on the device at every fill with a gfx950
+        update function (include/gcm_table_update.h) instead of a host
+        callback + upload.  jobs: iterable of (table, [input indices],
+        user_id)."""
+        jobs = list(jobs)
+        keep = [np.ascontiguousarray(j[1], dtype=np.int32) for j in jobs]
+        arr = (N.GcmTableJob * len(jobs))()
+        for a, (t, _, uid), ins in zip(arr, jobs, keep):
+            a.table, a.n_inputs, a.inputs, a.user_id = int(t), ins.size, ins.ctypes.data, int(uid)
+        N.check(self._lib.gcm_add_table_update(self._h, code_object.encode(), kernel.encode(),
+                                               len(jobs), arr))
+
+    def clear_table_updates(self) -> None:
+        N.check(self._lib.gcm_clear_table_updates(self._h))
+
+    def device_table_values(self, k: int = 0) -> np.ndarray:
+        """The lookup tables point k was last evaluated with."""
+        out = np.zeros(max(1, self.table_len))
+        N.check(self._lib.gcm_get_table_values(self._h, int(k), out.ctypes.data))
+        return out[:self.table_len]
+
+    def set_penalty_sets(self, sets) -> None:
+        """Correlated prior penalties (gcm_penalty_set): each set is a dict
+        with 'parameters', 'prior' and optional 'inv_cov' ([n][n]), 'lower',
+        'upper'; they are added after the diagonal prior term in set order."""
+        sets = list(sets)
+        keep, arr = [], (N.GcmPenaltySet * max(1, len(sets)))()
+
+        def ptr(a, dt):
+            if a is None:
+                return None
+            a = np.ascontiguousarray(a, dtype=dt)
+            keep.append(a)
+            return a.ctypes.data
+
+        for a, st in zip(arr, sets):
+            par = np.asarray(st["parameters"])
+            a.n = par.size
+            a.parameters = ptr(par, np.int32)
+            a.prior = ptr(st["prior"], np.float64)
+            a.inv_cov = ptr(st.get("inv_cov"), np.float64)
+            a.lower = ptr(st.get("lower"), np.float64)
+            a.upper = ptr(st.get("upper"), np.float64)
+        N.check(self._lib.gcm_set_penalty_sets(self._h, len(sets), arr))
+
     def fill(self, stream: Optional[int] = None) -> Future:
         N.check(self._lib.gcm_fill(self._h, stream))
         return Future(self)

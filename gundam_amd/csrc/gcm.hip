@@ -38,6 +38,7 @@
 #include <vector>
 
 #include "../../include/gcm.h"
+#include "../../include/gcm_table_update.h"
 #include "dial_eval.hpp"
 
 namespace gcm {
@@ -105,6 +106,7 @@ struct InputRec {
 static_assert(sizeof(InputRec) == 32, "input record is 32 bytes");
 
 constexpr int kPenaltyChunk = 256;
+constexpr int kTableUpdateBlock = 128;  // threads per (job, point) table update
 
 // Per point k (one block each, 256 threads): mirrored input values and the
 // prior penalty.
@@ -166,6 +168,74 @@ __global__ void prologue_kernel(const double* __restrict__ theta, int n_par,
     }
   }
   if (threadIdx.x == 0) penalty[k] = pen;
+}
+
+// Correlated penalty sets, one block per point, added to the prologue's
+// diagonal term in set order (LikelihoodInterface.cpp:186-213).  Per set:
+// d_i = theta[par_i] - prior_i; y_i = 0 + sum_j A[i][j] d_j with j ascending
+// (one thread per row, ROOT's TMatrixD x TVectorD order); the set's value is
+// sum_i d_i y_i from 0 in index order (one thread), or +inf when a bounded
+// parameter is NaN or out of its range.  meta[s] = {first, n, has_matrix,
+// has_bounds}; inv_t holds each matrix transposed so that row i is read down
+// column i by consecutive threads.
+__global__ void penalty_sets_kernel(const double* __restrict__ theta, int n_par,
+                                    int n_sets, const int64_t* __restrict__ meta,
+                                    const int32_t* __restrict__ par,
+                                    const double* __restrict__ prior,
+                                    const double* __restrict__ lo,
+                                    const double* __restrict__ hi,
+                                    const double* __restrict__ inv_t,
+                                    const int64_t* __restrict__ mat_off,
+                                    int64_t total_n, double* __restrict__ scratch,
+                                    double* __restrict__ penalty) {
+  const int k = blockIdx.x;
+  const double* th = theta + (size_t)k * n_par;
+  double* dv = scratch + (size_t)k * 2 * total_n;
+  double* yv = dv + total_n;
+  __shared__ int s_out;
+  double pen = 0;
+  if (threadIdx.x == 0) pen = penalty[k];
+  for (int s = 0; s < n_sets; ++s) {
+    const int64_t first = meta[4 * s], n = meta[4 * s + 1];
+    const bool has_mat = meta[4 * s + 2] != 0, has_bounds = meta[4 * s + 3] != 0;
+    if (threadIdx.x == 0) s_out = 0;
+    __syncthreads();
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+      const double v = th[par[first + i]];
+      dv[first + i] = v - prior[first + i];
+      // NaN bounds compare false: no bound on that side
+      if (has_bounds && (isnan(v) || v < lo[first + i] || v > hi[first + i])) s_out = 1;
+    }
+    __syncthreads();
+    if (has_mat) {
+      const double* A = inv_t + mat_off[s];
+      for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
+        double sum = 0;
+        for (int64_t j = 0; j < n; ++j) sum += A[j * n + i] * dv[first + j];
+        yv[first + i] = 0.0 + sum;
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      double buf = 0;
+      if (has_mat)
+        for (int64_t i = 0; i < n; ++i) buf += dv[first + i] * yv[first + i];
+      if (s_out) buf = __builtin_huge_val();
+      pen += buf;
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) penalty[k] = pen;
+}
+
+// The input values of every device table-update job: out[k][i] = the mirrored
+// value of input idx[i] at point k (DialInputBuffer semantics).
+__global__ void table_inputs_kernel(const InputRec* __restrict__ xin, int n_inputs,
+                                    const int32_t* __restrict__ idx, int n_idx,
+                                    double* __restrict__ out) {
+  const int k = blockIdx.x;
+  for (int i = threadIdx.x; i < n_idx; i += blockDim.x)
+    out[(size_t)k * n_idx + i] = xin[(size_t)k * n_inputs + idx[i]].x;
 }
 
 // Knuth two-sum: s + e == a + b exactly.
@@ -843,6 +913,31 @@ struct gcm_context {
   int64_t t_issued = 0, t_done = 0;
   double t_fill_ms = 0, t_fin_ms = 0;
 
+  // correlated penalty sets (gcm_set_penalty_sets), concatenated
+  int n_pen_sets = 0;
+  int64_t* d_pen_meta = nullptr;  // [n_sets][4]: first, n, has_matrix, has_bounds
+  int32_t* d_pen_par = nullptr;   // [sum n]
+  double *d_pen_prior = nullptr, *d_pen_lo = nullptr, *d_pen_hi = nullptr;
+  double* d_pen_inv_t = nullptr;  // per set [n][n], transposed (column i contiguous
+                                  // over rows is row i of the inverse)
+  int64_t* d_pen_mat_off = nullptr;  // [n_sets] offsets into d_pen_inv_t
+  double* d_pen_scratch = nullptr;   // [max_points][2 * sum n]: deltas, rows
+  int64_t pen_total_n = 0;
+  // device table refresh (gcm_add_table_update)
+  struct TableUpdate {
+    hipModule_t mod = nullptr;
+    hipFunction_t fn = nullptr;
+    void* d_jobs = nullptr;
+    int n_jobs = 0;
+  };
+  std::vector<TableUpdate> tab_updates;
+  std::vector<char> tab_covered;       // [n_tables]
+  std::vector<int32_t> tab_in_idx;     // every job's input indices, in job order
+  int32_t* d_tab_in_idx = nullptr;
+  double* d_tab_in = nullptr;          // [max_points][tab_in_idx.size()]
+  bool host_tab = false;               // gcm_set_table_values has been called
+  std::vector<int64_t> tab_begin;      // [n_tables+1]
+
   template <typename T>
   int alloc(T** p, size_t count) {
     *p = nullptr;
@@ -851,14 +946,32 @@ struct gcm_context {
     if (e != hipSuccess)
       return fail(GCM_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
     allocations.push_back(*p);
+    sizes.push_back((int64_t)(count * sizeof(T)));
     resident += (int64_t)(count * sizeof(T));
     return GCM_OK;
+  }
+  // Free one allocation made by alloc (no-op for NULL); the device must be
+  // done with it.
+  template <typename T>
+  void release(T** p) {
+    if (!*p) return;
+    for (size_t i = 0; i < allocations.size(); ++i)
+      if (allocations[i] == (void*)*p) {
+        (void)hipFree(allocations[i]);
+        resident -= sizes[i];
+        allocations.erase(allocations.begin() + i);
+        sizes.erase(sizes.begin() + i);
+        break;
+      }
+    *p = nullptr;
   }
   void free_all() {
     for (void* p : allocations) (void)hipFree(p);
     allocations.clear();
+    sizes.clear();
     resident = 0;
   }
+  std::vector<int64_t> sizes;  // bytes of each entry of allocations
 };
 
 #define GCM_TRY(expr)         \
@@ -1220,6 +1333,8 @@ int gcm_destroy(gcm_context* c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   c->drop_graphs();
+  for (auto& u : c->tab_updates) (void)hipModuleUnload(u.mod);
+  c->tab_updates.clear();
   c->free_all();
   for (auto& tr : c->tev)
     for (auto& e : tr)
@@ -1294,6 +1409,7 @@ int gcm_build(gcm_context* c, const gcm_tables* t, int32_t max_points) {
   c->n_bins = t->n_bins;
   c->n_tables = t->n_tables;
   c->table_len = t->n_tables ? t->table_begin[t->n_tables] : 0;
+  if (t->n_tables) c->tab_begin.assign(t->table_begin, t->table_begin + t->n_tables + 1);
   c->max_points = max_points;
   c->cap = std::isfinite(t->reweight_cap) ? t->reweight_cap : __builtin_huge_val();
   // dial slots per tile: enough for the densest events (norm dials have no
@@ -1426,6 +1542,10 @@ int gcm_set_table_values(gcm_context* c, const double* values, int64_t table_len
   GCM_TRY(wait_staging(c));
   std::memcpy(c->h_tab, values, (size_t)k_points * table_len * sizeof(double));
   c->tab_dirty = true;
+  if (!c->host_tab) {
+    c->host_tab = true;
+    c->drop_graphs();  // the captured uploads now include the tables
+  }
   return GCM_OK;
 }
 
@@ -1479,6 +1599,160 @@ int gcm_set_likelihood(gcm_context* c, const gcm_llh_config* cfg,
   return GCM_OK;
 }
 
+int gcm_set_penalty_sets(gcm_context* c, int32_t n_sets, const gcm_penalty_set* sets) {
+  if (!c || (n_sets > 0 && !sets)) return fail(GCM_ERR_ARG, "NULL argument");
+  if (!c->built) return fail(GCM_ERR_STATE, "gcm_build first");
+  if (n_sets < 0) return fail(GCM_ERR_ARG, "n_sets < 0");
+  std::vector<int64_t> meta, mat_off;
+  std::vector<int32_t> par;
+  std::vector<double> prior, lo, hi, inv_t;
+  const double nan = std::nan("");
+  for (int s = 0; s < n_sets; ++s) {
+    const gcm_penalty_set& ps = sets[s];
+    if (ps.n < 0 || (ps.n > 0 && (!ps.parameters || !ps.prior)))
+      return fail(GCM_ERR_ARG, "penalty set " + std::to_string(s) + ": missing arrays");
+    if ((ps.lower == nullptr) != (ps.upper == nullptr))
+      return fail(GCM_ERR_ARG, "penalty set " + std::to_string(s) + ": give both bounds or neither");
+    const int64_t first = (int64_t)par.size(), n = ps.n;
+    meta.insert(meta.end(), {first, n, ps.inv_cov ? 1 : 0, ps.lower ? 1 : 0});
+    mat_off.push_back((int64_t)inv_t.size());
+    for (int64_t i = 0; i < n; ++i) {
+      if (ps.parameters[i] < 0 || ps.parameters[i] >= c->n_par)
+        return fail(GCM_ERR_ARG, "penalty set " + std::to_string(s) + ": parameter index out of range");
+      par.push_back(ps.parameters[i]);
+      prior.push_back(ps.prior[i]);
+      lo.push_back(ps.lower ? ps.lower[i] : nan);
+      hi.push_back(ps.upper ? ps.upper[i] : nan);
+    }
+    if (ps.inv_cov)
+      for (int64_t j = 0; j < n; ++j)
+        for (int64_t i = 0; i < n; ++i) inv_t.push_back(ps.inv_cov[i * n + j]);
+  }
+  GCM_HIP(hipSetDevice(c->device));
+  GCM_HIP(hipStreamSynchronize(c->last_stream ? c->last_stream : c->stream));
+  c->release(&c->d_pen_meta);
+  c->release(&c->d_pen_par);
+  c->release(&c->d_pen_prior);
+  c->release(&c->d_pen_lo);
+  c->release(&c->d_pen_hi);
+  c->release(&c->d_pen_inv_t);
+  c->release(&c->d_pen_mat_off);
+  c->release(&c->d_pen_scratch);
+  c->n_pen_sets = 0;
+  c->pen_total_n = 0;
+  c->drop_graphs();  // kernel arguments change
+  if (n_sets == 0) return GCM_OK;
+  GCM_TRY(upload(c, &c->d_pen_meta, meta.data(), meta.size()));
+  GCM_TRY(upload(c, &c->d_pen_par, par.data(), par.size()));
+  GCM_TRY(upload(c, &c->d_pen_prior, prior.data(), prior.size()));
+  GCM_TRY(upload(c, &c->d_pen_lo, lo.data(), lo.size()));
+  GCM_TRY(upload(c, &c->d_pen_hi, hi.data(), hi.size()));
+  GCM_TRY(upload(c, &c->d_pen_inv_t, inv_t.data(), inv_t.size()));
+  GCM_TRY(upload(c, &c->d_pen_mat_off, mat_off.data(), mat_off.size()));
+  GCM_TRY(c->alloc(&c->d_pen_scratch, (size_t)c->max_points * 2 * std::max<size_t>(1, par.size())));
+  c->n_pen_sets = n_sets;
+  c->pen_total_n = (int64_t)par.size();
+  return GCM_OK;
+}
+
+int gcm_add_table_update(gcm_context* c, const char* path, const char* kernel_name,
+                         int32_t n_jobs, const gcm_table_job* jobs) {
+  if (!c || !path || !kernel_name || (n_jobs > 0 && !jobs)) return fail(GCM_ERR_ARG, "NULL argument");
+  if (!c->built) return fail(GCM_ERR_STATE, "gcm_build first");
+  if (n_jobs < 1) return fail(GCM_ERR_ARG, "n_jobs < 1");
+  if ((int64_t)n_jobs > 65535) return fail(GCM_ERR_ARG, "more than 65535 jobs in one update");
+  c->tab_covered.resize((size_t)c->n_tables, 0);
+  std::vector<char> covered = c->tab_covered;
+  std::vector<int32_t> idx = c->tab_in_idx;
+  std::vector<gcm_table_job_dev> dj((size_t)n_jobs);
+  for (int j = 0; j < n_jobs; ++j) {
+    const gcm_table_job& jb = jobs[j];
+    const std::string where = "table job " + std::to_string(j) + ": ";
+    if (jb.table < 0 || jb.table >= c->n_tables) return fail(GCM_ERR_ARG, where + "table out of range");
+    if (covered[jb.table]) return fail(GCM_ERR_ARG, where + "table already has an update");
+    if (jb.n_inputs < 0 || (jb.n_inputs > 0 && !jb.inputs)) return fail(GCM_ERR_ARG, where + "missing inputs");
+    covered[jb.table] = 1;
+    dj[j].table_offset = c->tab_begin[jb.table];
+    dj[j].size = (int32_t)(c->tab_begin[jb.table + 1] - c->tab_begin[jb.table]);
+    dj[j].input_offset = (int64_t)idx.size();
+    dj[j].n_inputs = jb.n_inputs;
+    dj[j].user_id = jb.user_id;
+    dj[j].reserved = 0;
+    for (int i = 0; i < jb.n_inputs; ++i) {
+      if (jb.inputs[i] < 0 || jb.inputs[i] >= c->n_inputs)
+        return fail(GCM_ERR_ARG, where + "input index out of range");
+      idx.push_back(jb.inputs[i]);
+    }
+  }
+  GCM_HIP(hipSetDevice(c->device));
+  gcm_context::TableUpdate u;
+  // a failed module call also leaves its error as the thread's last HIP
+  // error, which a later hipGetLastError() after a kernel launch would
+  // report: clear it on these paths
+  hipError_t e = hipModuleLoad(&u.mod, path);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return fail(GCM_ERR_ARG, std::string("cannot load code object ") + path + ": " + hipGetErrorString(e));
+  }
+  e = hipModuleGetFunction(&u.fn, u.mod, kernel_name);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    (void)hipModuleUnload(u.mod);
+    return fail(GCM_ERR_ARG, std::string("no kernel ") + kernel_name + " in " + path);
+  }
+  GCM_HIP(hipStreamSynchronize(c->last_stream ? c->last_stream : c->stream));
+  gcm_table_job_dev* d_jobs = nullptr;
+  int rc = upload(c, &d_jobs, dj.data(), dj.size());
+  if (rc == GCM_OK && idx.size() != c->tab_in_idx.size()) {
+    c->release(&c->d_tab_in_idx);
+    c->release(&c->d_tab_in);
+    rc = upload(c, &c->d_tab_in_idx, idx.data(), idx.size());
+    if (rc == GCM_OK) rc = c->alloc(&c->d_tab_in, (size_t)c->max_points * std::max<size_t>(1, idx.size()));
+  }
+  if (rc != GCM_OK) {
+    c->release(&d_jobs);
+    (void)hipModuleUnload(u.mod);
+    return rc;
+  }
+  u.d_jobs = d_jobs;
+  u.n_jobs = n_jobs;
+  c->tab_updates.push_back(u);
+  c->tab_covered = covered;
+  c->tab_in_idx = idx;
+  c->drop_graphs();
+  return GCM_OK;
+}
+
+int gcm_clear_table_updates(gcm_context* c) {
+  if (!c) return fail(GCM_ERR_ARG, "ctx is NULL");
+  if (c->tab_updates.empty()) return GCM_OK;
+  GCM_HIP(hipSetDevice(c->device));
+  GCM_HIP(hipStreamSynchronize(c->last_stream ? c->last_stream : c->stream));
+  c->drop_graphs();
+  for (auto& u : c->tab_updates) {
+    c->release(&u.d_jobs);
+    (void)hipModuleUnload(u.mod);
+  }
+  c->tab_updates.clear();
+  c->tab_covered.assign((size_t)c->n_tables, 0);
+  c->tab_in_idx.clear();
+  c->release(&c->d_tab_in_idx);
+  c->release(&c->d_tab_in);
+  return GCM_OK;
+}
+
+int gcm_get_table_values(gcm_context* c, int32_t k, double* values) {
+  if (!c || !values) return fail(GCM_ERR_ARG, "NULL argument");
+  if (!c->built) return fail(GCM_ERR_STATE, "gcm_build first");
+  if (k < 0 || k >= c->max_points) return fail(GCM_ERR_ARG, "k out of [0, max_points)");
+  if (c->table_len == 0) return GCM_OK;
+  GCM_HIP(hipSetDevice(c->device));
+  GCM_HIP(hipStreamSynchronize(c->last_stream ? c->last_stream : c->stream));
+  GCM_HIP(hipMemcpy(values, c->d_tab + (size_t)k * c->table_len, (size_t)c->table_len * sizeof(double),
+                    hipMemcpyDeviceToHost));
+  return GCM_OK;
+}
+
 int gcm_enable_weights(gcm_context* c, int enable) {
   if (!c) return fail(GCM_ERR_ARG, "ctx is NULL");
   if (!c->built) return fail(GCM_ERR_STATE, "gcm_build first");
@@ -1515,7 +1789,7 @@ static int enqueue_fill(gcm_context* c, hipStream_t s, int K, bool copies,
   if (copies) {
     GCM_HIP(hipMemcpyAsync(c->d_theta, c->h_theta, (size_t)K * c->n_par * sizeof(double),
                            hipMemcpyHostToDevice, s));
-    if (c->table_len)
+    if (c->table_len && c->host_tab)
       GCM_HIP(hipMemcpyAsync(c->d_tab, c->h_tab, (size_t)K * c->table_len * sizeof(double),
                              hipMemcpyHostToDevice, s));
   }
@@ -1525,6 +1799,33 @@ static int enqueue_fill(gcm_context* c, hipStream_t s, int K, bool copies,
                                     c->llh_set ? c->d_prior_mean : nullptr,
                                     c->llh_set ? c->d_prior_sigma : nullptr, c->d_penalty);
   GCM_HIP(hipGetLastError());
+  if (c->n_pen_sets > 0) {
+    penalty_sets_kernel<<<K, 256, 0, s>>>(c->d_theta, c->n_par, c->n_pen_sets, c->d_pen_meta,
+                                          c->d_pen_par, c->d_pen_prior, c->d_pen_lo, c->d_pen_hi,
+                                          c->d_pen_inv_t, c->d_pen_mat_off, c->pen_total_n,
+                                          c->d_pen_scratch, c->d_penalty);
+    GCM_HIP(hipGetLastError());
+  }
+  if (!c->tab_updates.empty()) {
+    // device table refresh: gather the jobs' mirrored inputs, then one launch
+    // of each update function over (job, point)
+    const int n_idx = (int)c->tab_in_idx.size();
+    if (n_idx > 0) {
+      table_inputs_kernel<<<K, 256, 0, s>>>(c->d_xin, c->n_inputs, c->d_tab_in_idx, n_idx,
+                                            c->d_tab_in);
+      GCM_HIP(hipGetLastError());
+    }
+    for (auto& u : c->tab_updates) {
+      const void* jobs = u.d_jobs;
+      double* tab = c->d_tab;
+      int64_t tab_stride = c->table_len;
+      const double* tin = c->d_tab_in;
+      int64_t in_stride = n_idx;
+      void* args[] = {&jobs, &tab, &tab_stride, &tin, &in_stride};
+      GCM_HIP(hipModuleLaunchKernel(u.fn, (unsigned)u.n_jobs, (unsigned)K, 1, kTableUpdateBlock,
+                                    1, 1, 0, s, args, nullptr));
+    }
+  }
   const bool fp32 = c->precision == GCM_PRECISION_FP32;
   FillArgs a;
   a.tiles = c->d_tiles;

@@ -84,6 +84,23 @@ class CacheManager {
           "gcm_set_likelihood");
   }
 
+  // Correlated prior penalties per parameter set (LikelihoodInterface.cpp:
+  // 186-213), added after the diagonal term in set order.
+  void SetPenaltySets(const std::vector<gcm_penalty_set>& sets) {
+    check(gcm_set_penalty_sets(ctx_, (int32_t)sets.size(), sets.empty() ? nullptr : sets.data()),
+          "gcm_set_penalty_sets");
+  }
+
+  // Device-side table refresh: the update kernel `kernel` of the code object
+  // at `code_object` (include/gcm_table_update.h) fills the jobs' tables at
+  // every Fill, in place of table_values (TabulatedDialFactory.cpp:133-141).
+  void AddTableUpdate(const std::string& code_object, const std::string& kernel,
+                      const std::vector<gcm_table_job>& jobs) {
+    check(gcm_add_table_update(ctx_, code_object.c_str(), kernel.c_str(), (int32_t)jobs.size(),
+                               jobs.data()),
+          "gcm_add_table_update");
+  }
+
   // Cache::Manager::Fill (CacheManager.cpp:750-810): upload the parameter
   // point(s) and the lookup tables, enqueue the evaluation, and return a
   // deferred future whose get() synchronises and copies the histograms when

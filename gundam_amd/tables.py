@@ -125,6 +125,10 @@ class Tables:
                    + 8 * self.n_parameters + 16 * self.n_bins)
 
 
+# generator parameter kinds that own a lookup table (synth.cpp ParType)
+PAR_TABULATED, PAR_KRIGED = 6, 7
+
+
 class Workload:
     """A seeded synthetic workload from libgcm_synth (see synth.cpp)."""
 
@@ -184,6 +188,19 @@ class Workload:
             N.synth().gcm_synth_table_values(self._h, theta[k].ctypes.data,
                                              out[k].ctypes.data)
         return out[:, :self.tables.table_len].copy()
+
+    def table_jobs(self) -> list:
+        """Device table-update jobs equivalent to table_values: table t is the
+        t-th tabulated/kriged parameter in parameter order, read through the
+        first input of that parameter; one job per table for the code object
+        table_update_cos.hsaco (kernel gcm_synth_cos_table)."""
+        pars = [p for p in range(self.n_parameters)
+                if self.par_type[p] in (PAR_TABULATED, PAR_KRIGED)]
+        assert len(pars) == self.tables.n_tables
+        first_input = {}
+        for i, p in enumerate(self.tables.inputs["parameter"]):
+            first_input.setdefault(int(p), i)
+        return [(t, [first_input[p]], 0) for t, p in enumerate(pars)]
 
     def nominal(self) -> np.ndarray:
         return self.prior_mean.copy()

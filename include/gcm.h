@@ -214,6 +214,58 @@ int gcm_set_table_values(gcm_context* ctx, const double* values,
 int gcm_set_likelihood(gcm_context* ctx, const gcm_llh_config* cfg,
                        const double* data_hist, const double* prior_mean,
                        const double* prior_sigma);
+/* Correlated prior penalty of one parameter set (LikelihoodInterface.cpp:
+ * 186-213, ParameterSet::updateDeltaVector ParameterSet.cpp:456-463): with
+ * d_i = theta[parameters[i]] - prior[i], the set adds
+ *   sum_i d_i * (sum_j inv_cov[i][j] * d_j)
+ * summed from 0 in index order (TMatrixD x TVectorD, then the dot product),
+ * or +inf when lower/upper are given and some theta[parameters[i]] is NaN or
+ * outside [lower[i], upper[i]] (NaN bound: none; the reference's
+ * applyInfinitePenaltyOnOutOfBoundPar, ParameterSet.cpp:441-453).  inv_cov
+ * NULL: bounds only.  The caller lists the set's valid correlated parameters
+ * (ParameterSet.cpp:225-233) and passes the inverse of their covariance. */
+typedef struct gcm_penalty_set {
+  int32_t n;
+  const int32_t* parameters;  /* [n] indices into the parameter vector      */
+  const double* prior;        /* [n]                                        */
+  const double* inv_cov;      /* [n][n] row-major, or NULL                  */
+  const double* lower;        /* [n] or NULL                                */
+  const double* upper;        /* [n] or NULL                                */
+} gcm_penalty_set;
+/* Replaces the penalty sets (n_sets = 0 clears them).  The penalty of a point
+ * is (the diagonal prior term of gcm_set_likelihood) + set 0 + set 1 + ...,
+ * in that order; with no diagonal term this is the reference's sum over
+ * parameter sets.  The arrays are copied. */
+int gcm_set_penalty_sets(gcm_context* ctx, int32_t n_sets,
+                         const gcm_penalty_set* sets);
+
+/* Device-side Tabulated/Kriged table refresh, in place of the per-Fill host
+ * callback and upload (TabulatedDialFactory.cpp:133-141 calls a user function
+ * found with dlsym; WeightTabulated.cpp:164-171 and WeightKriged.cpp:216-223
+ * copy the result to the device).  The user function is a gfx950 kernel in a
+ * code object (hipcc --genco), written with include/gcm_table_update.h; each
+ * gcm_fill launches it once, one block per (job, point), after the input
+ * mirroring and before the dial evaluation, so graph replays refresh too.
+ * A job fills table `table` of every point from the mirrored values of its
+ * inputs (gcm_input indices: DialInputBuffer semantics), in the order given.
+ * Tables that no job covers keep the values of gcm_set_table_values (or 0).
+ * A table may be covered by at most one job over all calls. */
+typedef struct gcm_table_job {
+  int32_t table;          /* index into gcm_tables.table_begin              */
+  int32_t n_inputs;
+  const int32_t* inputs;  /* [n_inputs] indices into gcm_tables.inputs      */
+  int32_t user_id;        /* passed to the function (the reference passes
+                             the factory name)                             */
+} gcm_table_job;
+/* Load `kernel_name` from the code object file and add its jobs. */
+int gcm_add_table_update(gcm_context* ctx, const char* code_object_path,
+                         const char* kernel_name, int32_t n_jobs,
+                         const gcm_table_job* jobs);
+/* Remove every device table update (and unload the code objects). */
+int gcm_clear_table_updates(gcm_context* ctx);
+/* The lookup tables point k was last evaluated with: [table_len]. */
+int gcm_get_table_values(gcm_context* ctx, int32_t k, double* values);
+
 /* Keep per-event weights on the device so gcm_get_weights can fetch them
  * (Cache::Manager::SetIsEventWeightCopyEnabled, CacheManager.h:104-109). */
 int gcm_enable_weights(gcm_context* ctx, int enable);

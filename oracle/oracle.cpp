@@ -666,6 +666,43 @@ int oracle_llh(const gcm_llh_config* cfg, int n_bins, const double* data,
   return 0;
 }
 
+// LikelihoodInterface::evalPenaltyLikelihood(const ParameterSet&)
+// (LikelihoodInterface.cpp:186-213) for a set with a prior covariance and no
+// eigen decomposition: the delta vector of the valid correlated parameters
+// (ParameterSet::updateDeltaVector, ParameterSet.cpp:456-463), the inverse
+// covariance times it (ROOT's TMatrixD * TVectorD: per row a sum from 0 over
+// the columns in order, added to the zeroed target), then the dot product
+// from 0 in index order.  With bounds (applyInfinitePenaltyOnOutOfBoundPar,
+// ParameterSet.cpp:441-453: a NaN value or one outside [lower, upper] is out
+// of bounds; a NaN bound is no bound) an out-of-bounds set gives +inf.
+// inv_cov NULL: no quadratic term (a set without a covariance matrix).
+double oracle_penalty_set(int n, const double* theta, const int32_t* parameters,
+                          const double* prior, const double* inv_cov,
+                          const double* lower, const double* upper) {
+  std::vector<double> delta(n), y(n, 0.0);
+  for (int i = 0; i < n; ++i) delta[i] = theta[parameters[i]] - prior[i];
+  double buffer = 0;
+  if (inv_cov) {
+    for (int i = 0; i < n; ++i) {
+      double sum = 0;
+      for (int j = 0; j < n; ++j) sum += delta[j] * inv_cov[(int64_t)i * n + j];
+      y[i] += sum;
+    }
+    double dot = 0;
+    for (int i = 0; i < n; ++i) dot += delta[i] * y[i];
+    buffer = dot;
+  }
+  if (lower && upper) {
+    bool out = false;
+    for (int i = 0; i < n; ++i) {
+      const double v = theta[parameters[i]];
+      if (std::isnan(v) || v < lower[i] || v > upper[i]) out = true;
+    }
+    if (out) return std::numeric_limits<double>::infinity();
+  }
+  return buffer;
+}
+
 // LoaderUtils.cpp:32-40 with Bin::isBetweenEdges (Bin.cpp:167-176), default
 // [min, max) edges.
 int oracle_bin_events(int64_t n_events, int n_vars, const double* values,

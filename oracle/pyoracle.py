@@ -60,6 +60,8 @@ def lib() -> C.CDLL:
         l.oracle_llh.argtypes = [C.POINTER(LlhConfig), C.c_int, _P, _P, _P,
                                  C.c_int, _P, _P, _P, _P, _P, _P]
         l.oracle_bin_events.argtypes = [_I64, C.c_int, _P, C.c_int, _P, _P]
+        l.oracle_penalty_set.restype = _D
+        l.oracle_penalty_set.argtypes = [C.c_int, _P, _P, _P, _P, _P, _P]
         l.oracle_calc.restype = _D
         l.oracle_calc.argtypes = [C.c_int, _D, _D, _D, _P, C.c_int]
         l.oracle_calc2d.restype = _D
@@ -157,6 +159,34 @@ def llh(kind, data, sums, sqrt_sum_sq, theta=None, prior_mean=None,
                      None if ps is None else ps.ctypes.data,
                      C.byref(tot), C.byref(st), C.byref(pen))
     return tot.value, st.value, pen.value
+
+
+def penalty_set(theta, parameters, prior, inv_cov=None, lower=None, upper=None) -> float:
+    """One parameter set's penalty (oracle_penalty_set)."""
+    keep = []
+
+    def ptr(a, dt):
+        if a is None:
+            return None
+        a = np.ascontiguousarray(a, dtype=dt)
+        keep.append(a)
+        return a.ctypes.data
+
+    par = np.asarray(parameters)
+    return lib().oracle_penalty_set(par.size, ptr(theta, np.float64), ptr(par, np.int32),
+                                    ptr(prior, np.float64), ptr(inv_cov, np.float64),
+                                    ptr(lower, np.float64), ptr(upper, np.float64))
+
+
+def penalty(theta, sets, prior_mean=None, prior_sigma=None) -> float:
+    """The diagonal prior term (as in llh) followed by the sets in order."""
+    pen = 0.0
+    if prior_sigma is not None:
+        pen = llh("poisson", [], [], [], theta, prior_mean, prior_sigma)[2]
+    for st in sets:
+        pen += penalty_set(theta, st["parameters"], st["prior"], st.get("inv_cov"),
+                           st.get("lower"), st.get("upper"))
+    return pen
 
 
 def bin_events(values: np.ndarray, edges: np.ndarray) -> np.ndarray:

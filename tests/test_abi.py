@@ -16,7 +16,9 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def _declared():
-    text = open(os.path.join(ROOT, "include", "gcm.h")).read()
+    inc = os.path.join(ROOT, "include")
+    text = "\n".join(open(os.path.join(inc, f)).read() for f in sorted(os.listdir(inc))
+                     if f.endswith(".h"))
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     return sorted(set(re.findall(r"\b(gcm_[a-z0-9_]+)\s*\(", text)))
 
@@ -102,7 +104,8 @@ def test_struct_layouts_match_the_header(tmp_path):
     """The ctypes mirrors in _native.py have the sizes and field offsets the
     C compiler gives the structs of include/gcm.h."""
     structs = {"gcm_input": N.GcmInput, "gcm_tables": N.GcmTables,
-               "gcm_llh_config": N.GcmLlhConfig, "gcm_launch_config": N.GcmLaunchConfig}
+               "gcm_llh_config": N.GcmLlhConfig, "gcm_launch_config": N.GcmLaunchConfig,
+               "gcm_penalty_set": N.GcmPenaltySet, "gcm_table_job": N.GcmTableJob}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "gcm.h"', "int main(void) {"]
     for cname, py in structs.items():
         lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
