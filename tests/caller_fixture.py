@@ -1,0 +1,154 @@
+"""A small end-to-end configuration for the YAML-driven caller
+(gundam_amd/propagator.py), written into a temporary directory: event stores
+for one MC and one data sample, the reference's 200CovarianceFit binning file
+(tests/golden/binning/), a 4x4 covariance in an .npz, and a configuration with
+the reference's keys (modelled on its tests/fast-tests/200CovarianceFit-config.yaml:
+two normalisations and two splines under a covariance, plus a set without
+covariance holding a mirrored graph and a bounded normalisation)."""
+import os
+import shutil
+
+import numpy as np
+
+from gundam_amd.eventstore import write_store
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BINNING = os.path.join(HERE, "golden", "binning", "200CovarianceFit-binning.txt")
+
+CONFIG = """
+fitterEngineConfig:
+  likelihoodInterfaceConfig:
+    applyInfinitePenaltyOnOutOfBoundPar: true
+    jointProbabilityConfig:
+      type: {llh}
+      {llh_opts}
+  propagatorConfig:
+    globalEventReweightCap: {{isEnabled: true, value: 3.5}}
+    dataSetList:
+      - name: "TestSample"
+        isEnabled: true
+        selectedDataEntry: "{data_entry}"
+        mc:
+          filePathList: ["${{CONFIG_DIR}}/mc.store"]
+          selectionCutFormula: "([A] > -9) && !([B] > 9.5)"
+          nominalWeightFormula: "(1.0 + 0.1*[C]*[C])"
+        data:
+          - name: "TestData"
+            filePathList: ["${{CONFIG_DIR}}/data.store"]
+    fitSampleSetConfig:
+      fitSampleList:
+        - name: AB
+          isEnabled: true
+          binning: "${{CONFIG_DIR}}/binning.txt"
+          datasets: ["TestSample"]
+        - name: AB_negC
+          binning: "${{CONFIG_DIR}}/binning.txt"
+          selectionCutStr: "[C] < 0"
+          dataSets: ["TestSample"]
+        - name: disabled
+          isEnabled: false
+          binning: "${{CONFIG_DIR}}/binning.txt"
+    parameterSetListConfig:
+      - name: CovarianceConstraints
+        isEnabled: true
+        covarianceMatrixFilePath: "${{CONFIG_DIR}}/cov.npz"
+        covarianceMatrixTMatrixD: CovarianceInputCovariance
+        parameterNameTObjArray: CovarianceInputNames
+        parameterPriorTVectorD: CovarianceInputPriors
+        parameterDefinitions:
+          - parameterName: "norm_A"
+            dialSetDefinitions:
+              - dialsType: Normalization
+                applyOnDataSets: ["TestSample"]
+                applyCondition: "[C] > 0"
+          - parameterName: "norm_B"
+            parameterLimits: [0.0, 3.0]
+            dialSetDefinitions:
+              - dialsType: Normalization
+                applyOnDataSets: ["*"]
+                applyCondition: "[C] <= 0"
+          - parameterName: "spline_C"
+            dialSetDefinitions:
+              - dialsType: Spline
+                dialLeafName: "spline_C"
+                applyOnDataSets: ["TestSample"]
+                maxDialResponse: 2.5
+          - parameterName: "spline_D"
+            dialSetDefinitions:
+              - dialType: Spline
+                treeExpression: "spline_D"
+                allowDialExtrapolation: false
+      - name: Extra
+        parameterDefinitions:
+          - parameterName: "graph_E"
+            priorValue: 0.0
+            parameterLimits: [-4.0, null]
+            dialSetDefinitions:
+              - dialsType: Graph
+                dialLeafName: "graph_E"
+                useMirrorDial: true
+                mirrorLowEdge: -1.5
+                mirrorHighEdge: 1.5
+                minDialResponse: 0.2
+          - parameterName: "flux"
+            priorValue: 1.0
+            dialSetDefinitions:
+              - dialsType: Normalization
+          - parameterName: "off"
+            isEnabled: false
+            dialSetDefinitions:
+              - dialsType: Normalization
+"""
+
+
+def _monotonic(rng, n):
+    y = np.maximum(1 + 0.1 * rng.standard_normal(7) * (np.arange(7) - 3.0), 1e-3)
+    return np.concatenate([[-3.0, 1.0], y])
+
+
+def _general(rng):
+    n = int(rng.integers(5, 9))
+    x = np.sort(np.concatenate([[-3.0, 3.0], rng.uniform(-2.9, 2.9, n - 2)]))
+    y = 1 + 0.15 * rng.standard_normal(n) * x
+    m = np.gradient(y, x)
+    out = np.empty(3 * n)
+    out[0::3], out[1::3], out[2::3] = y, m, x
+    return np.concatenate([[x[0], 0.0], out])
+
+
+def _graph(rng):
+    n = int(rng.integers(3, 7))
+    x = np.sort(np.concatenate([[-2.0, 2.0], rng.uniform(-1.9, 1.9, n - 2)]))
+    y = 1 + 0.2 * rng.standard_normal(n) * x
+    out = np.empty(2 * n)
+    out[0::2], out[1::2] = y, x
+    return out
+
+
+def write(tmp, n_mc=6000, n_data=4000, seed=3, llh="LeastSquares",
+          llh_opts="lsqPoissonianApproximation: true", data_entry="TestData"):
+    rng = np.random.default_rng(seed)
+
+    def events(n):
+        return {"A": rng.normal(0, 1.5, n), "B": rng.normal(0, 1.5, n), "C": rng.normal(0, 1, n)}
+
+    v = events(n_mc)
+    v["A"][:5] = [-10.0, 10.0, -2.0, 0.0, 2.0]  # on and beyond the bin edges
+    C = v["C"]
+    spl_c = [(_monotonic(rng, 7) if c > 0 else np.zeros(0)) for c in C]
+    spl_d = [(_general(rng) if c <= 0 else np.zeros(0)) for c in C]
+    gra_e = [(_graph(rng) if i % 3 else np.zeros(0)) for i in range(n_mc)]
+    write_store(os.path.join(tmp, "mc.store"), v, rng.uniform(0.5, 1.5, n_mc),
+                {"spline_C": ("monotonic", spl_c), "spline_D": ("general", spl_d),
+                 "graph_E": ("graph", gra_e)})
+    write_store(os.path.join(tmp, "data.store"), events(n_data), np.ones(n_data))
+    m = rng.standard_normal((4, 4))
+    cov = 0.05 * (m @ m.T) + 0.02 * np.eye(4)
+    np.savez(os.path.join(tmp, "cov.npz"), CovarianceInputCovariance=cov,
+             CovarianceInputNames=np.array(["norm_A", "norm_B", "spline_C", "spline_D"]),
+             CovarianceInputPriors=np.array([1.0, 1.0, 0.0, 0.0]))
+    shutil.copy(BINNING, os.path.join(tmp, "binning.txt"))
+    path = os.path.join(tmp, "config.yaml")
+    with open(path, "w") as f:
+        f.write(CONFIG.format(llh=llh, llh_opts=llh_opts, data_entry=data_entry))
+    return path
