@@ -13,24 +13,51 @@ from typing import Iterable, Optional
 
 import numpy as np
 
+from . import _native as N
 from .cache_manager import CacheManager
 from .tables import Tables
 
 LDS_BYTES_PER_CU = 160 * 1024
 
 
+def mean_record_bytes(t: Tables, precision: str = "fp64") -> float:
+    """Mean device record bytes per dial (gcm.hip record_words)."""
+    fp32 = precision == "fp32"
+    ty = t.dial_type.astype(np.int64)
+    ln = np.diff(t.data_begin).astype(np.int64)
+    w = ln.copy()  # splines and graphs: the payload length
+    w[ty == N.DIAL_NORM] = 0
+    w[ty == N.DIAL_SHIFT] = 1
+    w[ty == N.DIAL_TABULATED] = 2 if fp32 else 1
+    kr = ty == N.DIAL_KRIGED
+    w[kr] = ((ln[kr] - 1) // 2) * (2 if fp32 else 1)
+    grid = (ty == N.DIAL_BILINEAR) | (ty == N.DIAL_BICUBIC)
+    w[grid] = 1 + ln[grid]
+    n = max(1, w.size)
+    return max(float(w.sum()) * (4 if fp32 else 8) / n, 1.0) if w.size else 8.0
+
+
+def max_dials_for(tile_words: int, mean_bytes: float = 8.0) -> int:
+    """The library's dial slots per tile for a tile size (gcm.hip build_layout)."""
+    want = 1.12 * (tile_words * 8.0) / mean_bytes
+    return min(1024, max(256, (int(np.ceil(want)) + 127) // 128 * 128))
+
+
 def lds_per_block(tile_words: int, n_inputs: int, k_points: int = 1,
-                  ring_buffers: int = 2, blocks_per_cu: int = 1) -> int:
+                  ring_buffers: int = 2, blocks_per_cu: int = 1,
+                  mean_bytes: float = 8.0) -> int:
     """LDS of one fill block (mirrors the host computation in gcm.hip: the
-    input records are staged only when they are small and still let
-    blocks_per_cu blocks share the CU)."""
-    max_dials = min(1024, max(256, (tile_words // 8 + 127) // 128 * 128))
+    inputs' x values are staged only when they are small and still let
+    blocks_per_cu blocks share the CU; the 512 B of input-class records are
+    always staged)."""
+    max_dials = max_dials_for(tile_words, mean_bytes)
     ring = ring_buffers * (tile_words * 8 + max_dials * 8 + 64 * 16 + 32) + 2 * max_dials * 8
-    in_bytes = k_points * n_inputs * 32
-    staged = (in_bytes + 1023) // 1024 * 1024
+    in_bytes = k_points * n_inputs * 8
+    classes = 16 * 32
+    staged = (in_bytes + classes + 1023) // 1024 * 1024
     if in_bytes <= 16384 and (staged + ring) * blocks_per_cu <= LDS_BYTES_PER_CU:
         return staged + ring
-    return ring
+    return (classes + 1023) // 1024 * 1024 + ring
 
 
 def head(t: Tables, n_events: int) -> Tables:
@@ -47,13 +74,15 @@ def head(t: Tables, n_events: int) -> Tables:
                   _keep=[t])
 
 
-def candidates(n_inputs: int, precision: str = "fp64", k_points: int = 1) -> list:
+def candidates(n_inputs: int, precision: str = "fp64", k_points: int = 1,
+               mean_bytes: float = 8.0) -> list:
     out = []
     for nb, tile, block, bpc in itertools.product((2, 3), (1024, 1536, 2048, 2560, 3072, 3584,
                                                          3840, 4096),
                                                   (256, 384, 512, 768),
                                                   (1, 2, 3, 4, 5, 6)):
-        lds = lds_per_block(tile, n_inputs, k_points, ring_buffers=nb, blocks_per_cu=bpc)
+        lds = lds_per_block(tile, n_inputs, k_points, ring_buffers=nb, blocks_per_cu=bpc,
+                            mean_bytes=mean_bytes)
         # fill the CU's LDS: only the largest block count that fits
         if lds * bpc > LDS_BYTES_PER_CU or lds * (bpc + 1) <= LDS_BYTES_PER_CU:
             continue
@@ -74,7 +103,7 @@ def autotune(tables: Tables, theta: np.ndarray, table_values: Optional[np.ndarra
     k_points = theta.shape[0]
     sub = head(tables, n_events)
     cands = list(cands) if cands is not None else candidates(
-        int(tables.inputs.shape[0]), precision, k_points)
+        int(tables.inputs.shape[0]), precision, k_points, mean_record_bytes(sub, precision))
     results = []
     t_start = time.time()
     for c in cands:

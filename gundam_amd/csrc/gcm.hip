@@ -105,6 +105,24 @@ struct InputRec {
 };
 static_assert(sizeof(InputRec) == 32, "input record is 32 bytes");
 
+// Response supervisor and flags shared by the inputs of one class.  Dial
+// descriptors carry their input's class (4 bits), so that when the inputs'
+// x values are staged in LDS an evaluator lane reads x and its class record
+// from LDS and nothing from global memory; class kEscapeClass (workloads
+// with more than kInputClasses - 1 distinct (lo, hi, flags)) and the
+// unstaged case read the input's full record from global memory.
+struct ClassRec {
+  double lo;
+  double hi;
+  int32_t flags;
+  int32_t pad;
+  double pad2;
+};
+static_assert(sizeof(ClassRec) == 32, "class record is 32 bytes");
+constexpr int kInputClasses = 16;
+constexpr int kEscapeClass = kInputClasses - 1;
+constexpr size_t kClassBytes = kInputClasses * sizeof(ClassRec);
+
 constexpr int kPenaltyChunk = 256;
 constexpr int kTableUpdateBlock = 128;  // threads per (job, point) table update
 
@@ -247,6 +265,7 @@ __device__ __forceinline__ void two_sum(double a, double b, double& s,
 }
 
 struct FillArgs {
+  const ClassRec* cls;  // [kInputClasses]
   const TileHdr* tiles;
   int64_t n_tiles;
   const uint64_t* desc;
@@ -343,9 +362,11 @@ __global__ void __launch_bounds__(1024, 6) fill_kernel(FillArgs a) {
   const int eval_threads = n_eval * kWave;
   const int ev_wave = n_eval;
   const int prod0 = n_eval + 1;
-  const size_t in_bytes = kInputsLds ? (size_t)a.k_points * a.n_inputs * sizeof(InputRec) : 0;
-  const size_t in_pad = (in_bytes + 1023) / 1024 * 1024;
-  InputRec* s_in = reinterpret_cast<InputRec*>(lds);
+  // LDS: [input x values if staged][class records][tile ring][responses]
+  const size_t in_bytes = kInputsLds ? (size_t)a.k_points * a.n_inputs * sizeof(double) : 0;
+  const size_t in_pad = (in_bytes + kClassBytes + 1023) / 1024 * 1024;
+  double* s_x = reinterpret_cast<double*>(lds);
+  ClassRec* s_cls = reinterpret_cast<ClassRec*>(lds + in_bytes);
   const size_t hdr_off = (size_t)a.rec_bytes + (size_t)a.max_dials * 8 +
                          kMaxTileEvents * sizeof(EventRec);
   const size_t buf_bytes = hdr_off + kHeaderBytes;
@@ -358,8 +379,9 @@ __global__ void __launch_bounds__(1024, 6) fill_kernel(FillArgs a) {
 
   if (kInputsLds) {
     const int n = a.k_points * a.n_inputs;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) s_in[i] = a.xin[i];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) s_x[i] = a.xin[i].x;
   }
+  if (threadIdx.x < kInputClasses) s_cls[threadIdx.x] = a.cls[threadIdx.x];
   // producer p loads tile ordinal p into buffer p; tile 0 is waited for now,
   // tile p > 0 by producer p at the end of tile p - 1.  Each producer keeps
   // the header of its next tile in registers, fetched one DMA period ahead,
@@ -487,8 +509,8 @@ __global__ void __launch_bounds__(1024, 6) fill_kernel(FillArgs a) {
       {
         const R* s_rec = reinterpret_cast<const R*>(cur);
         const uint64_t* s_desc = reinterpret_cast<const uint64_t*>(cur + a.rec_bytes);
-        const InputRec* inp = kInputsLds ? s_in + (size_t)k * a.n_inputs
-                                         : a.xin + (size_t)k * a.n_inputs;
+        const InputRec* inp = a.xin + (size_t)k * a.n_inputs;
+        const double* xk = kInputsLds ? s_x + (size_t)k * a.n_inputs : nullptr;
         const double* tk = a.tab + (size_t)k * a.table_len;
         // 1-D dials; the 2-D dials (sorted last) follow in their own loop, so
         // that their register-hungry grid code stays out of this one
@@ -500,11 +522,25 @@ __global__ void __launch_bounds__(1024, 6) fill_kernel(FillArgs a) {
           const int n = (int)((d >> 20) & 0xff);
           const int dest = (int)((d >> 32) & 0xffff);
           const int off = (int)(d >> 48);
-          const InputRec ir = inp[in];
-          double r = eval_dial<R>(type, n, s_rec + off, ir.x, ir.flags, tk);
+          const int cl = (int)((d >> 28) & 0xf);
+          double x, lo, hi;
+          int flags;
+          if (kInputsLds && cl != kEscapeClass) {
+            x = xk[in];
+            lo = s_cls[cl].lo;
+            hi = s_cls[cl].hi;
+            flags = s_cls[cl].flags;
+          } else {
+            const InputRec ir = inp[in];
+            x = ir.x;
+            lo = ir.lo;
+            hi = ir.hi;
+            flags = ir.flags;
+          }
+          double r = eval_dial<R>(type, n, s_rec + off, x, flags, tk);
           // DialResponseSupervisor::process (DialResponseSupervisor.cpp:10-18)
-          if (r < ir.lo) r = ir.lo;
-          else if (r > ir.hi) r = ir.hi;
+          if (r < lo) r = lo;
+          else if (r > hi) r = hi;
           resp[dest] = r;
         }
         for (int d0 = grid_begin + (int)threadIdx.x; kGrid && d0 < h.n_dials;
@@ -884,6 +920,7 @@ struct gcm_context {
          *d_in_max = nullptr;
   double *d_theta = nullptr, *d_tab = nullptr;
   InputRec* d_xin = nullptr;
+  ClassRec* d_cls = nullptr;  // [kInputClasses]
   double *d_data = nullptr, *d_prior_mean = nullptr, *d_prior_sigma = nullptr;
   double *d_hist = nullptr, *d_hist2 = nullptr;
   double *d_llh_part = nullptr, *d_stat = nullptr, *d_pen = nullptr,
@@ -1045,6 +1082,31 @@ template <typename R>
 int build_layout(gcm_context* c, const gcm_tables* t) {
   const bool fp32 = sizeof(R) == 4;
   const int64_t N = t->n_events;
+  // input classes: the distinct (lo, hi, flags) of the response supervisors
+  // as the fill kernel applies them; inputs past the first kEscapeClass
+  // distinct ones keep kEscapeClass (full record read per dial)
+  std::vector<uint8_t> in_class((size_t)std::max(1, t->n_inputs), (uint8_t)kEscapeClass);
+  {
+    std::vector<ClassRec> classes(kInputClasses, ClassRec{0.0, 0.0, 0, 0, 0.0});
+    int n_cls = 0;
+    const double inf = __builtin_huge_val();
+    for (int i = 0; i < t->n_inputs; ++i) {
+      const gcm_input& g = t->inputs[i];
+      ClassRec r{std::isnan(g.min_response) ? -inf : g.min_response,
+                 std::isnan(g.max_response) ? inf : g.max_response, g.flags, 0, 0.0};
+      int found = -1;
+      for (int k = 0; k < n_cls && found < 0; ++k)
+        if (std::memcmp(&classes[k].lo, &r.lo, 8) == 0 && std::memcmp(&classes[k].hi, &r.hi, 8) == 0 &&
+            classes[k].flags == r.flags)
+          found = k;
+      if (found < 0 && n_cls < kEscapeClass) {
+        classes[n_cls] = r;
+        found = n_cls++;
+      }
+      if (found >= 0) in_class[i] = (uint8_t)found;
+    }
+    GCM_TRY(upload(c, &c->d_cls, classes.data(), classes.size()));
+  }
   // validate dials, record knot counts
   std::vector<uint8_t> knots(t->n_dials);
   for (int64_t d = 0; d < t->n_dials; ++d) {
@@ -1053,6 +1115,20 @@ int build_layout(gcm_context* c, const gcm_tables* t) {
     knots[d] = (uint8_t)n;
     if (t->dial_type[d] == GCM_DIAL_BILINEAR || t->dial_type[d] == GCM_DIAL_BICUBIC)
       c->has_grid = true;
+  }
+  // dial slots per tile (the LDS descriptor and response capacity): 12% above
+  // what a tile of mean-size dial records holds, a multiple of 128 (so the
+  // descriptor buffer is whole KiB: the producer wave's DMA writes whole KiB
+  // chunks) in [256, 1024]; a tile that reaches it is closed early
+  {
+    double bytes = 0;
+    for (int64_t d = 0; d < t->n_dials; ++d)
+      bytes += (double)record_words(t->dial_type[d], knots[d], fp32,
+                                    t->data_begin[d + 1] - t->data_begin[d]) * sizeof(R);
+    const double mean = t->n_dials ? std::max(bytes / (double)t->n_dials, 1.0) : 8.0;
+    const double want = 1.12 * ((double)c->launch.tile_words * 8.0) / mean;
+    const int64_t slots = ((int64_t)std::ceil(want) + 127) / 128 * 128;
+    c->max_dials = (int)std::min<int64_t>(1024, std::max<int64_t>(256, slots));
   }
   // stable counting sort of binned events by bin
   std::vector<int64_t> count(t->n_bins + 1, 0);
@@ -1261,6 +1337,7 @@ int build_layout(gcm_context* c, const gcm_tables* t) {
           const uint64_t in = (uint64_t)t->dial_input[d];
           const int code = device_type_code(type, n, p);
           desc[h.first_dial + q] = in | ((uint64_t)code << 16) | ((uint64_t)n << 20) |
+                                   ((uint64_t)in_class[in] << 28) |
                                    ((uint64_t)slot[idx[q]] << 32) | ((uint64_t)off << 48);
           off += record_words(type, n, fp32, t->data_begin[d + 1] - t->data_begin[d]);
         }
@@ -1412,11 +1489,6 @@ int gcm_build(gcm_context* c, const gcm_tables* t, int32_t max_points) {
   if (t->n_tables) c->tab_begin.assign(t->table_begin, t->table_begin + t->n_tables + 1);
   c->max_points = max_points;
   c->cap = std::isfinite(t->reweight_cap) ? t->reweight_cap : __builtin_huge_val();
-  // dial slots per tile: enough for the densest events (norm dials have no
-  // record words), bounded so that descriptors + responses stay small in LDS
-  // (a multiple of 128, so the descriptor buffer is whole KiB: the producer
-  // wave's DMA writes whole KiB chunks)
-  c->max_dials = std::min(1024, std::max(256, (c->launch.tile_words / 8 + 127) / 128 * 128));
   int rc = (c->precision == GCM_PRECISION_FP32) ? build_layout<float>(c, t)
                                                  : build_layout<double>(c, t);
   if (rc != GCM_OK) {
@@ -1828,6 +1900,7 @@ static int enqueue_fill(gcm_context* c, hipStream_t s, int K, bool copies,
   }
   const bool fp32 = c->precision == GCM_PRECISION_FP32;
   FillArgs a;
+  a.cls = c->d_cls;
   a.tiles = c->d_tiles;
   a.n_tiles = c->n_tiles;
   a.desc = c->d_desc;
@@ -1849,18 +1922,19 @@ static int enqueue_fill(gcm_context* c, hipStream_t s, int K, bool copies,
   a.flags = c->d_flags;
   a.rec_bytes = c->launch.tile_words * 8;
   a.max_dials = c->max_dials;
-  // The input records are staged in LDS when they are small and the staging
-  // still lets blocks_per_cu blocks share a CU; otherwise the evaluators read
-  // them from global memory (L1/L2) and the LDS goes to larger tiles.
-  const size_t in_bytes = (size_t)K * c->n_inputs * sizeof(InputRec);
+  // The inputs' x values are staged in LDS when they are small and the
+  // staging still lets blocks_per_cu blocks share a CU; otherwise the
+  // evaluators read them from global memory (L1/L2) and the LDS goes to
+  // larger tiles.  The class records always sit in LDS.
+  const size_t in_bytes = (size_t)K * c->n_inputs * sizeof(double);
   const size_t ring_lds =
       (size_t)c->launch.ring_buffers * ((size_t)a.rec_bytes + (size_t)a.max_dials * 8 +
                                         kMaxTileEvents * sizeof(EventRec) + kHeaderBytes) +
       2 * (size_t)a.max_dials * 8;
-  const size_t staged_pad = (in_bytes + 1023) / 1024 * 1024;
+  const size_t staged_pad = (in_bytes + kClassBytes + 1023) / 1024 * 1024;
   a.inputs_lds = in_bytes <= kInputsLdsMax &&
                  (staged_pad + ring_lds) * (size_t)c->launch.blocks_per_cu <= kLdsPerCU;
-  const size_t in_pad = a.inputs_lds ? staged_pad : 0;
+  const size_t in_pad = a.inputs_lds ? staged_pad : (kClassBytes + 1023) / 1024 * 1024;
   const size_t lds = in_pad + ring_lds;
   if (tev) GCM_HIP(hipEventRecord(tev[0], s));
   if (c->n_tiles > 0) {

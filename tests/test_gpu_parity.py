@@ -310,3 +310,39 @@ def test_batched_gradient_matches_single_points_and_oracle():
     assert np.all(np.abs(g - np.array(ref)) <= tol), (g, ref, tol)
     cm.close()
     orc.close()
+
+
+def test_many_clamp_classes_use_the_escape_path():
+    """More distinct response supervisors (lo, hi, flags) than the dial
+    descriptors' 15 input classes: the inputs past them read their full
+    record per dial.  Every input gets its own upper clamp here (and every
+    third one no extrapolation); bit-exact weights and 1e-10 sums against the
+    oracle on the feature workload, single point and batched."""
+    import dataclasses
+    print("escape classes workload=0", flush=True)
+    wl, t = _setup(0, 0)
+    inp = t.inputs.copy()
+    inp["max_response"] = 1.2 + 0.01 * np.arange(inp.shape[0])
+    inp["flags"][::3] |= 1
+    t2 = dataclasses.replace(t, inputs=inp, _keep=[t])
+    assert len({(a, b, c) for a, b, c in zip(inp["min_response"], inp["max_response"],
+                                             inp["flags"])}) > 15
+    orc = pyoracle.Oracle(t2, n_threads=8)
+    cm = CacheManager(0)
+    cm.build(t2, max_points=4)
+    cm.set_event_weight_copy(True)
+    pts = wl.random_points(4, 0.6, seed=13)
+    for theta in pts[:2]:
+        _check_point(wl, t2, cm, orc, theta)
+    tabs = wl.table_values(pts) if t2.table_len else None
+    cm.set_parameters(pts)
+    if tabs is not None:
+        cm.set_table_values(tabs)
+    cm.fill().get()
+    for k in range(4):
+        if tabs is not None:
+            orc.set_table_values(tabs[k])
+        _, s_ref, e_ref = orc.propagate(pts[k])
+        s, s2 = cm.hist(k)
+        np.testing.assert_allclose(s, s_ref, rtol=RTOL_SUM, atol=0)
+        np.testing.assert_allclose(np.sqrt(s2), e_ref, rtol=RTOL_SUM, atol=0)
