@@ -931,6 +931,10 @@ struct gcm_context {
   // pinned staging
   double* h_theta = nullptr;
   double* h_tab = nullptr;
+  // per-point results {total[K], stat[K], penalty[K], NaN flag}: one device
+  // block (d_res) copied to pinned memory (h_res) at the end of every fill
+  double* d_res = nullptr;
+  double* h_res = nullptr;
   bool tab_dirty = false, theta_dirty = false;
   hipEvent_t ev_staged = nullptr;  // staging buffers consumed by the device
   hipStream_t last_stream = nullptr;
@@ -1418,6 +1422,7 @@ int gcm_destroy(gcm_context* c) {
       if (e) (void)hipEventDestroy(e);
   if (c->h_theta) (void)hipHostFree(c->h_theta);
   if (c->h_tab) (void)hipHostFree(c->h_tab);
+  if (c->h_res) (void)hipHostFree(c->h_res);
   if (c->ev_staged) (void)hipEventDestroy(c->ev_staged);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
@@ -1532,12 +1537,9 @@ int gcm_build(gcm_context* c, const gcm_tables* t, int32_t max_points) {
   step(c->alloc(&c->d_hist, K * c->n_bins));
   step(c->alloc(&c->d_hist2, K * c->n_bins));
   step(c->alloc(&c->d_llh_part, K * c->fin_blocks));
-  step(c->alloc(&c->d_stat, K));
-  step(c->alloc(&c->d_pen, K));
-  step(c->alloc(&c->d_total, K));
+  step(c->alloc(&c->d_res, 3 * K + 1));
   step(c->alloc(&c->d_penalty, K));
   step(c->alloc(&c->d_tickets, K));
-  step(c->alloc(&c->d_flags, 1));
   step(c->alloc(&c->d_data, c->n_bins));
   step(c->alloc(&c->d_prior_mean, c->n_par));
   step(c->alloc(&c->d_prior_sigma, c->n_par));
@@ -1545,13 +1547,20 @@ int gcm_build(gcm_context* c, const gcm_tables* t, int32_t max_points) {
     c->free_all();
     return rc;
   }
+  c->d_total = c->d_res;
+  c->d_stat = c->d_res + K;
+  c->d_pen = c->d_res + 2 * K;
+  c->d_flags = reinterpret_cast<int*>(c->d_res + 3 * K);
+  GCM_HIP(hipMemset(c->d_res, 0, (3 * K + 1) * sizeof(double)));
   GCM_HIP(hipMemset(c->d_tickets, 0, K * sizeof(unsigned)));
   GCM_HIP(hipMemset(c->d_flags, 0, sizeof(int)));
   GCM_HIP(hipMemset(c->d_tab, 0, K * std::max<int64_t>(1, c->table_len) * sizeof(double)));
   GCM_HIP(hipMemset(c->d_theta, 0, K * c->n_par * sizeof(double)));
   if (hipHostMalloc((void**)&c->h_theta, K * c->n_par * sizeof(double)) != hipSuccess ||
-      hipHostMalloc((void**)&c->h_tab, K * std::max<int64_t>(1, c->table_len) * sizeof(double)) != hipSuccess)
+      hipHostMalloc((void**)&c->h_tab, K * std::max<int64_t>(1, c->table_len) * sizeof(double)) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_res, (3 * K + 1) * sizeof(double)) != hipSuccess)
     return fail(GCM_ERR_NOMEM, "pinned staging allocation failed");
+  std::memset(c->h_res, 0, (3 * K + 1) * sizeof(double));
   // default likelihood: no data histogram -> stat term 0 until set
   c->built = true;
   return GCM_OK;
@@ -1963,6 +1972,10 @@ static int enqueue_fill(gcm_context* c, hipStream_t s, int K, bool copies,
         c->d_llh_part, c->d_tickets, c->d_penalty, c->d_stat, c->d_pen, c->d_total);
   GCM_HIP(hipGetLastError());
   if (tev) GCM_HIP(hipEventRecord(tev[2], s));
+  // results and the NaN flag to pinned memory, so that fetching them after
+  // the stream's synchronisation is a host copy
+  GCM_HIP(hipMemcpyAsync(c->h_res, c->d_res, (3 * (size_t)c->max_points + 1) * sizeof(double),
+                         hipMemcpyDeviceToHost, s));
   return GCM_OK;
 }
 
@@ -2030,9 +2043,9 @@ int gcm_sync(gcm_context* c) {
   GCM_HIP(hipStreamSynchronize(c->last_stream ? c->last_stream : c->stream));
   if (c->timing) GCM_TRY(harvest_timing(c));
   if (c->built) {
-    int flag = 0;
-    GCM_HIP(hipMemcpy(&flag, c->d_flags, sizeof(int), hipMemcpyDeviceToHost));
-    if (flag) {
+    int* flag = reinterpret_cast<int*>(c->h_res + 3 * (size_t)c->max_points);
+    if (*flag) {
+      *flag = 0;
       GCM_HIP(hipMemset(c->d_flags, 0, sizeof(int)));
       return fail(GCM_ERR_NUMERIC, "NaN event weight");
     }
@@ -2072,9 +2085,10 @@ int gcm_get_llh(gcm_context* c, double* total, double* stat, double* penalty) {
   GCM_HIP(hipSetDevice(c->device));
   GCM_HIP(hipStreamSynchronize(c->last_stream ? c->last_stream : c->stream));
   const size_t K = (size_t)c->k_points;
-  if (total) GCM_HIP(hipMemcpy(total, c->d_total, K * sizeof(double), hipMemcpyDeviceToHost));
-  if (stat) GCM_HIP(hipMemcpy(stat, c->d_stat, K * sizeof(double), hipMemcpyDeviceToHost));
-  if (penalty) GCM_HIP(hipMemcpy(penalty, c->d_pen, K * sizeof(double), hipMemcpyDeviceToHost));
+  const size_t M = (size_t)c->max_points;
+  if (total) std::memcpy(total, c->h_res, K * sizeof(double));
+  if (stat) std::memcpy(stat, c->h_res + M, K * sizeof(double));
+  if (penalty) std::memcpy(penalty, c->h_res + 2 * M, K * sizeof(double));
   return GCM_OK;
 }
 

@@ -346,3 +346,36 @@ def test_many_clamp_classes_use_the_escape_path():
         s, s2 = cm.hist(k)
         np.testing.assert_allclose(s, s_ref, rtol=RTOL_SUM, atol=0)
         np.testing.assert_allclose(np.sqrt(s2), e_ref, rtol=RTOL_SUM, atol=0)
+
+
+def test_nan_weight_is_reported_once():
+    """A NaN event weight makes the next sync fail with GCM_ERR_NUMERIC
+    (the reference throws, CacheWeights.cpp:83-86); the flag is cleared, so
+    a later fill with finite parameters syncs cleanly.  Norm dials only: the
+    NaN parameter flows through no index computation."""
+    from gundam_amd import INPUT_DTYPE, Tables
+    from gundam_amd import _native as N
+    print("nan flag", flush=True)
+    n = 200
+    inputs = np.zeros(2, dtype=INPUT_DTYPE)
+    inputs["parameter"] = [0, 1]
+    inputs["mirror_lower"] = np.nan
+    inputs["mirror_range"] = np.nan
+    inputs["min_response"] = np.nan
+    inputs["max_response"] = np.nan
+    t = Tables(base_weight=np.ones(n), bin=(np.arange(n) % 4).astype(np.int32),
+               dial_begin=np.arange(n + 1, dtype=np.int64),
+               dial_type=np.full(n, N.DIAL_NORM, np.uint8),
+               dial_input=(np.arange(n) % 2).astype(np.int32),
+               data_begin=np.zeros(n + 1, np.int64), dial_data=np.zeros(0),
+               inputs=inputs, table_begin=np.zeros(1, np.int64), n_parameters=2, n_bins=4)
+    cm = CacheManager(0)
+    cm.build(t)
+    cm.set_parameters(np.array([np.nan, 1.0]))
+    with pytest.raises(N.GcmError) as e:
+        cm.fill().get()
+    assert e.value.code == N.GCM_ERR_NUMERIC
+    cm.set_parameters(np.array([2.0, 1.0]))
+    cm.fill().get()
+    np.testing.assert_array_equal(cm.hist(0)[0], [100.0, 50.0, 100.0, 50.0])
+    cm.close()
