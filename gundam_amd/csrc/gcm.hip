@@ -48,8 +48,8 @@ constexpr int kMaxTileEvents = 64;
 constexpr int kDefaultBlock = 256;
 constexpr int kDefaultTileWords = 2048;  // 16 KiB of records per tile
 constexpr int kDefaultBlocksPerCU = 3;
-constexpr int kFinalizeBlock = 256;
-constexpr int kFinalizeMaxBlocks = 128;
+constexpr int kFinalizeBlock = 1024;
+constexpr int kFinalizeMaxBlocks = 128;  // bounds the ticket atomics per point
 constexpr size_t kInputsLdsMax = 16384;  // stage input records in LDS up to this
 constexpr size_t kLdsPerCU = 160 * 1024;   // gfx950 LDS per compute unit
 constexpr size_t kHeaderBytes = 32;      // tile header copy in each LDS buffer
@@ -126,8 +126,28 @@ constexpr size_t kClassBytes = kInputClasses * sizeof(ClassRec);
 constexpr int kPenaltyChunk = 256;
 constexpr int kTableUpdateBlock = 128;  // threads per (job, point) table update
 
-// Per point k (one block each, 256 threads): mirrored input values and the
-// prior penalty.
+// The value input i hands its dials: the parameter, mirrored into
+// [lo, lo + range] when the input has mirror edges (DialInputBuffer.cpp:56-70).
+__device__ __forceinline__ double input_value(const double* th, const int32_t* in_par,
+                                              const double* in_mlo, const double* in_mrange,
+                                              int i) {
+  double v = th[in_par[i]];
+  const double lo = in_mlo[i];
+  if (!isnan(lo)) {
+    const double range = in_mrange[i];
+    v = fabs(fmod(v - lo, 2 * range));
+    if (v > range) {
+      v -= 2 * range;
+      v = -v;
+    }
+    v += lo;
+  }
+  return v;
+}
+
+// Per point k (one block each, 256 threads): mirrored input values, the
+// inputs of the device table-update jobs (tab_in[k][j] = value of input
+// tab_idx[j]) and the prior penalty.
 __global__ void prologue_kernel(const double* __restrict__ theta, int n_par,
                                 const int32_t* __restrict__ in_par,
                                 const double* __restrict__ in_mlo,
@@ -136,23 +156,17 @@ __global__ void prologue_kernel(const double* __restrict__ theta, int n_par,
                                 const double* __restrict__ in_max,
                                 const int32_t* __restrict__ in_flags,
                                 int n_inputs, InputRec* __restrict__ xin,
+                                const int32_t* __restrict__ tab_idx, int n_tab_idx,
+                                double* __restrict__ tab_in,
                                 const double* __restrict__ prior_mean,
                                 const double* __restrict__ prior_sigma,
                                 double* __restrict__ penalty) {
   const int k = blockIdx.x;
   const double* th = theta + (size_t)k * n_par;
+  for (int j = threadIdx.x; j < n_tab_idx; j += blockDim.x)
+    tab_in[(size_t)k * n_tab_idx + j] = input_value(th, in_par, in_mlo, in_mrange, tab_idx[j]);
   for (int i = threadIdx.x; i < n_inputs; i += blockDim.x) {
-    double v = th[in_par[i]];
-    const double lo = in_mlo[i];
-    if (!isnan(lo)) {  // DialInputBuffer.cpp:56-70
-      const double range = in_mrange[i];
-      v = fabs(fmod(v - lo, 2 * range));
-      if (v > range) {
-        v -= 2 * range;
-        v = -v;
-      }
-      v += lo;
-    }
+    const double v = input_value(th, in_par, in_mlo, in_mrange, i);
     InputRec r;
     r.x = v;
     r.lo = in_min[i];
@@ -246,15 +260,6 @@ __global__ void penalty_sets_kernel(const double* __restrict__ theta, int n_par,
   if (threadIdx.x == 0) penalty[k] = pen;
 }
 
-// The input values of every device table-update job: out[k][i] = the mirrored
-// value of input idx[i] at point k (DialInputBuffer semantics).
-__global__ void table_inputs_kernel(const InputRec* __restrict__ xin, int n_inputs,
-                                    const int32_t* __restrict__ idx, int n_idx,
-                                    double* __restrict__ out) {
-  const int k = blockIdx.x;
-  for (int i = threadIdx.x; i < n_idx; i += blockDim.x)
-    out[(size_t)k * n_idx + i] = xin[(size_t)k * n_inputs + idx[i]].x;
-}
 
 // Knuth two-sum: s + e == a + b exactly.
 __device__ __forceinline__ void two_sum(double a, double b, double& s,
@@ -1514,7 +1519,8 @@ int gcm_build(gcm_context* c, const gcm_tables* t, int32_t max_points) {
     mx[i] = std::isnan(in.max_response) ? inf : in.max_response;
   }
   const size_t K = (size_t)max_points;
-  // one wave per bin, grid-strided; few blocks keep the ticket count low
+  // one wave per bin, grid-strided over at most 128 blocks of 16 waves (a few
+  // bins per wave; the per-point ticket atomics stay few)
   c->fin_blocks = std::max(1, std::min(kFinalizeMaxBlocks,
                                        (c->n_bins + kFinalizeBlock / kWave - 1) / (kFinalizeBlock / kWave)));
   rc = GCM_OK;
@@ -1874,9 +1880,11 @@ static int enqueue_fill(gcm_context* c, hipStream_t s, int K, bool copies,
       GCM_HIP(hipMemcpyAsync(c->d_tab, c->h_tab, (size_t)K * c->table_len * sizeof(double),
                              hipMemcpyHostToDevice, s));
   }
+  const int n_tab_idx = c->tab_updates.empty() ? 0 : (int)c->tab_in_idx.size();
   prologue_kernel<<<K, 256, 0, s>>>(c->d_theta, c->n_par, c->d_in_par, c->d_in_mlo,
                                     c->d_in_mrange, c->d_in_min, c->d_in_max,
                                     c->d_in_flags, c->n_inputs, c->d_xin,
+                                    c->d_tab_in_idx, n_tab_idx, c->d_tab_in,
                                     c->llh_set ? c->d_prior_mean : nullptr,
                                     c->llh_set ? c->d_prior_sigma : nullptr, c->d_penalty);
   GCM_HIP(hipGetLastError());
@@ -1888,14 +1896,9 @@ static int enqueue_fill(gcm_context* c, hipStream_t s, int K, bool copies,
     GCM_HIP(hipGetLastError());
   }
   if (!c->tab_updates.empty()) {
-    // device table refresh: gather the jobs' mirrored inputs, then one launch
-    // of each update function over (job, point)
-    const int n_idx = (int)c->tab_in_idx.size();
-    if (n_idx > 0) {
-      table_inputs_kernel<<<K, 256, 0, s>>>(c->d_xin, c->n_inputs, c->d_tab_in_idx, n_idx,
-                                            c->d_tab_in);
-      GCM_HIP(hipGetLastError());
-    }
+    // device table refresh: the prologue gathered the jobs' mirrored inputs;
+    // one launch of each update function over (job, point)
+    const int n_idx = n_tab_idx;
     for (auto& u : c->tab_updates) {
       const void* jobs = u.d_jobs;
       double* tab = c->d_tab;
