@@ -30,19 +30,24 @@ configuration keys are the reference's; what is read:
           covarianceMatrixTMatrixD, parameterPriorTVectorD,
           parameterNameTObjArray (array names inside the .npz),
           parameterDefinitions: [{parameterName | parameterIndex, isEnabled,
-              isFixed, isFree, isPenaltyDisabled, priorValue, parameterLimits,
+              isFixed, isFree, priorType (Flat: free), isPenaltyDisabled,
+              priorValue, parameterLimits,
               dialSetDefinitions: [{dialType | dialsType | type, isEnabled,
                   dialLeafName | treeExpression, applyOnDataSets,
                   applyCondition, minDialResponse, maxDialResponse,
                   useMirrorDial, mirrorLowEdge, mirrorHighEdge,
-                  allowDialExtrapolation}]}]
+                  allowDialExtrapolation}]}],
+          dialSetDefinitions: [{... as above, dialInputList: [{name}, ...],
+                                dialSubType}]   (set level: one input, or two
+                                                 for dialType Surface)
     likelihoodInterfaceConfig:
       jointProbabilityConfig: {type, <llhConfig keys>}
       applyInfinitePenaltyOnOutOfBoundPar
 
 ``${NAME}`` in strings is replaced from the environment; CONFIG_DIR defaults
 to the configuration file's directory.  Dial types: Normalization (no leaf),
-Spline / Graph / Shift (the payload and its type come from the store leaf).
+Spline / Graph / Shift (the payload and its type come from the store leaf),
+Surface (a bilinear or bicubic store leaf, two inputs).
 """
 from __future__ import annotations
 
@@ -95,11 +100,12 @@ def _get(d: dict, *names, default=None):
 
 @dataclass
 class DialSet:
-    kind: str                    # "normalization" or "leaf"
+    kind: str                    # "normalization", "leaf" or "surface" (2-D leaf)
     leaf: Optional[str]
     datasets: Optional[List[str]]  # None: every dataset
     condition: Optional[str]
     input_index: int
+    input2_index: int = -1       # the y input of a surface
 
 
 @dataclass
@@ -233,6 +239,7 @@ class Propagator:
         self.parameter_sets: List[ParameterSet] = []
         self.parameters: List[Parameter] = []
         self._inputs = []
+        self.dial_sets: List[DialSet] = []  # collection order
         for ps in lst or []:
             if not ps.get("isEnabled", True):
                 continue
@@ -277,7 +284,8 @@ class Propagator:
             for p, dd in pairs:
                 p.enabled = bool(dd.get("isEnabled", True))
                 p.fixed = bool(dd.get("isFixed", False))
-                p.free = bool(dd.get("isFree", False))
+                # a flat prior makes the parameter free (Parameter.cpp:64)
+                p.free = bool(dd.get("isFree", False)) or dd.get("priorType") == "Flat"
                 p.penalty_disabled = bool(dd.get("isPenaltyDisabled", False))
                 if "priorValue" in dd:
                     p.prior = float(dd["priorValue"])
@@ -293,16 +301,44 @@ class Propagator:
                     if not dd.get("isEnabled", True):
                         continue
                     self._add_dial_set(p, dd)
+            # set-level dial sets name their inputs (DialCollection.cpp:78-100)
+            for dd in ps.get("dialSetDefinitions") or []:
+                if not dd.get("isEnabled", True):
+                    continue
+                names_in = [x.get("name") for x in (dd.get("dialInputList") or [])]
+                ins = []
+                for nm in names_in:
+                    hit = [q for q in pars if q.name == nm]
+                    if not hit:
+                        raise ConfigError(f"{ps['name']}: dialInputList names unknown parameter {nm}")
+                    ins.append(hit[0])
+                if len(ins) == 1:
+                    if ins[0].enabled:
+                        self._add_dial_set(ins[0], dd)
+                elif len(ins) == 2:
+                    if ins[0].enabled and ins[1].enabled:
+                        self._add_dial_set(ins[0], dd, ins[1])
+                else:
+                    raise ConfigError(f"{ps['name']}: a dial set needs one or two dialInputList entries")
             self.parameter_sets.append(ParameterSet(ps["name"], pars, cov))
         if not self.parameters:
             raise ConfigError("no enabled parameter")
 
-    def _add_dial_set(self, p: Parameter, dd: dict):
+    def _add_dial_set(self, p: Parameter, dd: dict, p2: Optional[Parameter] = None):
         t = str(_get(dd, "dialType", "type", "dialsType", default="")).lower()
         leaf = _get(dd, "dialLeafName", "treeExpression")
+        if (t == "surface") != (p2 is not None):
+            raise ConfigError(f"parameter {p.name}: Surface dials take exactly two inputs")
         if t in ("normalization", "norm"):
             kind = "normalization"
-        elif t in ("tabulated", "kriged", "surface"):
+        elif t == "surface":
+            if not leaf:
+                raise ConfigError(f"parameter {p.name}: Surface dials need dialLeafName")
+            sub = str(_get(dd, "dialSubType", "options", default="")).lower()
+            if sub and sub not in ("bilinear", "bicubic"):
+                raise ConfigError(f"parameter {p.name}: surface sub-type {sub} not supported")
+            kind = "surface"
+        elif t in ("tabulated", "kriged"):
             raise ConfigError(f"parameter {p.name}: dial type {t} is not read from "
                               f"configuration files (use the C ABI directly)")
         elif t in ("spline", "graph", "shift"):
@@ -324,11 +360,19 @@ class Propagator:
         mx = dd.get("maxDialResponse", None)
         self._inputs.append((p.index, flags, mir_lo, mir_r,
                              nan if mn is None else float(mn), nan if mx is None else float(mx)))
+        in1 = len(self._inputs) - 1
+        in2 = -1
+        if p2 is not None:
+            # the y input: the same mirror edges (they apply to every input of
+            # the collection); the response clamp is the x input's
+            self._inputs.append((p2.index, flags, mir_lo, mir_r, nan, nan))
+            in2 = len(self._inputs) - 1
         ds = dd.get("applyOnDataSets")
         if ds is not None and ("*" in ds or "all" in ds):
             ds = None
-        p.dial_sets.append(DialSet(kind, leaf, ds, _get(dd, "applyCondition", "applyConditions"),
-                                   len(self._inputs) - 1))
+        dset = DialSet(kind, leaf, ds, _get(dd, "applyCondition", "applyConditions"), in1, in2)
+        p.dial_sets.append(dset)
+        self.dial_sets.append(dset)
 
     # -- event and dial tables ---------------------------------------------
     def _bin(self, sample: Sample, store: EventStore, mask: np.ndarray) -> np.ndarray:
@@ -385,7 +429,7 @@ class Propagator:
             dial_data=cat("payload", np.float64),
             inputs=inputs, table_begin=np.zeros(1, np.int64),
             n_parameters=len(self.parameters), n_bins=self.n_bins,
-            reweight_cap=self.reweight_cap)
+            reweight_cap=self.reweight_cap, dial_input2=cat("dinput2", np.int32))
         self.n_events = int(self.tables.n_events)
 
     def _block(self, s: Sample, ds: dict, st: EventStore):
@@ -399,17 +443,22 @@ class Propagator:
         if ds["mc_weight"]:
             w = w * evaluate(ds["mc_weight"], st.column, n)
         # the dial sets that apply to this dataset, in collection order
-        sets = [d for p in self.parameters for d in p.dial_sets
-                if d.datasets is None or ds["name"] in d.datasets]
+        sets = [d for d in self.dial_sets if d.datasets is None or ds["name"] in d.datasets]
         on = np.zeros((keep.size, len(sets)), dtype=bool)
         types = np.zeros(len(sets), np.uint8)
         inp = np.zeros(len(sets), np.int32)
+        inp2 = np.full(len(sets), -1, np.int32)
         leaves = []
         for j, d in enumerate(sets):
             m_j = select(d.condition, st.column, n)[keep]
             inp[j] = d.input_index
-            if d.kind == "leaf":
+            inp2[j] = d.input2_index
+            if d.kind in ("leaf", "surface"):
                 t, beg, data = st.dial_leaf(d.leaf)
+                is2d = t in (N.DIAL_BILINEAR, N.DIAL_BICUBIC)
+                if is2d != (d.kind == "surface"):
+                    raise ConfigError(f"dial leaf {d.leaf}: a {d.kind} dial set needs "
+                                      f"{'a 2-D' if d.kind == 'surface' else 'a 1-D'} leaf")
                 beg = np.asarray(beg)
                 m_j &= (beg[keep + 1] - beg[keep]) > 0
                 types[j] = t
@@ -445,7 +494,7 @@ class Propagator:
                 np.asarray(lf[1])[np.repeat(starts[sel] - cx, L) + r]
         return {"keep": keep, "base": w[keep], "bin": s.bin_offset + b[keep],
                 "n_dials": np.bincount(ev, minlength=keep.size), "dtype": types[sj],
-                "dinput": inp[sj], "lens": lens, "payload": payload}
+                "dinput": inp[sj], "dinput2": inp2[sj], "lens": lens, "payload": payload}
 
     # -- parameters and evaluation ------------------------------------------
     def prior_values(self) -> np.ndarray:

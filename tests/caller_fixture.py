@@ -4,7 +4,9 @@ for one MC and one data sample, the reference's 200CovarianceFit binning file
 (tests/golden/binning/), a 4x4 covariance in an .npz, and a configuration with
 the reference's keys (modelled on its tests/fast-tests/200CovarianceFit-config.yaml:
 two normalisations and two splines under a covariance, plus a set without
-covariance holding a mirrored graph and a bounded normalisation)."""
+covariance holding a mirrored graph, a bounded normalisation and, as in its
+200MultiParFitBicubic-config.yaml, set-level Surface dials (bicubic and
+bilinear) on two flat-prior parameters named by dialInputList)."""
 import os
 import shutil
 
@@ -98,6 +100,26 @@ fitterEngineConfig:
             isEnabled: false
             dialSetDefinitions:
               - dialsType: Normalization
+          - parameterName: "Var_C"
+            priorValue: 0.0
+            priorType: Flat
+          - parameterName: "Var_D"
+            priorValue: 0.0
+            priorType: Flat
+        dialSetDefinitions:
+          - dialType: Surface
+            dialSubType: Bicubic
+            dialLeafName: "surf_CD"
+            dialInputList:
+              - name: "Var_C"
+              - name: "Var_D"
+          - dialType: Surface
+            dialSubType: Bilinear
+            dialLeafName: "lin_CD"
+            applyCondition: "[B] > 0"
+            dialInputList:
+              - name: "Var_C"
+              - name: "Var_D"
 """
 
 
@@ -125,6 +147,15 @@ def _graph(rng):
     return out
 
 
+def _grid(rng):
+    """A 2-D dial payload: nx, ny, x[nx], y[ny], v[nx*ny] (x-major)."""
+    nx, ny = int(rng.integers(2, 6)), int(rng.integers(2, 5))
+    x = np.sort(rng.uniform(-2.5, 2.5, nx)) + np.arange(nx) * 1e-3
+    y = np.sort(rng.uniform(-2.5, 2.5, ny)) + np.arange(ny) * 1e-3
+    v = 1 + 0.1 * rng.standard_normal(nx * ny)
+    return np.concatenate([[nx, ny], x, y, v])
+
+
 def write(tmp, n_mc=6000, n_data=4000, seed=3, llh="LeastSquares",
           llh_opts="lsqPoissonianApproximation: true", data_entry="TestData"):
     rng = np.random.default_rng(seed)
@@ -138,9 +169,12 @@ def write(tmp, n_mc=6000, n_data=4000, seed=3, llh="LeastSquares",
     spl_c = [(_monotonic(rng, 7) if c > 0 else np.zeros(0)) for c in C]
     spl_d = [(_general(rng) if c <= 0 else np.zeros(0)) for c in C]
     gra_e = [(_graph(rng) if i % 3 else np.zeros(0)) for i in range(n_mc)]
+    surf = [(_grid(rng) if i % 2 else np.zeros(0)) for i in range(n_mc)]
+    lin = [(_grid(rng) if i % 5 else np.zeros(0)) for i in range(n_mc)]
     write_store(os.path.join(tmp, "mc.store"), v, rng.uniform(0.5, 1.5, n_mc),
                 {"spline_C": ("monotonic", spl_c), "spline_D": ("general", spl_d),
-                 "graph_E": ("graph", gra_e)})
+                 "graph_E": ("graph", gra_e), "surf_CD": ("bicubic", surf),
+                 "lin_CD": ("bilinear", lin)})
     write_store(os.path.join(tmp, "data.store"), events(n_data), np.ones(n_data))
     m = rng.standard_normal((4, 4))
     cov = 0.05 * (m @ m.T) + 0.02 * np.eye(4)

@@ -99,7 +99,7 @@ def test_event_store_round_trip(tmp_path):
 def _expected_tables(prop):
     """Event-by-event restatement of the configuration semantics."""
     cfg_bins = [s.binning for s in prop.samples]
-    base, bins, types, inputs, payload, counts = [], [], [], [], [], []
+    base, bins, types, inputs, payload, counts, inputs2 = [], [], [], [], [], [], []
     for s, bs in zip(prop.samples, cfg_bins):
         for ds in prop.datasets:
             if s.datasets is not None and ds["name"] not in s.datasets:
@@ -126,25 +126,25 @@ def _expected_tables(prop):
                     base.append(st.weight[e] * wf[e])
                     bins.append(s.bin_offset + b)
                     c = 0
-                    for p in prop.parameters:
-                        for d in p.dial_sets:
-                            if d.datasets is not None and ds["name"] not in d.datasets:
+                    for d in prop.dial_sets:
+                        if d.datasets is not None and ds["name"] not in d.datasets:
+                            continue
+                        if not select(d.condition, col, n)[e]:
+                            continue
+                        if d.kind in ("leaf", "surface"):
+                            t, beg, data = st.dial_leaf(d.leaf)
+                            if beg[e + 1] == beg[e]:
                                 continue
-                            if not select(d.condition, col, n)[e]:
-                                continue
-                            if d.kind == "leaf":
-                                t, beg, data = st.dial_leaf(d.leaf)
-                                if beg[e + 1] == beg[e]:
-                                    continue
-                                payload.append(np.asarray(data[beg[e]:beg[e + 1]]))
-                            else:
-                                t = N.DIAL_NORM
-                                payload.append(np.zeros(0))
-                            types.append(t)
-                            inputs.append(d.input_index)
-                            c += 1
+                            payload.append(np.asarray(data[beg[e]:beg[e + 1]]))
+                        else:
+                            t = N.DIAL_NORM
+                            payload.append(np.zeros(0))
+                        types.append(t)
+                        inputs.append(d.input_index)
+                        inputs2.append(d.input2_index)
+                        c += 1
                     counts.append(c)
-    return base, bins, types, inputs, payload, counts
+    return base, bins, types, inputs, payload, counts, inputs2
 
 
 def test_config_assembly_matches_event_by_event_restatement(tmp_path):
@@ -154,21 +154,25 @@ def test_config_assembly_matches_event_by_event_restatement(tmp_path):
     prop = Propagator(cfg, str(tmp_path), build=False, binner=pyoracle.bin_events)
     t = prop.tables
     assert [p.name for p in prop.parameters] == ["norm_A", "norm_B", "spline_C", "spline_D",
-                                                 "graph_E", "flux", "off"]
-    np.testing.assert_array_equal(prop.prior_values(), [1, 1, 0, 0, 0, 1, 0])
+                                                 "graph_E", "flux", "off", "Var_C", "Var_D"]
+    np.testing.assert_array_equal(prop.prior_values(), [1, 1, 0, 0, 0, 1, 0, 0, 0])
+    assert [p.free for p in prop.parameters] == [False] * 7 + [True, True]
     assert [s.name for s in prop.samples] == ["AB", "AB_negC"] and prop.n_bins == 72
     assert prop.reweight_cap == 3.5 and prop.llh_type == "LeastSquares"
-    base, bins, types, inputs, payload, counts = _expected_tables(prop)
+    base, bins, types, inputs, payload, counts, inputs2 = _expected_tables(prop)
     np.testing.assert_array_equal(t.base_weight, base)
     np.testing.assert_array_equal(t.bin, bins)
     np.testing.assert_array_equal(np.diff(t.dial_begin), counts)
     np.testing.assert_array_equal(t.dial_type, types)
     np.testing.assert_array_equal(t.dial_input, inputs)
+    np.testing.assert_array_equal(t.dial_input2, inputs2)
+    assert {N.DIAL_BICUBIC, N.DIAL_BILINEAR} <= set(np.unique(t.dial_type).tolist())
     np.testing.assert_array_equal(np.diff(t.data_begin), [len(p) for p in payload])
     np.testing.assert_array_equal(t.dial_data, np.concatenate(payload))
     # one input per enabled dial set, with the configured supervisor / mirror
     inp = t.inputs
-    assert list(inp["parameter"]) == [0, 1, 2, 3, 4, 5]
+    # two surfaces on (Var_C, Var_D): an x and a y input each
+    assert list(inp["parameter"]) == [0, 1, 2, 3, 4, 5, 7, 8, 7, 8]
     assert inp["min_response"][0] == 0.0 and inp["max_response"][2] == 2.5
     assert inp["flags"][3] == N.INPUT_NO_EXTRAPOLATION and inp["flags"][2] == 0
     assert inp["mirror_lower"][4] == -1.5 and inp["mirror_range"][4] == 3.0
