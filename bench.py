@@ -248,8 +248,9 @@ def precision_c4(args, log):
     """FP32 dial mode against the FP64 path on the 20M-event workload (6
     samples of 60x50 bins, 500 parameters, 15 dials/event): max per-bin
     relative error of the bin sums and |dLLH| (Barlow-Beeston against the
-    FP64 Asimov histogram) at points drawn from N(0,1) around the prior
-    means, K points per launch; plus the plain fill kernel of each mode.
+    FP64 Asimov histogram) at points drawn from the priors (shape parameters
+    N(0,1), norms 1 + 0.1 N(0,1)), K points per launch; plus the plain fill
+    kernel of each mode.
     The FP64 path is the reference here: it matches the CPU oracle to 1e-10
     on the bin sums (tests/test_gpu_parity.py)."""
     from gundam_amd import CacheManager, Workload
@@ -263,7 +264,7 @@ def precision_c4(args, log):
     cms = {}
     shapes = {}
     from gundam_amd.autotune import autotune
-    tp = wl.random_points(1, 1.0, seed=args.seed + 5)
+    tp = wl.prior_points(1, seed=args.seed + 5)
     for prec in ("fp64", "fp32"):
         tuned = autotune(t, tp, wl.table_values(tp), precision=prec,
                          n_events=args.tune_events, reps=3)
@@ -281,7 +282,7 @@ def precision_c4(args, log):
         cm.set_likelihood("barlow", data, wl.prior_mean, wl.prior_sigma)
     fill = {}
     for prec, cm in cms.items():
-        p1 = wl.random_points(6, 1.0, seed=args.seed + 7)
+        p1 = wl.prior_points(6, seed=args.seed + 7)
         tb = wl.table_values(p1)
         cm.enable_timing(True)
         for i in range(6):
@@ -296,7 +297,7 @@ def precision_c4(args, log):
         fill[prec] = {"fill_kernel_ms": ms, "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS}
         log(f"c4: {prec} plain fill {ms:.3f} ms ({gbs:.0f} GB/s)")
     n = args.c4_points
-    pts = wl.random_points(n, 1.0, seed=args.seed + 4)
+    pts = wl.prior_points(n, seed=args.seed + 4)
     rel_max = np.zeros(n)
     rel2_max = np.zeros(n)
     dllh = np.zeros(n)

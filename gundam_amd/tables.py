@@ -212,6 +212,13 @@ class Workload:
         return self.prior_mean[None, :] + scale * rng.standard_normal(
             (k, self.n_parameters))
 
+    def prior_points(self, k: int, seed: int) -> np.ndarray:
+        """Parameter points drawn from the priors: prior mean + prior sigma *
+        N(0,1) (shape parameters N(0,1), norm parameters 1 + 0.1 N(0,1))."""
+        rng = np.random.default_rng(seed)
+        return self.prior_mean[None, :] + self.prior_sigma[None, :] * rng.standard_normal(
+            (k, self.n_parameters))
+
     def bin_events_on_device(self, device: int = 0) -> None:
         v = self.view
         N.check(N.lib().gcm_bin_events(device, v.n_events, v.n_vars,
