@@ -48,6 +48,16 @@ def read_traffic(path):
     return None
 
 
+def cpu_model() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(workload, n_sample, n_full, seed, threads, evals=20):
     from gundam_amd import Workload
     from oracle import pyoracle
@@ -71,16 +81,83 @@ def cpu_baseline(workload, n_sample, n_full, seed, threads, evals=20):
             times.append(dt)
     med = float(np.median(times))
     binned = int((t.bin >= 0).sum())
+    # the 100k-event model whole (the reference's CPU-runnable case)
+    w1 = Workload(1, seed=seed)
+    t1 = w1.tables
+    t1.bin[:] = pyoracle.bin_events(w1.vars, w1.edges)
+    o1 = pyoracle.Oracle(t1, n_threads=threads)
+    _, d1, _ = o1.propagate(w1.nominal(), want_weights=False)
+    times1 = []
+    for i, th in enumerate(w1.random_points(evals + 3, 0.5, seed=seed + 2)):
+        t0 = time.perf_counter()
+        _, s1, e1 = o1.propagate(th, want_weights=False)
+        pyoracle.llh("poisson", d1, s1, e1, th, w1.prior_mean, w1.prior_sigma)
+        if i >= 3:
+            times1.append(time.perf_counter() - t0)
+    o1.close()
     return {
+        "workload1_evals_per_s": 1.0 / float(np.median(times1)),
         "value": (1.0 / med) * (n_sample / n_full),
         "unit": "LLH evals/s (scaled to the full event count)",
         "cores": threads,
+        "cpu_model": cpu_model(),
         "kind": "port",
         "sample": (f"workload {workload} restricted to {n_sample} events "
                    f"({binned} binned), median of {len(times)} evaluations "
                    f"after 3 warm-ups = {med*1e3:.2f} ms, scaled linearly by "
                    f"{n_sample}/{n_full} events"),
     }
+
+
+def plain_workload1(args, log):
+    """The 100k-event model (20 parameters, 400 bins, 4 monotonic splines +
+    1 norm per event, Poisson LLH against the Asimov histogram) one point per
+    evaluation, as a fitter calls it: about 31 MB of algorithmic bytes, which
+    the 256 MB Infinity Cache holds, so launch latency dominates.  Reported:
+    evaluations/s eager and with the hipGraph replay, and the fill kernel's
+    time and algorithmic bandwidth."""
+    from gundam_amd import CacheManager, Workload
+
+    wl = Workload(1, seed=args.seed)
+    t = wl.tables
+    wl.bin_events_on_device(0)
+    cm = CacheManager(0)
+    cm.build(t)
+    cm.propagate_parameters(wl.nominal())
+    cm.set_likelihood("poisson", cm.hist(0)[0].copy(), wl.prior_mean, wl.prior_sigma)
+    pts = wl.random_points(200, 0.5, seed=args.seed + 31)
+    out = {"workload": "100k events, 20 parameters, 400 bins, 4 monotonic splines + 1 "
+                       "norm per event, Poisson LLH (Asimov)",
+           "algorithmic_bytes": int(t.algorithmic_bytes(8))}
+    for graph in (False, True):
+        cm.enable_graph(graph)
+        for i in range(10):
+            cm.set_parameters(pts[i])
+            cm.fill()
+            cm.llh()
+        t0 = time.perf_counter()
+        for i in range(10, 200):
+            cm.set_parameters(pts[i])
+            cm.fill()
+            cm.llh()
+        el = (time.perf_counter() - t0) / 190
+        out["evals_per_s_graph" if graph else "evals_per_s"] = 1.0 / el
+    cm.enable_graph(False)
+    cm.enable_timing(True)
+    for i in range(20):
+        cm.set_parameters(pts[i])
+        cm.fill()
+        cm.llh()
+    tm = cm.timing()
+    fill_ms = tm["fill_ms"] / max(1, tm["count"])
+    out["fill_kernel_ms"] = fill_ms
+    out["fill_achieved_GBs"] = (t.algorithmic_bytes(8) - 16 * t.n_bins) / (fill_ms * 1e-3) / 1e9
+    out["launch"] = cm.launch_config()
+    log(f"c1 (100k events, 1 point): {out['evals_per_s']:.0f} evals/s eager, "
+        f"{out['evals_per_s_graph']:.0f} with graph replay; fill {fill_ms*1e3:.1f} us "
+        f"({out['fill_achieved_GBs']:.0f} GB/s algorithmic)")
+    cm.close()
+    return out
 
 
 def batched_workload3(args, log):
@@ -437,7 +514,7 @@ def main():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-c3", action="store_true",
-                    help="skip the 100k-event K=64 batched measurement")
+                    help="skip the 100k-event measurements (K=64 batched, single point)")
     ap.add_argument("--tune-events", type=int, default=500000)
     ap.add_argument("--no-c5", action="store_true",
                     help="skip the surrogate delayed-acceptance MCMC leg")
@@ -579,6 +656,7 @@ def main():
         log(f"batched K={K}: {K/el:.1f} evals/s, {el*1e3:.2f} ms/batch")
 
     batched_c3 = None if args.no_c3 else batched_workload3(args, log)
+    plain_c1 = None if args.no_c3 else plain_workload1(args, log)
 
     c4 = precision_c4(args, log) if args.c4 else None
 
@@ -639,6 +717,7 @@ def main():
         },
         "batched": batched,
         "batched_c3": batched_c3,
+        "plain_c1": plain_c1,
         "surrogate_da": c5,
         "precision_c4": c4,
         "cpu_baseline": cpu,
