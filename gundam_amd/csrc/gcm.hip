@@ -48,8 +48,8 @@ constexpr int kMaxTileEvents = 64;
 constexpr int kDefaultBlock = 256;
 constexpr int kDefaultTileWords = 2048;  // 16 KiB of records per tile
 constexpr int kDefaultBlocksPerCU = 3;
-constexpr int kFinalizeBlock = 1024;
-constexpr int kFinalizeMaxBlocks = 128;  // bounds the ticket atomics per point
+constexpr int kFinalizeBlock = 256;
+constexpr int kFinalizeMaxBlocks = 8192;  // waves in flight for the per-bin pass
 constexpr size_t kInputsLdsMax = 16384;  // stage input records in LDS up to this
 constexpr size_t kLdsPerCU = 160 * 1024;   // gfx950 LDS per compute unit
 constexpr size_t kHeaderBytes = 32;      // tile header copy in each LDS buffer
@@ -726,30 +726,26 @@ __device__ __forceinline__ void wave_sum(double& s, double& c) {
   }
 }
 
-// One wave per bin: the lanes read the bin's partial slots coalesced, then a
-// fixed-shape wave reduction; the block's bins are summed in bin order, and
-// the last block of each point reduces the block results with a fixed tree.
+// One wave per (point, bin), grid-strided over all K x n_bins of them: the
+// lanes read the bin's partial slots coalesced, then a fixed-shape wave
+// reduction; lane 0 writes the bin's sums and its LLH term.
 template <bool kComp>
 __global__ void __launch_bounds__(kFinalizeBlock)
-    finalize_kernel(const double* __restrict__ psum,
-                    const double* __restrict__ psum2,
-                    const double* __restrict__ pcomp,
-                    const double* __restrict__ pcomp2, int64_t n_partials,
-                    const uint32_t* __restrict__ bin_pbeg, int n_bins,
-                    const double* __restrict__ data, LlhParams cfg,
-                    double* __restrict__ hist, double* __restrict__ hist2,
-                    double* llh_part, unsigned* tickets,
-                    const double* __restrict__ penalty, double* stat_out,
-                    double* pen_out, double* total_out) {
-  constexpr int kWavesPerBlock = kFinalizeBlock / kWave;
-  __shared__ double s_red[kFinalizeBlock];
-  __shared__ int s_last;
-  const int k = blockIdx.y;
-  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-  const size_t kp = (size_t)k * n_partials;
-  double wave_llh = 0.0;  // this wave's bins, summed in bin order
-  for (int b = blockIdx.x * kWavesPerBlock + wave; b < n_bins;
-       b += gridDim.x * kWavesPerBlock) {
+    finalize_bins_kernel(const double* __restrict__ psum,
+                         const double* __restrict__ psum2,
+                         const double* __restrict__ pcomp,
+                         const double* __restrict__ pcomp2, int64_t n_partials,
+                         const uint32_t* __restrict__ bin_pbeg, int n_bins, int k_points,
+                         const double* __restrict__ data, LlhParams cfg,
+                         double* __restrict__ hist, double* __restrict__ hist2,
+                         double* __restrict__ llh_terms) {
+  const int lane = threadIdx.x % kWave;
+  const int64_t n_waves = (int64_t)gridDim.x * (blockDim.x / kWave);
+  const int64_t items = (int64_t)k_points * n_bins;
+  for (int64_t it = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+       it < items; it += n_waves) {
+    const int k = (int)(it / n_bins), b = (int)(it % n_bins);
+    const size_t kp = (size_t)k * n_partials;
     double s = 0.0, s2 = 0.0, c = 0.0, c2 = 0.0;
     const uint32_t p0 = bin_pbeg[b], p1 = bin_pbeg[b + 1];
     for (uint32_t p = p0 + lane; p < p1; p += kWave) {
@@ -775,38 +771,33 @@ __global__ void __launch_bounds__(kFinalizeBlock)
     if (lane == 0) {
       hist[(size_t)k * n_bins + b] = s;
       hist2[(size_t)k * n_bins + b] = s2;
-      if (data) wave_llh += llh_bin(cfg, b, data[b], s, sqrt(s2));
+      llh_terms[(size_t)k * n_bins + b] = data ? llh_bin(cfg, b, data[b], s, sqrt(s2)) : 0.0;
     }
   }
-  if (lane == 0) s_red[wave] = wave_llh;
+}
+
+// One block per point: the per-bin LLH terms summed in a fixed order (each
+// thread a strided run in bin order, then a fixed tree), plus the penalty.
+__global__ void __launch_bounds__(kFinalizeBlock)
+    finalize_total_kernel(const double* __restrict__ llh_terms, int n_bins,
+                          const double* __restrict__ penalty, double* stat_out,
+                          double* pen_out, double* total_out) {
+  __shared__ double s_red[kFinalizeBlock];
+  const int k = blockIdx.x;
+  const double* lb = llh_terms + (size_t)k * n_bins;
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < n_bins; b += blockDim.x) acc += lb[b];
+  s_red[threadIdx.x] = acc;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double part = 0.0;
-    for (int w = 0; w < kWavesPerBlock; ++w) part += s_red[w];
-    llh_part[(size_t)k * gridDim.x + blockIdx.x] = part;
-    __threadfence();
-    const unsigned ticket = atomicAdd(&tickets[k], 1u);
-    s_last = (ticket == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (s_last) {
-    __threadfence();
-    const double* parts = llh_part + (size_t)k * gridDim.x;
-    double acc = 0.0;
-    for (unsigned j = threadIdx.x; j < gridDim.x; j += blockDim.x) acc += parts[j];
-    s_red[threadIdx.x] = acc;
+  for (int st = blockDim.x / 2; st > 0; st >>= 1) {
+    if ((int)threadIdx.x < st) s_red[threadIdx.x] += s_red[threadIdx.x + st];
     __syncthreads();
-    for (int st = blockDim.x / 2; st > 0; st >>= 1) {
-      if (threadIdx.x < st) s_red[threadIdx.x] += s_red[threadIdx.x + st];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-      const double stat = s_red[0];
-      stat_out[k] = stat;
-      pen_out[k] = penalty[k];
-      total_out[k] = stat + penalty[k];
-      tickets[k] = 0;  // ready for the next launch, no memset needed
-    }
+  }
+  if (threadIdx.x == 0) {
+    const double stat = s_red[0];
+    stat_out[k] = stat;
+    pen_out[k] = penalty[k];
+    total_out[k] = stat + penalty[k];
   }
 }
 
@@ -930,7 +921,6 @@ struct gcm_context {
   double *d_hist = nullptr, *d_hist2 = nullptr;
   double *d_llh_part = nullptr, *d_stat = nullptr, *d_pen = nullptr,
          *d_total = nullptr, *d_penalty = nullptr;
-  unsigned* d_tickets = nullptr;
   int* d_flags = nullptr;
   double* d_weights = nullptr;
   // pinned staging
@@ -1519,10 +1509,14 @@ int gcm_build(gcm_context* c, const gcm_tables* t, int32_t max_points) {
     mx[i] = std::isnan(in.max_response) ? inf : in.max_response;
   }
   const size_t K = (size_t)max_points;
-  // one wave per bin, grid-strided over at most 128 blocks of 16 waves (a few
-  // bins per wave; the per-point ticket atomics stay few)
-  c->fin_blocks = std::max(1, std::min(kFinalizeMaxBlocks,
-                                       (c->n_bins + kFinalizeBlock / kWave - 1) / (kFinalizeBlock / kWave)));
+  // the per-bin pass: one wave per (point, bin), grid-strided over at most
+  // kFinalizeMaxBlocks blocks (about what the device holds at once)
+  {
+    const int64_t items = (int64_t)max_points * c->n_bins;
+    const int64_t per_block = kFinalizeBlock / kWave;
+    c->fin_blocks = (int)std::max<int64_t>(1, std::min<int64_t>(kFinalizeMaxBlocks,
+                                                                (items + per_block - 1) / per_block));
+  }
   rc = GCM_OK;
   auto step = [&](int r) { if (rc == GCM_OK) rc = r; };
   step(upload(c, &c->d_in_par, par.data(), par.size()));
@@ -1542,10 +1536,9 @@ int gcm_build(gcm_context* c, const gcm_tables* t, int32_t max_points) {
   }
   step(c->alloc(&c->d_hist, K * c->n_bins));
   step(c->alloc(&c->d_hist2, K * c->n_bins));
-  step(c->alloc(&c->d_llh_part, K * c->fin_blocks));
+  step(c->alloc(&c->d_llh_part, K * c->n_bins));  // per-bin LLH terms
   step(c->alloc(&c->d_res, 3 * K + 1));
   step(c->alloc(&c->d_penalty, K));
-  step(c->alloc(&c->d_tickets, K));
   step(c->alloc(&c->d_data, c->n_bins));
   step(c->alloc(&c->d_prior_mean, c->n_par));
   step(c->alloc(&c->d_prior_sigma, c->n_par));
@@ -1558,7 +1551,6 @@ int gcm_build(gcm_context* c, const gcm_tables* t, int32_t max_points) {
   c->d_pen = c->d_res + 2 * K;
   c->d_flags = reinterpret_cast<int*>(c->d_res + 3 * K);
   GCM_HIP(hipMemset(c->d_res, 0, (3 * K + 1) * sizeof(double)));
-  GCM_HIP(hipMemset(c->d_tickets, 0, K * sizeof(unsigned)));
   GCM_HIP(hipMemset(c->d_flags, 0, sizeof(int)));
   GCM_HIP(hipMemset(c->d_tab, 0, K * std::max<int64_t>(1, c->table_len) * sizeof(double)));
   GCM_HIP(hipMemset(c->d_theta, 0, K * c->n_par * sizeof(double)));
@@ -1962,17 +1954,22 @@ static int enqueue_fill(gcm_context* c, hipStream_t s, int K, bool copies,
     GCM_HIP(hipGetLastError());
   }
   if (tev) GCM_HIP(hipEventRecord(tev[1], s));
-  dim3 fg(c->fin_blocks, K);
+  // the per-bin pass over K x n_bins items, then one block per point
+  const int fb = (int)std::max<int64_t>(1, std::min<int64_t>(
+      c->fin_blocks, ((int64_t)K * c->n_bins + kFinalizeBlock / kWave - 1) / (kFinalizeBlock / kWave)));
   if (fp32)
-    finalize_kernel<true><<<fg, kFinalizeBlock, 0, s>>>(
+    finalize_bins_kernel<true><<<fb, kFinalizeBlock, 0, s>>>(
         c->d_psum, c->d_psum2, c->d_pcomp, c->d_pcomp2, c->n_partials, c->d_bin_pbeg,
-        c->n_bins, c->llh_set ? c->d_data : nullptr, c->llh, c->d_hist, c->d_hist2,
-        c->d_llh_part, c->d_tickets, c->d_penalty, c->d_stat, c->d_pen, c->d_total);
+        c->n_bins, K, c->llh_set ? c->d_data : nullptr, c->llh, c->d_hist, c->d_hist2,
+        c->d_llh_part);
   else
-    finalize_kernel<false><<<fg, kFinalizeBlock, 0, s>>>(
+    finalize_bins_kernel<false><<<fb, kFinalizeBlock, 0, s>>>(
         c->d_psum, c->d_psum2, c->d_pcomp, c->d_pcomp2, c->n_partials, c->d_bin_pbeg,
-        c->n_bins, c->llh_set ? c->d_data : nullptr, c->llh, c->d_hist, c->d_hist2,
-        c->d_llh_part, c->d_tickets, c->d_penalty, c->d_stat, c->d_pen, c->d_total);
+        c->n_bins, K, c->llh_set ? c->d_data : nullptr, c->llh, c->d_hist, c->d_hist2,
+        c->d_llh_part);
+  GCM_HIP(hipGetLastError());
+  finalize_total_kernel<<<K, kFinalizeBlock, 0, s>>>(c->d_llh_part, c->n_bins, c->d_penalty,
+                                                     c->d_stat, c->d_pen, c->d_total);
   GCM_HIP(hipGetLastError());
   if (tev) GCM_HIP(hipEventRecord(tev[2], s));
   // results and the NaN flag to pinned memory, so that fetching them after
