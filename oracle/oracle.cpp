@@ -20,8 +20,19 @@
 //        ChiSquared::eval, LeastSquares::eval      ChiSquared.h:19-29, LeastSquares.h:45-52
 //        zero-prediction-at-prior bin mask         JointProbabilityBase.h:56-77
 //        LikelihoodInterface penalty + total       LikelihoodInterface.cpp:165-215
+//        per-set covariance penalty, bounds        LikelihoodInterface.cpp:186-213,
+//                                                  ParameterSet.cpp:441-463
 //   Event -> bin: LoaderUtils::fillBinIndex        LoaderUtils.cpp:32-40
 //                 Bin::isBetweenEdges              Bin.cpp:167-176
+//
+// How each part is pinned (the tests named): the dial math bit-exact against
+// golden vectors from the reference's own Calculate*.h headers compiled in
+// place (tests/test_oracle_golden.py); event assembly against an independent
+// Python restatement (tests/test_oracle_path.py); every LLH type against the
+// numerical profile minimum or its closed form and the reference unit test's
+// sweeps (tests/test_llh_variants.py); the penalty sets against exact
+// rational arithmetic and an independent solve (tests/test_penalty_oracle.py).
+// No function is left unpinned.
 //
 // Threading follows the reference partitioning: contiguous event ranges per
 // thread for the reweight, contiguous bin ranges per thread for the refill,
