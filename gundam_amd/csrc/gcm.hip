@@ -49,6 +49,7 @@ constexpr int kDefaultBlock = 256;
 constexpr int kDefaultTileWords = 2048;  // 16 KiB of records per tile
 constexpr int kDefaultBlocksPerCU = 3;
 constexpr int kFinalizeBlock = 256;
+constexpr int kTotalBlock = 1024;  // threads of the per-point LLH sum
 constexpr int kFinalizeMaxBlocks = 8192;  // waves in flight for the per-bin pass
 constexpr size_t kInputsLdsMax = 16384;  // stage input records in LDS up to this
 constexpr size_t kLdsPerCU = 160 * 1024;   // gfx950 LDS per compute unit
@@ -778,11 +779,11 @@ __global__ void __launch_bounds__(kFinalizeBlock)
 
 // One block per point: the per-bin LLH terms summed in a fixed order (each
 // thread a strided run in bin order, then a fixed tree), plus the penalty.
-__global__ void __launch_bounds__(kFinalizeBlock)
+__global__ void __launch_bounds__(kTotalBlock)
     finalize_total_kernel(const double* __restrict__ llh_terms, int n_bins,
                           const double* __restrict__ penalty, double* stat_out,
                           double* pen_out, double* total_out) {
-  __shared__ double s_red[kFinalizeBlock];
+  __shared__ double s_red[kTotalBlock];
   const int k = blockIdx.x;
   const double* lb = llh_terms + (size_t)k * n_bins;
   double acc = 0.0;
@@ -1968,7 +1969,7 @@ static int enqueue_fill(gcm_context* c, hipStream_t s, int K, bool copies,
         c->n_bins, K, c->llh_set ? c->d_data : nullptr, c->llh, c->d_hist, c->d_hist2,
         c->d_llh_part);
   GCM_HIP(hipGetLastError());
-  finalize_total_kernel<<<K, kFinalizeBlock, 0, s>>>(c->d_llh_part, c->n_bins, c->d_penalty,
+  finalize_total_kernel<<<K, kTotalBlock, 0, s>>>(c->d_llh_part, c->n_bins, c->d_penalty,
                                                      c->d_stat, c->d_pen, c->d_total);
   GCM_HIP(hipGetLastError());
   if (tev) GCM_HIP(hipEventRecord(tev[2], s));
