@@ -37,20 +37,21 @@ def mean_record_bytes(t: Tables, precision: str = "fp64") -> float:
     return max(float(w.sum()) * (4 if fp32 else 8) / n, 1.0) if w.size else 8.0
 
 
-def max_dials_for(tile_words: int, mean_bytes: float = 8.0) -> int:
-    """The library's dial slots per tile for a tile size (gcm.hip build_layout)."""
-    want = 1.12 * (tile_words * 8.0) / mean_bytes
-    return min(1024, max(256, (int(np.ceil(want)) + 127) // 128 * 128))
+def max_dials_for(tile_words: int, mean_bytes: float = 8.0, most: int = 0) -> int:
+    """The library's dial slots per tile for a tile size (gcm.hip build_layout);
+    most = the largest binned event's dial count."""
+    want = max(int(np.ceil(1.12 * (tile_words * 8.0) / mean_bytes)), int(most))
+    return min(1024, max(256, (want + 127) // 128 * 128))
 
 
 def lds_per_block(tile_words: int, n_inputs: int, k_points: int = 1,
                   ring_buffers: int = 2, blocks_per_cu: int = 1,
-                  mean_bytes: float = 8.0) -> int:
+                  mean_bytes: float = 8.0, most: int = 0) -> int:
     """LDS of one fill block (mirrors the host computation in gcm.hip: the
     inputs' x values are staged only when they are small and still let
     blocks_per_cu blocks share the CU; the 512 B of input-class records are
     always staged)."""
-    max_dials = max_dials_for(tile_words, mean_bytes)
+    max_dials = max_dials_for(tile_words, mean_bytes, most)
     ring = ring_buffers * (tile_words * 8 + max_dials * 8 + 64 * 16 + 32) + 2 * max_dials * 8
     in_bytes = k_points * n_inputs * 8
     classes = 16 * 32
@@ -75,14 +76,14 @@ def head(t: Tables, n_events: int) -> Tables:
 
 
 def candidates(n_inputs: int, precision: str = "fp64", k_points: int = 1,
-               mean_bytes: float = 8.0) -> list:
+               mean_bytes: float = 8.0, most: int = 0) -> list:
     out = []
     for nb, tile, block, bpc in itertools.product((2, 3), (1024, 1536, 2048, 2560, 3072, 3584,
                                                          3840, 4096),
                                                   (256, 384, 512, 768),
                                                   (1, 2, 3, 4, 5, 6)):
         lds = lds_per_block(tile, n_inputs, k_points, ring_buffers=nb, blocks_per_cu=bpc,
-                            mean_bytes=mean_bytes)
+                            mean_bytes=mean_bytes, most=most)
         # fill the CU's LDS: only the largest block count that fits
         if lds * bpc > LDS_BYTES_PER_CU or lds * (bpc + 1) <= LDS_BYTES_PER_CU:
             continue
@@ -102,8 +103,10 @@ def autotune(tables: Tables, theta: np.ndarray, table_values: Optional[np.ndarra
     theta = np.atleast_2d(theta)
     k_points = theta.shape[0]
     sub = head(tables, n_events)
+    binned = sub.bin >= 0
+    most = int(np.diff(sub.dial_begin)[binned].max()) if binned.any() else 0
     cands = list(cands) if cands is not None else candidates(
-        int(tables.inputs.shape[0]), precision, k_points, mean_record_bytes(sub, precision))
+        int(tables.inputs.shape[0]), precision, k_points, mean_record_bytes(sub, precision), most)
     results = []
     t_start = time.time()
     for c in cands:

@@ -1117,7 +1117,8 @@ int build_layout(gcm_context* c, const gcm_tables* t) {
       c->has_grid = true;
   }
   // dial slots per tile (the LDS descriptor and response capacity): 12% above
-  // what a tile of mean-size dial records holds, a multiple of 128 (so the
+  // what a tile of mean-size dial records holds and at least the largest
+  // binned event's dial count, a multiple of 128 (so the
   // descriptor buffer is whole KiB: the producer wave's DMA writes whole KiB
   // chunks) in [256, 1024]; a tile that reaches it is closed early
   {
@@ -1127,7 +1128,11 @@ int build_layout(gcm_context* c, const gcm_tables* t) {
                                     t->data_begin[d + 1] - t->data_begin[d]) * sizeof(R);
     const double mean = t->n_dials ? std::max(bytes / (double)t->n_dials, 1.0) : 8.0;
     const double want = 1.12 * ((double)c->launch.tile_words * 8.0) / mean;
-    const int64_t slots = ((int64_t)std::ceil(want) + 127) / 128 * 128;
+    // never below the largest event's dial count (an event is never split)
+    int64_t most = 0;
+    for (int64_t e = 0; e < N; ++e)
+      if (t->bin[e] >= 0) most = std::max<int64_t>(most, t->dial_begin[e + 1] - t->dial_begin[e]);
+    const int64_t slots = (std::max<int64_t>((int64_t)std::ceil(want), most) + 127) / 128 * 128;
     c->max_dials = (int)std::min<int64_t>(1024, std::max<int64_t>(256, slots));
   }
   // stable counting sort of binned events by bin
