@@ -99,8 +99,9 @@ typedef struct gcm_input {
   int32_t flags;        /* GCM_INPUT_*                                      */
   double mirror_lower;  /* NaN: no mirroring                                */
   double mirror_range;  /* mirror range (upper - lower)                     */
-  double min_response;  /* NaN: unset (collections default to 0)            */
-  double max_response;  /* NaN: unset                                       */
+  double min_response;  /* NaN: no lower clamp (the reference's collections
+                           default to 0, DialCollection.cpp:453: pass 0)   */
+  double max_response;  /* NaN: no upper clamp                              */
 } gcm_input;
 
 /* Dial tables in the reference's event-major EventDialCache order
@@ -313,7 +314,7 @@ int gcm_bin_events(int device, int64_t n_events, int32_t n_vars,
                    const double* values, int32_t n_bins, const double* edges,
                    int32_t* out_bin);
 
-/* ---- learned surrogate (no reference counterpart: the north star's MLP) --
+/* ---- learned surrogate (no reference counterpart: an MLP that predicts the bin contents) --
  * An MLP n_in -> n_hidden -> n_hidden -> n_out with GELU, BF16 weights and
  * FP32 accumulation on MFMA, trained with Adam on the mean squared error.
  * Parameter layout (FP32 master copy): W1[h][n_in] b1[h] W2[h][h] b2[h]

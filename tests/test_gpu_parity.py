@@ -3,7 +3,7 @@ identical seeded inputs (the reference CPU path: EventDialCache.cpp:156-171,
 Histogram.cpp:107-135, PoissonLogLikelihood.h, BarlowBeestonBanff2022.h).
 
 Bars: event -> bin indices bit-exact; per-event weights bit-exact in FP64
-mode (the device rounds every operation as the host does
+mode (the device rounds every operation as the host does; the required bar
 is 1e-12 relative); bin sums, sqrt(sum w^2) and LLH within 1e-10 relative.
 """
 import numpy as np
