@@ -240,8 +240,11 @@ def surrogate_da(args, cm, wl, data, err, log, device_tables=False):
     burn = run_batched_exact(ex, mode + sc * rng.standard_normal((C_, d)),
                              (2.38 ** 2 / d) * np.diag(sc ** 2), args.c5_burn, rng)
     extra = args.c5_pairs - sum(a.shape[0] for a in ex.rec_theta)
-    if extra > 0:
-        ex(mode + sc * rng.standard_normal((extra, d)))
+    for j0 in range(0, max(0, extra), 16384):  # same draws as one (extra, d) block
+        m = min(16384, extra - j0)
+        ex(mode + sc * rng.standard_normal((m, d)))
+        if extra > 16384:
+            log(f"c5: {j0 + m}/{extra} Laplace-draw training points")
     ex.record = False
     X, Y = ex.recorded()
     gen_s = time.perf_counter() - t0
@@ -250,7 +253,7 @@ def surrogate_da(args, cm, wl, data, err, log, device_tables=False):
         f"{burn.accepted / burn.proposals:.3f})")
     sur = Surrogate(d, wl.tables.n_bins, hidden=512, seed=args.seed + 2)
     t0 = time.perf_counter()
-    losses = sur.fit(X, Y, epochs=args.c5_epochs, batch=256, lr=1e-3)
+    losses = sur.fit(X, Y, epochs=args.c5_epochs, batch=256, lr=1e-3, log=log, log_every=50)
     train_s = time.perf_counter() - t0
     del X, Y
     ex.rec_theta, ex.rec_sums = [], []

@@ -134,10 +134,11 @@ class Surrogate:
 
     def fit(self, theta: np.ndarray, sums: np.ndarray, epochs: int = 30,
             batch: int = 256, lr: float = 1e-3, lr_final: Optional[float] = None,
-            seed: int = 0) -> np.ndarray:
+            seed: int = 0, log=None, log_every: int = 25) -> np.ndarray:
         """Adam epochs on (theta, bin contents) pairs; the learning rate decays
         geometrically from lr to lr_final (default lr / 30).  Returns the
-        sampled training loss of each epoch."""
+        sampled training loss of each epoch; log(message), when given, gets a
+        progress line every log_every epochs."""
         theta = np.atleast_2d(np.asarray(theta, np.float64))
         self.x_mu = theta.mean(0)
         sd = theta.std(0)
@@ -149,6 +150,8 @@ class Surrogate:
         for e in range(epochs):
             le = lr * (lr_final / lr) ** (e / max(1, epochs - 1))
             losses[e] = self.mlp.train(1, batch, le, seed * 100003 + e)[0]
+            if log is not None and (e + 1) % max(1, log_every) == 0:
+                log(f"surrogate epoch {e + 1}/{epochs}: loss {losses[e]:.4f}")
         self.head_ready = False
         return losses
 
