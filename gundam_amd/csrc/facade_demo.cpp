@@ -88,6 +88,35 @@ int main() {
     std::printf("batched gradient differs from single-point evaluations\n");
     return 1;
   }
+  // a correlated prior on parameters 0 and 1 (LikelihoodInterface's per-set
+  // penalty): the device's penalty must grow by exactly d^T (C^-1 d), formed
+  // in the same order on the host
+  {
+    const int32_t pars[2] = {0, 1};
+    const double prior[2] = {nominal[0], nominal[1]};
+    const double inv[4] = {2.0, -0.5, -0.5, 1.5};
+    gcm_penalty_set ps{2, pars, prior, inv, nullptr, nullptr};
+    cm.Fill(th0).get();
+    const double pen0 = cm.Penalty()[0];
+    cm.SetPenaltySets({ps});
+    cm.Fill(th0).get();
+    const double pen1 = cm.Penalty()[0];
+    const double d0 = th0[0] - prior[0], d1 = th0[1] - prior[1];
+    double y0 = 0, y1 = 0;
+    y0 += d0 * inv[0];
+    y0 += d1 * inv[1];
+    y1 += d0 * inv[2];
+    y1 += d1 * inv[3];
+    double q = 0;
+    q += d0 * (0.0 + y0);
+    q += d1 * (0.0 + y1);
+    std::printf("penalty %.10g -> %.10g with the correlated set (expected +%.10g)\n", pen0,
+                pen1, q);
+    if (pen1 != pen0 + q) {
+      std::printf("correlated penalty differs\n");
+      return 1;
+    }
+  }
   std::printf("resident %.1f MiB, nominal LLH check %.3g\n",
               cm.GetResidentMemory() / 1048576.0, total);
   gcm_synth_destroy(wl);

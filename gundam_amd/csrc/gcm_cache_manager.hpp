@@ -84,6 +84,13 @@ class CacheManager {
           "gcm_set_likelihood");
   }
 
+  // The prior penalty of each of the last k_points points.
+  std::vector<double> Penalty(int k_points = 1) {
+    std::vector<double> out(k_points);
+    check(gcm_get_llh(ctx_, nullptr, nullptr, out.data()), "gcm_get_llh");
+    return out;
+  }
+
   // Correlated prior penalties per parameter set (LikelihoodInterface.cpp:
   // 186-213), added after the diagonal term in set order.
   void SetPenaltySets(const std::vector<gcm_penalty_set>& sets) {
