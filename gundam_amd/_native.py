@@ -148,6 +148,7 @@ GCM_SIGNATURES = {
     "gcm_sync": (C.c_int, [_P]),
     "gcm_enable_graph": (C.c_int, [_P, C.c_int]),
     "gcm_get_hist": (C.c_int, [_P, _I32, _P, _P]),
+    "gcm_validate_hist": (C.c_int, [_P, _I32, _P, _P, _D, C.c_int, C.POINTER(_I32)]),
     "gcm_get_weights": (C.c_int, [_P, _P]),
     "gcm_get_llh": (C.c_int, [_P, _P, _P, _P]),
     "gcm_enable_timing": (C.c_int, [_P, C.c_int]),

@@ -259,6 +259,23 @@ class CacheManager:
         s, s2 = self.hist(k)
         return s, np.sqrt(s2)
 
+    def validate_histogram_contents(self, sums: np.ndarray, sqrt_sum_w2: np.ndarray,
+                                    k: int = 0, fraction_of_bits_lost: float = 0.3,
+                                    verbose: bool = False) -> bool:
+        """Cache::Manager::ValidateHistogramContents: the device's bin sums and
+        sqrt(sum w^2) of point k against CPU-filled ones (GundamUtils::almostEqual
+        rule); True when every bin matches."""
+        s = np.ascontiguousarray(sums, dtype=np.float64)
+        e = np.ascontiguousarray(sqrt_sum_w2, dtype=np.float64)
+        if s.shape != (self.n_bins,) or e.shape != (self.n_bins,):
+            raise ValueError("per-bin array has the wrong size")
+        bad = C.c_int32()
+        N.check(self._lib.gcm_validate_hist(self._h, int(k), s.ctypes.data, e.ctypes.data,
+                                            float(fraction_of_bits_lost), int(bool(verbose)),
+                                            C.byref(bad)))
+        self.last_validation_failures = bad.value
+        return bad.value == 0
+
     def copy_event_weights(self, out: Optional[np.ndarray] = None):
         if out is None:
             out = np.full(self.n_events, np.nan)

@@ -31,6 +31,7 @@
 #include <climits>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <limits>
 #include <string>
@@ -2068,6 +2069,45 @@ int gcm_get_hist(gcm_context* c, int32_t k, double* sums, double* sums2) {
   GCM_HIP(hipStreamSynchronize(s));
   if (sums) GCM_HIP(hipMemcpy(sums, c->d_hist + (size_t)k * c->n_bins, c->n_bins * sizeof(double), hipMemcpyDeviceToHost));
   if (sums2) GCM_HIP(hipMemcpy(sums2, c->d_hist2 + (size_t)k * c->n_bins, c->n_bins * sizeof(double), hipMemcpyDeviceToHost));
+  return GCM_OK;
+}
+
+namespace {
+// GundamUtils::almostEqual (GundamAlmostEqual.h:24-58) for two doubles.
+bool almost_equal(double a, double b, double fraction_of_bits_lost) {
+  if (!std::isfinite(a) || !std::isfinite(b)) return false;
+  double eps = std::numeric_limits<double>::epsilon();
+  double eps_a = eps, eps_b = eps;
+  if (fraction_of_bits_lost > 0.0) {
+    fraction_of_bits_lost = std::min(fraction_of_bits_lost, 0.99);
+    eps = std::pow(eps, 1.0 - fraction_of_bits_lost);
+  }
+  if (std::isnormal(a * eps)) eps_a = std::abs(a * eps);
+  if (std::isnormal(b * eps)) eps_b = std::abs(b * eps);
+  return std::abs(a - b) < std::max(eps_a, eps_b);
+}
+}  // namespace
+
+int gcm_validate_hist(gcm_context* c, int32_t k, const double* sums,
+                      const double* sqrt_sum_w2, double fraction_of_bits_lost, int verbose,
+                      int32_t* n_bad) {
+  if (!c || !sums || !sqrt_sum_w2 || !n_bad) return fail(GCM_ERR_ARG, "NULL argument");
+  if (!c->built) return fail(GCM_ERR_STATE, "gcm_build first");
+  if (k < 0 || k >= std::max(1, c->k_points)) return fail(GCM_ERR_ARG, "point out of range");
+  std::vector<double> s((size_t)c->n_bins), s2((size_t)c->n_bins);
+  GCM_TRY(gcm_get_hist(c, k, s.data(), s2.data()));
+  int32_t bad = 0;
+  for (int b = 0; b < c->n_bins; ++b) {
+    const double err = std::sqrt(s2[b]);
+    if (almost_equal(sums[b], s[b], fraction_of_bits_lost) &&
+        almost_equal(sqrt_sum_w2[b], err, fraction_of_bits_lost))
+      continue;
+    ++bad;
+    if (verbose)
+      std::fprintf(stderr, "gcm: invalid bin %d -- GPU: %.17g +/- %.17g CPU: %.17g +/- %.17g\n", b,
+                   s[b], err, sums[b], sqrt_sum_w2[b]);
+  }
+  *n_bad = bad;
   return GCM_OK;
 }
 

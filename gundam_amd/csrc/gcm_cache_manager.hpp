@@ -148,6 +148,18 @@ class CacheManager {
     return true;
   }
 
+  // Cache::Manager::ValidateHistogramContents against CPU-filled histograms
+  // (sumWeights, sqrtSumSqWeights per global bin).
+  bool ValidateHistogramContents(const std::vector<double>& cpu_sums,
+                                 const std::vector<double>& cpu_sqrt_sum_w2, bool verbose = false,
+                                 int k = 0) {
+    int32_t bad = 0;
+    check(gcm_validate_hist(ctx_, k, cpu_sums.data(), cpu_sqrt_sum_w2.data(), 0.3,
+                            verbose ? 1 : 0, &bad),
+          "gcm_validate_hist");
+    return bad == 0;
+  }
+
   std::vector<double> Likelihood(int k_points = 1) {
     std::vector<double> tot(k_points);
     check(gcm_get_llh(ctx_, tot.data(), nullptr, nullptr), "gcm_get_llh");

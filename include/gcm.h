@@ -289,6 +289,17 @@ int gcm_enable_graph(gcm_context* ctx, int enable);
  *      CacheManager.cpp:832-866) ------------------------------------------- */
 /* Sum of weights and sum of squared weights per bin for point k. */
 int gcm_get_hist(gcm_context* ctx, int32_t k, double* sums, double* sums2);
+/* Replaces Cache::Manager::ValidateHistogramContents (CacheManager.cpp:868-887,
+ * CacheSampleHistFiller.cpp:21-50): compares point k's bin sums and
+ * sqrt(sum w^2) with the caller's (CPU-filled) ones, [n_bins] each, with the
+ * reference's GundamUtils::almostEqual rule (GundamAlmostEqual.h:24-58: a
+ * relative tolerance eps^(1 - fraction_of_bits_lost), eps the FP64 epsilon;
+ * non-finite values never match; the reference's default fraction is 0.3).
+ * *n_bad receives the number of mismatching bins; verbose != 0 prints each
+ * to stderr. */
+int gcm_validate_hist(gcm_context* ctx, int32_t k, const double* sums,
+                      const double* sqrt_sum_w2, double fraction_of_bits_lost,
+                      int verbose, int32_t* n_bad);
 /* Per-event weights of point 0 in the caller's event order; unbinned events
  * are left untouched. */
 int gcm_get_weights(gcm_context* ctx, double* weights);

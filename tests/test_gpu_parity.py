@@ -379,3 +379,28 @@ def test_nan_weight_is_reported_once():
     cm.fill().get()
     np.testing.assert_array_equal(cm.hist(0)[0], [100.0, 50.0, 100.0, 50.0])
     cm.close()
+
+
+def test_validate_histogram_contents():
+    """Cache::Manager::ValidateHistogramContents: the device histogram matches
+    the oracle's (CPU-filled) one under the reference's almostEqual rule, and
+    a bin moved by one part in 1e9 is reported."""
+    print("validate histograms workload=3", flush=True)
+    wl, t = _setup(3, 0)
+    orc = pyoracle.Oracle(t, n_threads=8)
+    cm = CacheManager(0)
+    cm.build(t)
+    theta = wl.random_points(1, 0.5, seed=19)[0]
+    _, s_ref, e_ref = orc.propagate(theta, want_weights=False)
+    cm.propagate_parameters(theta)
+    assert cm.validate_histogram_contents(s_ref, e_ref)
+    bad = s_ref.copy()
+    j = int(np.argmax(bad))
+    bad[j] *= 1 + 1e-9
+    assert not cm.validate_histogram_contents(bad, e_ref, verbose=True)
+    assert cm.last_validation_failures == 1
+    err = e_ref.copy()
+    err[0] = np.nan
+    assert not cm.validate_histogram_contents(s_ref, err)
+    cm.close()
+    orc.close()
