@@ -30,6 +30,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA, published
 
 
 def log(msg):
@@ -424,6 +425,36 @@ def precision_c4(args, log):
     }
 
 
+def mlp_kernels(args, log, n_in=300, hidden=512, n_out=6000, rows=16384, batch=256,
+                epochs=3):
+    """The surrogate network's training kernels on their own: n_in -> 512 ->
+    512 -> n_out (the 5M-event model's 300 parameters and 6000 bins), Adam
+    minibatches of 256 over `rows` synthetic pairs resident on the device.
+    FLOPs per row and step: 2 (3 (n_in h + h^2 + h n_out) - n_in h) (forward,
+    weight gradients, and input gradients of the two upper layers), against
+    the dense BF16 MFMA peak."""
+    from gundam_amd.surrogate import MLP
+    rng = np.random.default_rng(args.seed + 97)
+    x = rng.standard_normal((rows, n_in)).astype(np.float32)
+    tgt = rng.standard_normal((rows, n_out)).astype(np.float32)
+    m = MLP(n_in, hidden, n_out, seed=args.seed + 5)
+    m.set_data(x, tgt)
+    m.train(1, batch, 1e-3, 0)  # warm-up epoch
+    t0 = time.perf_counter()
+    m.train(epochs, batch, 1e-3, 1)
+    el = time.perf_counter() - t0
+    m.close()
+    steps = epochs * (rows // batch)
+    flop = 2.0 * (3 * (n_in * hidden + hidden * hidden + hidden * n_out) - n_in * hidden)
+    tflops = flop * batch * steps / el / 1e12
+    log(f"surrogate MLP training: {el / steps * 1e6:.1f} us per step of {batch} rows, "
+        f"{tflops:.1f} TFLOP/s ({tflops / MFMA_BF16_PEAK_TFLOPS:.2%} of the BF16 MFMA peak)")
+    return {"network": f"{n_in}-{hidden}-{hidden}-{n_out}", "batch": batch,
+            "steps": steps, "us_per_step": el / steps * 1e6, "tflops": tflops,
+            "peak_tflops": MFMA_BF16_PEAK_TFLOPS, "frac": tflops / MFMA_BF16_PEAK_TFLOPS,
+            "bound": "latency (about 20 kernels per 5.3-GFLOP step)"}
+
+
 def posterior_agreement(args, log):
     """Posterior means of the delayed-acceptance chains (surrogate sub-chains
     of args.c5_sub steps) against batched exact chains on the 100k-event
@@ -537,6 +568,8 @@ def main():
     ap.add_argument("--only-posterior", action="store_true",
                     help="run only the posterior-agreement study (100k-event model)")
     ap.add_argument("--post-seconds", type=float, default=60.0)
+    ap.add_argument("--only-mlp", action="store_true",
+                    help="run only the surrogate network's training-kernel timing")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -552,6 +585,9 @@ def main():
         return
     if args.only_posterior:
         print(json.dumps({"posterior_agreement": posterior_agreement(args, log)}), flush=True)
+        return
+    if args.only_mlp:
+        print(json.dumps({"surrogate_mlp": mlp_kernels(args, log)}), flush=True)
         return
 
     t0 = time.time()
@@ -666,6 +702,7 @@ def main():
     c5 = None
     if not args.no_c5 and args.batch >= 32:
         c5 = surrogate_da(args, cm, wl, data, data_err, log, device_tables)
+    mlp = None if args.no_c5 else mlp_kernels(args, log)
 
     cpu = None
     if not args.no_cpu:
@@ -722,6 +759,7 @@ def main():
         "batched_c3": batched_c3,
         "plain_c1": plain_c1,
         "surrogate_da": c5,
+        "surrogate_mlp": mlp,
         "precision_c4": c4,
         "cpu_baseline": cpu,
     }

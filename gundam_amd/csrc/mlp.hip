@@ -2,12 +2,13 @@
 // that maps a parameter point to all predicted log bin contents, trained on
 // the GPU with hand-written BF16 MFMA kernels (FP32 accumulation) and Adam.
 //
-// GEMM: C[M x N] = op(A)[M x K] . op(B)[K x N] (+ bias, optional GELU), with
-// A and B given by element strides, converted to BF16 while staged in LDS and
-// multiplied with v_mfma_f32_32x32x16_bf16 (gfx950).  Lane layout of that
-// instruction (checked on the device): lane l holds A[l%32][8(l/32)+j] and
-// B[8(l/32)+j][l%32] (j < 8); accumulator r of lane l is
-// C[8(r/4) + 4(l/32) + r%4][l%32].
+// GEMM: C[M x N] = A[M x K] . B[K x N] (+ bias, optional GELU), each operand
+// stored either with K contiguous or with M (N) contiguous, read with 16-byte
+// loads along its contiguous dimension, converted to BF16 and staged in LDS as
+// [row][k] tiles, and multiplied with v_mfma_f32_32x32x16_bf16 (gfx950).
+// Lane layout of that instruction (checked on the device): lane l holds
+// A[l%32][8(l/32)+j] and B[8(l/32)+j][l%32] (j < 8); accumulator r of lane l
+// is C[8(r/4) + 4(l/32) + r%4][l%32].
 //
 // Parameters, flat FP32 master copy: W1[H][n_in] b1[H] W2[H][H] b2[H]
 // W3[n_out][H] b3[n_out]; the GEMMs read BF16 copies of the weights.
@@ -19,6 +20,7 @@
 #include <cstring>
 #include <random>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/gcm.h"
@@ -41,30 +43,145 @@ using gcm::fail;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int BM = 64, BN = 64, BK = 32;
-constexpr int LDSK = BK + 8;  // padded row (bf16 elements) against bank conflicts
+// Block tile 64 x 64 with a K step of 64; each of the four waves owns a
+// 32 x 32 sub-tile (one accumulator).  Operand tiles are double-buffered in
+// LDS as [row][k] BF16 with a padded row of 72 elements (144 B, 16-B aligned
+// for the 128-bit operand reads); the next K step is loaded into registers
+// while the current one is multiplied, one barrier per step.
+constexpr int TM = 64, TN = 64, TK = 64;
+constexpr int LDK = TK + 8;
+constexpr int kMaxSplit = 32;
+
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
 struct GemmArgs {
   int M, N, K;
-  const void* A;  // element (m, k) at A[m * a_rs + k * a_cs]
-  int64_t a_rs, a_cs;
-  int a_bf16;
-  const void* B;  // element (k, n) at B[k * b_rs + n * b_cs]
-  int64_t b_rs, b_cs;
-  int b_bf16;
-  float* C;  // C[m * ldc + n]
-  int64_t ldc;
+  // element (m, k) of A: A[m * lda + k] when A is K-contiguous, else A[k * lda + m]
+  const void* A;
+  int64_t lda;
+  // element (k, n) of B: B[n * ldb + k] when B is K-contiguous, else B[k * ldb + n]
+  const void* B;
+  int64_t ldb;
+  int vec;   // both operands meet the alignment of the unchecked vector loads
+  float* C;  // C[m * ldc + n]; split z of a split-K launch writes C + z * split_stride
+  int64_t ldc, split_stride;
+  int k_chunk;        // K range of one split, a multiple of TK
   const float* bias;  // [N] or null
   float* Z;           // pre-activation copy (GELU only) or null
   int gelu;
 };
 
-__device__ __forceinline__ float load_elem(const void* p, int64_t i, int is_bf16) {
-  return is_bf16 ? (float)reinterpret_cast<const __bf16*>(p)[i]
-                 : reinterpret_cast<const float*>(p)[i];
-}
+// One 64 (rows) x 64 (k) operand tile held in registers between its global
+// load and its LDS store, for an operand that is K-contiguous (KC) or
+// row-contiguous, FP32 or BF16 (BF).  load() is unchecked (interior tile,
+// aligned operand); load_checked() reads rows >= R and k >= kend as zero.
+//  KC: thread t holds rows t/8 and t/8 + 32, k = 8 (t%8) .. + 7, read as
+//      16-byte vectors along k, stored as one 16-byte LDS write per row.
+//  row-contiguous: thread t holds rows 2 (t%32) and + 1, k = 8 (t/32) .. + 7,
+//      read as 2-element vectors along the rows (one per k), stored
+//      transposed as one 16-byte LDS write per row.
+template <bool KC, bool BF>
+struct Stage {
+  using E = typename std::conditional<BF, __bf16, float>::type;
+  // raw loaded values: converted to BF16 (FP32 operands) only when stored,
+  // so that the loads stay in flight across the multiply of the previous step
+  float f[2][8];   // FP32: [row][k]
+  bf16x8 hk[2];    // BF16, K-contiguous: [row] -> 8 k
+  bf16x2 hr[8];    // BF16, row-contiguous: [k] -> 2 rows
 
-// GELU with the exact error function, and its derivative.
+  __device__ __forceinline__ void load(const E* base, int64_t ld, int r0, int k0, int tid) {
+    if constexpr (KC) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const E* p = base + (int64_t)(r0 + tid / 8 + 32 * i) * ld + k0 + (tid % 8) * 8;
+        if constexpr (BF) {
+          hk[i] = *reinterpret_cast<const bf16x8*>(p);
+        } else {
+          const float4 u = *reinterpret_cast<const float4*>(p);
+          const float4 w = *reinterpret_cast<const float4*>(p + 4);
+          f[i][0] = u.x; f[i][1] = u.y; f[i][2] = u.z; f[i][3] = u.w;
+          f[i][4] = w.x; f[i][5] = w.y; f[i][6] = w.z; f[i][7] = w.w;
+        }
+      }
+    } else {
+      const E* p = base + (int64_t)(k0 + 8 * (tid / 32)) * ld + r0 + 2 * (tid % 32);
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        if constexpr (BF) {
+          hr[q] = *reinterpret_cast<const bf16x2*>(p + q * ld);
+        } else {
+          const float2 v = *reinterpret_cast<const float2*>(p + q * ld);
+          f[0][q] = v.x;
+          f[1][q] = v.y;
+        }
+      }
+    }
+  }
+
+  // Branch-free: every element is loaded (an out-of-range one from element 0
+  // of the operand, which exists) and replaced by zero afterwards, so the
+  // loads are all in flight together.
+  __device__ __forceinline__ void load_checked(const E* base, int64_t ld, int R, int r0, int kend,
+                                               int k0, int tid) {
+    const E zero = (E)0.0f;
+    if constexpr (KC) {
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const int r = r0 + tid / 8 + 32 * i, k = k0 + (tid % 8) * 8;
+        E v[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const bool ok = r < R && k + j < kend;
+          v[j] = base[ok ? (int64_t)r * ld + k + j : 0];
+        }
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          const bool ok = r < R && k + j < kend;
+          if constexpr (BF) hk[i][j] = ok ? v[j] : zero;
+          else f[i][j] = ok ? v[j] : zero;
+        }
+      }
+    } else {
+      const int r = r0 + 2 * (tid % 32), k = k0 + 8 * (tid / 32);
+      E v[8][2];
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const bool ok = r + e < R && k + q < kend;
+          v[q][e] = base[ok ? (int64_t)(k + q) * ld + r + e : 0];
+        }
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const bool ok = r + e < R && k + q < kend;
+          if constexpr (BF) hr[q][e] = ok ? v[q][e] : zero;
+          else f[e][q] = ok ? v[q][e] : zero;
+        }
+    }
+  }
+
+  __device__ __forceinline__ void store(__bf16* S, int tid) const {
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      bf16x8 v;
+      if constexpr (BF && KC) {
+        v = hk[i];
+      } else {
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+          if constexpr (BF) v[j] = hr[j][i];
+          else v[j] = (__bf16)f[i][j];
+        }
+      }
+      const int r = KC ? tid / 8 + 32 * i : 2 * (tid % 32) + i;
+      const int k = KC ? (tid % 8) * 8 : 8 * (tid / 32);
+      *reinterpret_cast<bf16x8*>(S + r * LDK + k) = v;
+    }
+  }
+};
+
 __device__ __forceinline__ float gelu(float z) {
   return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f));
 }
@@ -73,67 +190,109 @@ __device__ __forceinline__ float gelu_grad(float z) {
          z * 0.39894228040143268f * expf(-0.5f * z * z);
 }
 
+// One K step (64) of the wave's 32 x 32 sub-tile from the LDS tiles.
+__device__ __forceinline__ void mfma_step(const __bf16* As, const __bf16* Bs, f32x16& acc,
+                                          int wm, int wn, int lane) {
+  const __bf16* as = As + (wm * 32 + lane % 32) * LDK + (lane / 32) * 8;
+  const __bf16* bs = Bs + (wn * 32 + lane % 32) * LDK + (lane / 32) * 8;
+#pragma unroll
+  for (int kk = 0; kk < TK; kk += 16) {
+    const bf16x8 a = *reinterpret_cast<const bf16x8*>(as + kk);
+    const bf16x8 b = *reinterpret_cast<const bf16x8*>(bs + kk);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+  }
+}
+
+// Grid (ceil(N / 64), ceil(M / 64), splits); split z covers
+// k in [z k_chunk, min(K, (z + 1) k_chunk)).  Interior tiles of aligned
+// operands run the full K steps pipelined with unchecked loads; edge tiles,
+// the K tail and unaligned operands take checked, unpipelined steps.
+template <bool AKC, bool ABF, bool BKC, bool BBF>
 __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
-  __shared__ __align__(16) __bf16 As[BM * LDSK];
-  __shared__ __align__(16) __bf16 Bs[BN * LDSK];
+  __shared__ __align__(16) __bf16 As[2][TM * LDK];
+  __shared__ __align__(16) __bf16 Bs[2][TN * LDK];
+  using SA = Stage<AKC, ABF>;
+  using SB = Stage<BKC, BBF>;
+  const auto* A = reinterpret_cast<const typename SA::E*>(g.A);
+  const auto* B = reinterpret_cast<const typename SB::E*>(g.B);
   const int tid = threadIdx.x;
   const int lane = tid % 64, wave = tid / 64;
   const int wm = wave / 2, wn = wave % 2;
-  const int m0 = blockIdx.y * BM, n0 = blockIdx.x * BN;
+  const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+  const int kb = blockIdx.z * g.k_chunk;
+  const int ke = min(g.K, kb + g.k_chunk);
   f32x16 acc = {0};
-  // staging: thread -> (row, 8 consecutive k)
-  const int srow = tid / 4, sk = (tid % 4) * 8;
-  for (int k0 = 0; k0 < g.K; k0 += BK) {
-    {
-      const int m = m0 + srow;
-      __bf16 v[8];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int k = k0 + sk + j;
-        v[j] = (__bf16)((m < g.M && k < g.K) ? load_elem(g.A, (int64_t)m * g.a_rs + (int64_t)k * g.a_cs, g.a_bf16) : 0.0f);
-      }
-#pragma unroll
-      for (int j = 0; j < 8; ++j) As[srow * LDSK + sk + j] = v[j];
-    }
-    {
-      const int n = n0 + srow;
-      __bf16 v[8];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        const int k = k0 + sk + j;
-        v[j] = (__bf16)((n < g.N && k < g.K) ? load_elem(g.B, (int64_t)k * g.b_rs + (int64_t)n * g.b_cs, g.b_bf16) : 0.0f);
-      }
-#pragma unroll
-      for (int j = 0; j < 8; ++j) Bs[srow * LDSK + sk + j] = v[j];
-    }
+  SA sa;
+  SB sb;
+  const int nfull = (g.vec && m0 + TM <= g.M && n0 + TN <= g.N && ke > kb) ? (ke - kb) / TK : 0;
+  int k0 = kb;
+  if (nfull > 0) {
+    sa.load(A, g.lda, m0, k0, tid);
+    sb.load(B, g.ldb, n0, k0, tid);
+    sa.store(As[0], tid);
+    sb.store(Bs[0], tid);
     __syncthreads();
-#pragma unroll
-    for (int kk = 0; kk < BK; kk += 16) {
-      bf16x8 a, b;
-      const __bf16* ap = As + (wm * 32 + lane % 32) * LDSK + kk + (lane / 32) * 8;
-      const __bf16* bp = Bs + (wn * 32 + lane % 32) * LDSK + kk + (lane / 32) * 8;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) {
-        a[j] = ap[j];
-        b[j] = bp[j];
-      }
-      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    for (int s = 0; s + 1 < nfull; ++s) {
+      sa.load(A, g.lda, m0, k0 + TK, tid);
+      sb.load(B, g.ldb, n0, k0 + TK, tid);
+      mfma_step(As[s & 1], Bs[s & 1], acc, wm, wn, lane);
+      sa.store(As[(s + 1) & 1], tid);
+      sb.store(Bs[(s + 1) & 1], tid);
+      __syncthreads();
+      k0 += TK;
     }
+    mfma_step(As[(nfull - 1) & 1], Bs[(nfull - 1) & 1], acc, wm, wn, lane);
+    k0 += TK;
     __syncthreads();
   }
+  for (; k0 < ke; k0 += TK) {
+    sa.load_checked(A, g.lda, g.M, m0, ke, k0, tid);
+    sb.load_checked(B, g.ldb, g.N, n0, ke, k0, tid);
+    sa.store(As[0], tid);
+    sb.store(Bs[0], tid);
+    __syncthreads();
+    mfma_step(As[0], Bs[0], acc, wm, wn, lane);
+    __syncthreads();
+  }
+  float* C = g.C + (int64_t)blockIdx.z * g.split_stride;
   const int col = n0 + wn * 32 + lane % 32;
+  const float bcol = (g.bias && col < g.N) ? g.bias[col] : 0.0f;  // one load per lane
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int row = m0 + wm * 32 + (r / 4) * 8 + (lane / 32) * 4 + r % 4;
     if (row < g.M && col < g.N) {
       float v = acc[r];
-      if (g.bias) v += g.bias[col];
+      if (g.bias) v += bcol;
       if (g.gelu) {
         g.Z[(int64_t)row * g.ldc + col] = v;
         v = gelu(v);
       }
-      g.C[(int64_t)row * g.ldc + col] = v;
+      C[(int64_t)row * g.ldc + col] = v;
     }
+  }
+}
+
+// Split-K epilogue: C = sum over s (in order) of ws[s][m][n], then the bias
+// and GELU of the forward pass (Z keeps the pre-activation) or, when gz is
+// given, the GELU derivative of the backward pass (C *= gelu'(gz)).
+__global__ void splitk_reduce_kernel(const float* __restrict__ ws, int splits, int M, int N,
+                                     float* __restrict__ C, int64_t ldc,
+                                     const float* __restrict__ bias, float* __restrict__ Z,
+                                     int gelu_out, const float* __restrict__ gz) {
+  const int64_t n = (int64_t)M * N;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int m = (int)(i / N), c = (int)(i % N);
+    float v = ws[i];
+    for (int s = 1; s < splits; ++s) v += ws[s * n + i];
+    if (bias) v += bias[c];
+    const int64_t o = (int64_t)m * ldc + c;
+    if (gelu_out) {
+      Z[o] = v;
+      v = gelu(v);
+    }
+    if (gz) v *= gelu_grad(gz[o]);
+    C[o] = v;
   }
 }
 
@@ -165,13 +324,25 @@ __global__ void mse_kernel(const float* __restrict__ y, const float* __restrict_
   if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
 }
 
-// db[n] = sum over rows of d[m][n].
-__global__ void colsum_kernel(const float* __restrict__ d, int M, int N, float* __restrict__ db) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
+// db[n] = sum over rows of d[m][n]: block (64 columns x 16 row slices); each
+// slice sums its rows in order, the 16 slice sums are added in order.
+constexpr int kColsumSlices = 16;
+__global__ void __launch_bounds__(64 * kColsumSlices)
+colsum_kernel(const float* __restrict__ d, int M, int N, float* __restrict__ db) {
+  __shared__ float part[kColsumSlices][65];
+  const int c = blockIdx.x * 64 + threadIdx.x;
   float s = 0.0f;
-  for (int m = 0; m < M; ++m) s += d[(int64_t)m * N + n];
-  db[n] = s;
+  if (c < N) {
+#pragma unroll 4
+    for (int m = threadIdx.y; m < M; m += kColsumSlices) s += d[(int64_t)m * N + c];
+  }
+  part[threadIdx.y][threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.y == 0 && c < N) {
+    float t = 0.0f;
+    for (int i = 0; i < kColsumSlices; ++i) t += part[i][threadIdx.x];
+    db[c] = t;
+  }
 }
 
 __global__ void gather_rows_kernel(const float* __restrict__ src, const int32_t* __restrict__ idx,
@@ -278,9 +449,12 @@ struct gcm_mlp {
   float *x = nullptr, *z1 = nullptr, *h1 = nullptr, *z2 = nullptr, *h2 = nullptr,
         *y = nullptr, *t = nullptr, *dy = nullptr, *d2 = nullptr, *d1 = nullptr,
         *partial = nullptr;
-  int32_t* idx = nullptr;
-  // resident training set
+  // split-K partial sums
+  float* ws = nullptr;
+  size_t ws_cap = 0;
+  // resident training set and the device copy of an epoch's shuffled order
   float *dx = nullptr, *dt = nullptr;
+  int32_t* order = nullptr;
   int64_t n_data = 0;
   // likelihood head: mean, scale, data, ratio (n_out each) and per-row chi2
   double* head = nullptr;
@@ -299,7 +473,7 @@ struct gcm_mlp {
   int ensure_batch(int rows) {
     if (rows <= cap) return GCM_OK;
     for (void* q : {(void*)x, (void*)z1, (void*)h1, (void*)z2, (void*)h2, (void*)y,
-                    (void*)t, (void*)dy, (void*)d2, (void*)d1, (void*)idx}) {
+                    (void*)t, (void*)dy, (void*)d2, (void*)d1}) {
       if (!q) continue;
       (void)hipFree(q);
       allocs.erase(std::remove(allocs.begin(), allocs.end(), q), allocs.end());
@@ -317,19 +491,93 @@ struct gcm_mlp {
     step(alloc(&dy, (size_t)rows * n_out));
     step(alloc(&d2, (size_t)rows * H));
     step(alloc(&d1, (size_t)rows * H));
-    step(alloc(&idx, (size_t)rows));
     return rc;
+  }
+  int ensure_ws(size_t floats) {
+    if (floats <= ws_cap) return GCM_OK;
+    if (ws) {
+      MLP_HIP(hipStreamSynchronize(stream));  // queued launches may still read it
+      (void)hipFree(ws);
+      allocs.erase(std::remove(allocs.begin(), allocs.end(), (void*)ws), allocs.end());
+      ws = nullptr;
+    }
+    ws_cap = floats;
+    return alloc(&ws, floats);
   }
 };
 
 namespace {
 
-int gemm(gcm_mlp* m, int M, int N, int K, const void* A, int64_t a_rs, int64_t a_cs,
-         int a_bf16, const void* B, int64_t b_rs, int64_t b_cs, int b_bf16, float* C,
-         int64_t ldc, const float* bias, float* Z) {
-  GemmArgs g{M, N, K, A, a_rs, a_cs, a_bf16, B, b_rs, b_cs, b_bf16, C, ldc, bias, Z, Z != nullptr};
-  dim3 grid((N + BN - 1) / BN, (M + BM - 1) / BM);
-  gemm_bf16_kernel<<<grid, 256, 0, m->stream>>>(g);
+// Whether an operand meets the alignment of the kernel's unchecked loads:
+// 16-byte vectors along K (row pitch a multiple of the vector), 2-element
+// vectors along the rows (even row pitch).
+bool vec_ok(const void* p, int64_t ld, int kc, int bf16) {
+  const uintptr_t a = reinterpret_cast<uintptr_t>(p);
+  const int es = bf16 ? 2 : 4;
+  if (kc) return a % 16 == 0 && (ld * es) % 16 == 0;
+  return a % (2 * es) == 0 && ld % 2 == 0;
+}
+
+hipError_t launch_gemm(int sel, dim3 grid, hipStream_t stream, const GemmArgs& g) {
+  switch (sel) {
+#define GCM_GEMM_CASE(i)                                                                   \
+  case i:                                                                                  \
+    gemm_bf16_kernel<((i)&8) != 0, ((i)&4) != 0, ((i)&2) != 0, ((i)&1) != 0><<<grid, 256, 0, \
+                                                                             stream>>>(g); \
+    break;
+    GCM_GEMM_CASE(0) GCM_GEMM_CASE(1) GCM_GEMM_CASE(2) GCM_GEMM_CASE(3)
+    GCM_GEMM_CASE(4) GCM_GEMM_CASE(5) GCM_GEMM_CASE(6) GCM_GEMM_CASE(7)
+    GCM_GEMM_CASE(8) GCM_GEMM_CASE(9) GCM_GEMM_CASE(10) GCM_GEMM_CASE(11)
+    GCM_GEMM_CASE(12) GCM_GEMM_CASE(13) GCM_GEMM_CASE(14) GCM_GEMM_CASE(15)
+#undef GCM_GEMM_CASE
+  }
+  return hipGetLastError();
+}
+
+// C[M x N] = A . B (+ bias, GELU with the pre-activation in Z), operand
+// layouts as in GemmArgs.  With gz, C is dense (ldc == N) and is multiplied by
+// gelu'(gz) elementwise (the backward pass through a GELU layer).  Output
+// tilings of fewer than 128 tiles are split along K, towards 1024 blocks, into
+// up to 32 chunks of at least two K steps (a block's K steps are bound by load
+// latency, so more blocks in flight finish sooner); the partials go to a
+// workspace and are summed in split order, so the result does not depend on
+// scheduling.
+int gemm(gcm_mlp* m, int M, int N, int K, const void* A, int64_t lda, int a_kc, int a_bf16,
+         const void* B, int64_t ldb, int b_kc, int b_bf16, float* C, int64_t ldc,
+         const float* bias, float* Z, const float* gz = nullptr) {
+  if (M <= 0 || N <= 0) return GCM_OK;
+  if (gz && ldc != N) return fail(GCM_ERR_ARG, "gemm: GELU backward needs a dense output");
+  const int tiles = ((M + TM - 1) / TM) * ((N + TN - 1) / TN);
+  const int steps = std::max(1, (K + TK - 1) / TK);
+  int splits = 1;
+  if (tiles < 128 && steps >= 4)
+    splits = std::max(1, std::min({steps / 2, (1024 + tiles - 1) / tiles, kMaxSplit}));
+  const int chunk_steps = (steps + splits - 1) / splits;
+  splits = (steps + chunk_steps - 1) / chunk_steps;
+  GemmArgs g{M, N, K, A, lda, B, ldb, vec_ok(A, lda, a_kc, a_bf16) && vec_ok(B, ldb, b_kc, b_bf16),
+             C, ldc, 0, chunk_steps * TK, bias, Z, Z != nullptr};
+  const int sel = (a_kc ? 8 : 0) | (a_bf16 ? 4 : 0) | (b_kc ? 2 : 0) | (b_bf16 ? 1 : 0);
+  dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM, splits);
+  if (splits == 1) {
+    MLP_HIP(launch_gemm(sel, grid, m->stream, g));
+    if (gz) {
+      gelu_backward_kernel<<<grid_for((int64_t)M * N), 256, 0, m->stream>>>(C, gz, (int64_t)M * N);
+      MLP_HIP(hipGetLastError());
+    }
+    return GCM_OK;
+  }
+  const int64_t mn = (int64_t)M * N;
+  int rc = m->ensure_ws((size_t)splits * mn);
+  if (rc) return rc;
+  g.C = m->ws;
+  g.ldc = N;
+  g.split_stride = mn;
+  g.bias = nullptr;
+  g.Z = nullptr;
+  g.gelu = 0;
+  MLP_HIP(launch_gemm(sel, grid, m->stream, g));
+  splitk_reduce_kernel<<<grid_for(mn), 256, 0, m->stream>>>(m->ws, splits, M, N, C, ldc, bias, Z,
+                                                            Z != nullptr, gz);
   MLP_HIP(hipGetLastError());
   return GCM_OK;
 }
@@ -338,13 +586,13 @@ int gemm(gcm_mlp* m, int M, int N, int K, const void* A, int64_t a_rs, int64_t a
 // + b2), y = h2 W3^T + b3.
 int forward(gcm_mlp* m, int rows) {
   const __bf16* w = m->pb;
-  int rc = gemm(m, rows, m->H, m->n_in, m->x, m->n_in, 1, 0, w + m->off_w1, 1, m->n_in, 1,
-                m->h1, m->H, m->p + m->off_b1, m->z1);
+  int rc = gemm(m, rows, m->H, m->n_in, m->x, m->n_in, 1, 0, w + m->off_w1, m->n_in, 1, 1, m->h1,
+                m->H, m->p + m->off_b1, m->z1);
   if (rc) return rc;
-  rc = gemm(m, rows, m->H, m->H, m->h1, m->H, 1, 0, w + m->off_w2, 1, m->H, 1, m->h2, m->H,
+  rc = gemm(m, rows, m->H, m->H, m->h1, m->H, 1, 0, w + m->off_w2, m->H, 1, 1, m->h2, m->H,
             m->p + m->off_b2, m->z2);
   if (rc) return rc;
-  return gemm(m, rows, m->n_out, m->H, m->h2, m->H, 1, 0, w + m->off_w3, 1, m->H, 1, m->y,
+  return gemm(m, rows, m->n_out, m->H, m->h2, m->H, 1, 0, w + m->off_w3, m->H, 1, 1, m->y,
               m->n_out, m->p + m->off_b3, nullptr);
 }
 
@@ -356,28 +604,30 @@ int backward(gcm_mlp* m, int rows, double* loss) {
   MLP_HIP(hipGetLastError());
   const __bf16* w = m->pb;
   float* g = m->g;
+  const dim3 cs_block(64, kColsumSlices);
   // dW3 = dY^T h2 : (n_out x rows) . (rows x H)
-  int rc = gemm(m, m->n_out, m->H, rows, m->dy, 1, m->n_out, 0, m->h2, m->H, 1, 0,
+  int rc = gemm(m, m->n_out, m->H, rows, m->dy, m->n_out, 0, 0, m->h2, m->H, 0, 0,
                 g + m->off_w3, m->H, nullptr, nullptr);
   if (rc) return rc;
-  colsum_kernel<<<(m->n_out + 255) / 256, 256, 0, m->stream>>>(m->dy, rows, m->n_out, g + m->off_b3);
-  // dH2 = dY W3 : (rows x n_out) . (n_out x H)
-  rc = gemm(m, rows, m->H, m->n_out, m->dy, m->n_out, 1, 0, w + m->off_w3, m->H, 1, 1, m->d2,
-            m->H, nullptr, nullptr);
+  colsum_kernel<<<(m->n_out + 63) / 64, cs_block, 0, m->stream>>>(m->dy, rows, m->n_out, g + m->off_b3);
+  // dH2 = (dY W3) * gelu'(z2) : (rows x n_out) . (n_out x H)
+  rc = gemm(m, rows, m->H, m->n_out, m->dy, m->n_out, 1, 0, w + m->off_w3, m->H, 0, 1, m->d2,
+            m->H, nullptr, nullptr, m->z2);
   if (rc) return rc;
-  gelu_backward_kernel<<<grid_for((int64_t)rows * m->H), 256, 0, m->stream>>>(m->d2, m->z2, (int64_t)rows * m->H);
-  rc = gemm(m, m->H, m->H, rows, m->d2, 1, m->H, 0, m->h1, m->H, 1, 0, g + m->off_w2, m->H,
+  // dW2 = dH2^T h1
+  rc = gemm(m, m->H, m->H, rows, m->d2, m->H, 0, 0, m->h1, m->H, 0, 0, g + m->off_w2, m->H,
             nullptr, nullptr);
   if (rc) return rc;
-  colsum_kernel<<<(m->H + 255) / 256, 256, 0, m->stream>>>(m->d2, rows, m->H, g + m->off_b2);
-  rc = gemm(m, rows, m->H, m->H, m->d2, m->H, 1, 0, w + m->off_w2, m->H, 1, 1, m->d1, m->H,
-            nullptr, nullptr);
+  colsum_kernel<<<(m->H + 63) / 64, cs_block, 0, m->stream>>>(m->d2, rows, m->H, g + m->off_b2);
+  // dH1 = (dH2 W2) * gelu'(z1)
+  rc = gemm(m, rows, m->H, m->H, m->d2, m->H, 1, 0, w + m->off_w2, m->H, 0, 1, m->d1, m->H,
+            nullptr, nullptr, m->z1);
   if (rc) return rc;
-  gelu_backward_kernel<<<grid_for((int64_t)rows * m->H), 256, 0, m->stream>>>(m->d1, m->z1, (int64_t)rows * m->H);
-  rc = gemm(m, m->H, m->n_in, rows, m->d1, 1, m->H, 0, m->x, m->n_in, 1, 0, g + m->off_w1,
+  // dW1 = dH1^T x
+  rc = gemm(m, m->H, m->n_in, rows, m->d1, m->H, 0, 0, m->x, m->n_in, 0, 0, g + m->off_w1,
             m->n_in, nullptr, nullptr);
   if (rc) return rc;
-  colsum_kernel<<<(m->H + 255) / 256, 256, 0, m->stream>>>(m->d1, rows, m->H, g + m->off_b1);
+  colsum_kernel<<<(m->H + 63) / 64, cs_block, 0, m->stream>>>(m->d1, rows, m->H, g + m->off_b1);
   MLP_HIP(hipGetLastError());
   if (loss) {
     std::vector<float> part(nb);
@@ -572,14 +822,16 @@ int gcm_mlp_set_data(gcm_mlp* m, const float* x, const float* t, int64_t rows) {
   MLP_HIP(hipSetDevice(m->device));
   MLP_HIP(hipStreamSynchronize(m->stream));
   if (m->dx) {
-    (void)hipFree(m->dx);
-    (void)hipFree(m->dt);
-    m->allocs.erase(std::remove(m->allocs.begin(), m->allocs.end(), (void*)m->dx), m->allocs.end());
-    m->allocs.erase(std::remove(m->allocs.begin(), m->allocs.end(), (void*)m->dt), m->allocs.end());
+    for (void* q : {(void*)m->dx, (void*)m->dt, (void*)m->order}) {
+      (void)hipFree(q);
+      m->allocs.erase(std::remove(m->allocs.begin(), m->allocs.end(), q), m->allocs.end());
+    }
     m->dx = m->dt = nullptr;
+    m->order = nullptr;
   }
   int rc = m->alloc(&m->dx, (size_t)rows * m->n_in);
   if (rc == GCM_OK) rc = m->alloc(&m->dt, (size_t)rows * m->n_out);
+  if (rc == GCM_OK) rc = m->alloc(&m->order, (size_t)rows);
   if (rc) return rc;
   MLP_HIP(hipMemcpy(m->dx, x, (size_t)rows * m->n_in * 4, hipMemcpyHostToDevice));
   MLP_HIP(hipMemcpy(m->dt, t, (size_t)rows * m->n_out * 4, hipMemcpyHostToDevice));
@@ -601,13 +853,17 @@ int gcm_mlp_train(gcm_mlp* m, int epochs, int batch, float lr, uint64_t seed,
   std::mt19937_64 rng(seed);
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   for (int ep = 0; ep < epochs; ++ep) {
+    // the previous epoch's copy of `order` has completed before it is reshuffled
+    MLP_HIP(hipStreamSynchronize(m->stream));
     std::shuffle(order.begin(), order.end(), rng);
+    MLP_HIP(hipMemcpyAsync(m->order, order.data(), (size_t)m->n_data * 4, hipMemcpyHostToDevice,
+                           m->stream));
     double acc = 0;
     int nbatch = 0, nsampled = 0;
     for (int64_t r0 = 0; r0 + batch <= m->n_data; r0 += batch, ++nbatch) {
-      MLP_HIP(hipMemcpyAsync(m->idx, order.data() + r0, batch * 4, hipMemcpyHostToDevice, m->stream));
-      gather_rows_kernel<<<grid_for((int64_t)batch * m->n_in), 256, 0, m->stream>>>(m->dx, m->idx, batch, m->n_in, m->x);
-      gather_rows_kernel<<<grid_for((int64_t)batch * m->n_out), 256, 0, m->stream>>>(m->dt, m->idx, batch, m->n_out, m->t);
+      const int32_t* idx = m->order + r0;
+      gather_rows_kernel<<<grid_for((int64_t)batch * m->n_in), 256, 0, m->stream>>>(m->dx, idx, batch, m->n_in, m->x);
+      gather_rows_kernel<<<grid_for((int64_t)batch * m->n_out), 256, 0, m->stream>>>(m->dt, idx, batch, m->n_out, m->t);
       if ((rc = forward(m, batch))) return rc;
       // the loss is fetched (a synchronisation) on every 16th minibatch only
       const bool want = loss_out != nullptr && (nbatch % 16) == 0;

@@ -21,7 +21,11 @@ def _rel(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
 
 
-@pytest.mark.parametrize("shape,rows", [((20, 64, 40), 100), ((37, 96, 130), 257)])
+# The last two shapes take the split-K paths: at (300, 512, 600) every GEMM
+# of the step is split 2-5 ways along K; at (24, 64, 3000) the 3000-long
+# reduction of the output layer's input gradient is split sixteen ways.
+@pytest.mark.parametrize("shape,rows", [((20, 64, 40), 100), ((37, 96, 130), 257),
+                                        ((300, 512, 600), 256), ((24, 64, 3000), 256)])
 def test_mlp_forward_and_gradients_match_fp64(shape, rows):
     print("mlp grad check", shape, rows, flush=True)
     n_in, h, n_out = shape
