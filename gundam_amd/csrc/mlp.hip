@@ -11,7 +11,8 @@
 // is C[8(r/4) + 4(l/32) + r%4][l%32].
 //
 // Parameters, flat FP32 master copy: W1[H][n_in] b1[H] W2[H][H] b2[H]
-// W3[n_out][H] b3[n_out]; the GEMMs read BF16 copies of the weights.
+// W3[n_out][H] b3[n_out]; the GEMMs read BF16 copies of the weights (W1 rows
+// padded to a multiple of 8 elements).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -46,8 +47,8 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 // Block tile 64 x 64 with a K step of 64; each of the four waves owns a
 // 32 x 32 sub-tile (one accumulator).  Operand tiles are double-buffered in
 // LDS as [row][k] BF16 with a padded row of 72 elements (144 B, 16-B aligned
-// for the 128-bit operand reads); the next K step is loaded into registers
-// while the current one is multiplied, one barrier per step.
+// for the 128-bit operand reads); the K step after next is loaded into
+// registers while the current one is multiplied, one barrier per step.
 constexpr int TM = 64, TN = 64, TK = 64;
 constexpr int LDK = TK + 8;
 constexpr int kMaxSplit = 32;
@@ -73,8 +74,9 @@ struct GemmArgs {
 
 // One 64 (rows) x 64 (k) operand tile held in registers between its global
 // load and its LDS store, for an operand that is K-contiguous (KC) or
-// row-contiguous, FP32 or BF16 (BF).  load() is unchecked (interior tile,
-// aligned operand); load_checked() reads rows >= R and k >= kend as zero.
+// row-contiguous, FP32 or BF16 (BF).  load() is unchecked along K (full K
+// steps of an aligned operand); load_checked() reads rows >= R and k >= kend
+// as zero.
 //  KC: thread t holds rows t/8 and t/8 + 32, k = 8 (t%8) .. + 7, read as
 //      16-byte vectors along k, stored as one 16-byte LDS write per row.
 //  row-contiguous: thread t holds rows 2 (t%32) and + 1, k = 8 (t/32) .. + 7,
@@ -89,11 +91,15 @@ struct Stage {
   bf16x8 hk[2];    // BF16, K-contiguous: [row] -> 8 k
   bf16x2 hr[8];    // BF16, row-contiguous: [k] -> 2 rows
 
-  __device__ __forceinline__ void load(const E* base, int64_t ld, int r0, int k0, int tid) {
+  // Rows past R are read as row R - 1 (K-contiguous) or as rows R - 2, R - 1
+  // (row-contiguous, R even): they only feed output rows that are not stored.
+  __device__ __forceinline__ void load(const E* base, int64_t ld, int R, int r0, int k0,
+                                       int tid) {
     if constexpr (KC) {
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
-        const E* p = base + (int64_t)(r0 + tid / 8 + 32 * i) * ld + k0 + (tid % 8) * 8;
+        const int r = min(r0 + tid / 8 + 32 * i, R - 1);
+        const E* p = base + (int64_t)r * ld + k0 + (tid % 8) * 8;
         if constexpr (BF) {
           hk[i] = *reinterpret_cast<const bf16x8*>(p);
         } else {
@@ -104,7 +110,8 @@ struct Stage {
         }
       }
     } else {
-      const E* p = base + (int64_t)(k0 + 8 * (tid / 32)) * ld + r0 + 2 * (tid % 32);
+      const int r = min(r0 + 2 * (tid % 32), R - 2);
+      const E* p = base + (int64_t)(k0 + 8 * (tid / 32)) * ld + r;
 #pragma unroll
       for (int q = 0; q < 8; ++q) {
         if constexpr (BF) {
@@ -204,8 +211,8 @@ __device__ __forceinline__ void mfma_step(const __bf16* As, const __bf16* Bs, f3
 }
 
 // Grid (ceil(N / 64), ceil(M / 64), splits); split z covers
-// k in [z k_chunk, min(K, (z + 1) k_chunk)).  Interior tiles of aligned
-// operands run the full K steps pipelined with unchecked loads; edge tiles,
+// k in [z k_chunk, min(K, (z + 1) k_chunk)).  Aligned operands run the full
+// K steps pipelined with unchecked loads (edge tiles by clamping the rows);
 // the K tail and unaligned operands take checked, unpipelined steps.
 template <bool AKC, bool ABF, bool BKC, bool BBF>
 __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
@@ -222,29 +229,52 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
   const int kb = blockIdx.z * g.k_chunk;
   const int ke = min(g.K, kb + g.k_chunk);
   f32x16 acc = {0};
-  SA sa;
-  SB sb;
-  const int nfull = (g.vec && m0 + TM <= g.M && n0 + TN <= g.N && ke > kb) ? (ke - kb) / TK : 0;
-  int k0 = kb;
+  SA sa, sa2;
+  SB sb, sb2;
+  const int nfull = (g.vec && ke > kb) ? (ke - kb) / TK : 0;
   if (nfull > 0) {
-    sa.load(A, g.lda, m0, k0, tid);
-    sb.load(B, g.ldb, n0, k0, tid);
+    // Two register sets: step s + 2 is loaded while step s is multiplied and
+    // step s + 1 (loaded one step earlier) is written to LDS.  Loads past the
+    // last full step re-read that step (in bounds, unused), so no load is
+    // conditional.
+    auto kstep = [&](int st) { return kb + min(st, nfull - 1) * TK; };
+    sa.load(A, g.lda, g.M, m0, kstep(0), tid);
+    sb.load(B, g.ldb, g.N, n0, kstep(0), tid);
+    sa2.load(A, g.lda, g.M, m0, kstep(1), tid);
+    sb2.load(B, g.ldb, g.N, n0, kstep(1), tid);
     sa.store(As[0], tid);
     sb.store(Bs[0], tid);
     __syncthreads();
-    for (int s = 0; s + 1 < nfull; ++s) {
-      sa.load(A, g.lda, m0, k0 + TK, tid);
-      sb.load(B, g.ldb, n0, k0 + TK, tid);
-      mfma_step(As[s & 1], Bs[s & 1], acc, wm, wn, lane);
-      sa.store(As[(s + 1) & 1], tid);
-      sb.store(Bs[(s + 1) & 1], tid);
+    int st = 0;
+    for (; st + 2 < nfull; st += 2) {
+      // LDS 0: step st; set 2: step st + 1
+      sa.load(A, g.lda, g.M, m0, kstep(st + 2), tid);
+      sb.load(B, g.ldb, g.N, n0, kstep(st + 2), tid);
+      __builtin_amdgcn_sched_barrier(0);  // issue the loads before the multiply
+      mfma_step(As[0], Bs[0], acc, wm, wn, lane);
+      sa2.store(As[1], tid);
+      sb2.store(Bs[1], tid);
       __syncthreads();
-      k0 += TK;
+      // LDS 1: step st + 1; set 1: step st + 2
+      sa2.load(A, g.lda, g.M, m0, kstep(st + 3), tid);
+      sb2.load(B, g.ldb, g.N, n0, kstep(st + 3), tid);
+      __builtin_amdgcn_sched_barrier(0);
+      mfma_step(As[1], Bs[1], acc, wm, wn, lane);
+      sa.store(As[0], tid);
+      sb.store(Bs[0], tid);
+      __syncthreads();
     }
-    mfma_step(As[(nfull - 1) & 1], Bs[(nfull - 1) & 1], acc, wm, wn, lane);
-    k0 += TK;
+    // one or two full steps left: LDS 0 holds step st, set 2 step st + 1
+    mfma_step(As[0], Bs[0], acc, wm, wn, lane);
+    if (st + 1 < nfull) {
+      sa2.store(As[1], tid);
+      sb2.store(Bs[1], tid);
+      __syncthreads();
+      mfma_step(As[1], Bs[1], acc, wm, wn, lane);
+    }
     __syncthreads();
   }
+  int k0 = kb + nfull * TK;
   for (; k0 < ke; k0 += TK) {
     sa.load_checked(A, g.lda, g.M, m0, ke, k0, tid);
     sb.load_checked(B, g.ldb, g.N, n0, ke, k0, tid);
@@ -355,10 +385,25 @@ __global__ void gather_rows_kernel(const float* __restrict__ src, const int32_t*
   }
 }
 
+// Where flat parameter i lives in the BF16 weight copy: W1 with its rows
+// padded to a multiple of 8 elements (16-byte aligned rows for the GEMM's
+// vector loads), then W2 and W3; biases (read in FP32) have no copy (-1).
+struct Bf16Map {
+  int64_t off_b1, off_w2, off_b2, off_w3, off_b3, bw2, bw3;
+  int n_in, n_in_pad;
+};
+
+__device__ __forceinline__ int64_t bf16_index(const Bf16Map& q, int64_t i) {
+  if (i < q.off_b1) return (i / q.n_in) * q.n_in_pad + i % q.n_in;
+  if (i >= q.off_w2 && i < q.off_b2) return q.bw2 + (i - q.off_w2);
+  if (i >= q.off_w3 && i < q.off_b3) return q.bw3 + (i - q.off_w3);
+  return -1;
+}
+
 // Adam (Kingma & Ba 2015) on the FP32 master copy; refreshes the BF16 copy.
 __global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g,
                             float* __restrict__ m1, float* __restrict__ m2,
-                            __bf16* __restrict__ pb, int64_t n, float lr, float b1,
+                            __bf16* __restrict__ pb, Bf16Map q, int64_t n, float lr, float b1,
                             float b2, float eps, float c1, float c2) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
@@ -369,14 +414,18 @@ __global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g,
     m2[i] = b;
     const float v = p[i] - lr * (a / c1) / (sqrtf(b / c2) + eps);
     p[i] = v;
-    pb[i] = (__bf16)v;
+    const int64_t j = bf16_index(q, i);
+    if (j >= 0) pb[j] = (__bf16)v;
   }
 }
 
-__global__ void to_bf16_kernel(const float* __restrict__ p, __bf16* __restrict__ pb, int64_t n) {
+__global__ void to_bf16_kernel(const float* __restrict__ p, __bf16* __restrict__ pb, Bf16Map q,
+                               int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x)
-    pb[i] = (__bf16)p[i];
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t j = bf16_index(q, i);
+    if (j >= 0) pb[j] = (__bf16)p[i];
+  }
 }
 
 int grid_for(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 4096); }
@@ -442,7 +491,9 @@ struct gcm_mlp {
   int64_t n_params = 0;
   int64_t off_w1 = 0, off_b1 = 0, off_w2 = 0, off_b2 = 0, off_w3 = 0, off_b3 = 0;
   float *p = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr;
-  __bf16* pb = nullptr;
+  __bf16* pb = nullptr;  // BF16 weights, layout of Bf16Map
+  Bf16Map bmap{};
+  int64_t n_bf16 = 0;
   int64_t adam_t = 0;
   // activations for a batch of up to cap rows
   int cap = 0;
@@ -510,12 +561,12 @@ namespace {
 
 // Whether an operand meets the alignment of the kernel's unchecked loads:
 // 16-byte vectors along K (row pitch a multiple of the vector), 2-element
-// vectors along the rows (even row pitch).
-bool vec_ok(const void* p, int64_t ld, int kc, int bf16) {
+// vectors along the rows (even row pitch and an even row count R).
+bool vec_ok(const void* p, int64_t ld, int kc, int bf16, int R) {
   const uintptr_t a = reinterpret_cast<uintptr_t>(p);
   const int es = bf16 ? 2 : 4;
   if (kc) return a % 16 == 0 && (ld * es) % 16 == 0;
-  return a % (2 * es) == 0 && ld % 2 == 0;
+  return a % (2 * es) == 0 && ld % 2 == 0 && R % 2 == 0;
 }
 
 hipError_t launch_gemm(int sel, dim3 grid, hipStream_t stream, const GemmArgs& g) {
@@ -554,7 +605,7 @@ int gemm(gcm_mlp* m, int M, int N, int K, const void* A, int64_t lda, int a_kc, 
     splits = std::max(1, std::min({steps / 2, (1024 + tiles - 1) / tiles, kMaxSplit}));
   const int chunk_steps = (steps + splits - 1) / splits;
   splits = (steps + chunk_steps - 1) / chunk_steps;
-  GemmArgs g{M, N, K, A, lda, B, ldb, vec_ok(A, lda, a_kc, a_bf16) && vec_ok(B, ldb, b_kc, b_bf16),
+  GemmArgs g{M, N, K, A, lda, B, ldb, vec_ok(A, lda, a_kc, a_bf16, M) && vec_ok(B, ldb, b_kc, b_bf16, N),
              C, ldc, 0, chunk_steps * TK, bias, Z, Z != nullptr};
   const int sel = (a_kc ? 8 : 0) | (a_bf16 ? 4 : 0) | (b_kc ? 2 : 0) | (b_bf16 ? 1 : 0);
   dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM, splits);
@@ -586,13 +637,13 @@ int gemm(gcm_mlp* m, int M, int N, int K, const void* A, int64_t lda, int a_kc, 
 // + b2), y = h2 W3^T + b3.
 int forward(gcm_mlp* m, int rows) {
   const __bf16* w = m->pb;
-  int rc = gemm(m, rows, m->H, m->n_in, m->x, m->n_in, 1, 0, w + m->off_w1, m->n_in, 1, 1, m->h1,
+  int rc = gemm(m, rows, m->H, m->n_in, m->x, m->n_in, 1, 0, w, m->bmap.n_in_pad, 1, 1, m->h1,
                 m->H, m->p + m->off_b1, m->z1);
   if (rc) return rc;
-  rc = gemm(m, rows, m->H, m->H, m->h1, m->H, 1, 0, w + m->off_w2, m->H, 1, 1, m->h2, m->H,
+  rc = gemm(m, rows, m->H, m->H, m->h1, m->H, 1, 0, w + m->bmap.bw2, m->H, 1, 1, m->h2, m->H,
             m->p + m->off_b2, m->z2);
   if (rc) return rc;
-  return gemm(m, rows, m->n_out, m->H, m->h2, m->H, 1, 0, w + m->off_w3, m->H, 1, 1, m->y,
+  return gemm(m, rows, m->n_out, m->H, m->h2, m->H, 1, 0, w + m->bmap.bw3, m->H, 1, 1, m->y,
               m->n_out, m->p + m->off_b3, nullptr);
 }
 
@@ -611,7 +662,7 @@ int backward(gcm_mlp* m, int rows, double* loss) {
   if (rc) return rc;
   colsum_kernel<<<(m->n_out + 63) / 64, cs_block, 0, m->stream>>>(m->dy, rows, m->n_out, g + m->off_b3);
   // dH2 = (dY W3) * gelu'(z2) : (rows x n_out) . (n_out x H)
-  rc = gemm(m, rows, m->H, m->n_out, m->dy, m->n_out, 1, 0, w + m->off_w3, m->H, 0, 1, m->d2,
+  rc = gemm(m, rows, m->H, m->n_out, m->dy, m->n_out, 1, 0, w + m->bmap.bw3, m->H, 0, 1, m->d2,
             m->H, nullptr, nullptr, m->z2);
   if (rc) return rc;
   // dW2 = dH2^T h1
@@ -620,7 +671,7 @@ int backward(gcm_mlp* m, int rows, double* loss) {
   if (rc) return rc;
   colsum_kernel<<<(m->H + 63) / 64, cs_block, 0, m->stream>>>(m->d2, rows, m->H, g + m->off_b2);
   // dH1 = (dH2 W2) * gelu'(z1)
-  rc = gemm(m, rows, m->H, m->H, m->d2, m->H, 1, 0, w + m->off_w2, m->H, 0, 1, m->d1, m->H,
+  rc = gemm(m, rows, m->H, m->H, m->d2, m->H, 1, 0, w + m->bmap.bw2, m->H, 0, 1, m->d1, m->H,
             nullptr, nullptr, m->z1);
   if (rc) return rc;
   // dW1 = dH1^T x
@@ -664,13 +715,17 @@ int gcm_mlp_create(int device, int n_in, int n_hidden, int n_out, uint64_t seed,
   m->off_w3 = m->off_b2 + n_hidden;
   m->off_b3 = m->off_w3 + (int64_t)n_out * n_hidden;
   m->n_params = m->off_b3 + n_out;
+  const int n_in_pad = (n_in + 7) / 8 * 8;
+  const int64_t bw2 = (int64_t)n_hidden * n_in_pad, bw3 = bw2 + (int64_t)n_hidden * n_hidden;
+  m->bmap = Bf16Map{m->off_b1, m->off_w2, m->off_b2, m->off_w3, m->off_b3, bw2, bw3, n_in, n_in_pad};
+  m->n_bf16 = bw3 + (int64_t)n_out * n_hidden;
   int rc = GCM_OK;
   auto step = [&](int r) { if (rc == GCM_OK) rc = r; };
   step(m->alloc(&m->p, m->n_params));
   step(m->alloc(&m->g, m->n_params));
   step(m->alloc(&m->m1, m->n_params));
   step(m->alloc(&m->m2, m->n_params));
-  step(m->alloc(&m->pb, m->n_params));
+  step(m->alloc(&m->pb, m->n_bf16));
   step(m->alloc(&m->partial, 4096));
   if (rc == GCM_OK && hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess)
     rc = fail(GCM_ERR_HIP, "stream creation failed");
@@ -690,9 +745,11 @@ int gcm_mlp_create(int device, int n_in, int n_hidden, int n_out, uint64_t seed,
   init(m->off_w2, (int64_t)n_hidden * n_hidden, n_hidden);
   init(m->off_w3, (int64_t)n_out * n_hidden, n_hidden);
   MLP_HIP(hipMemcpy(m->p, host.data(), m->n_params * 4, hipMemcpyHostToDevice));
-  MLP_HIP(hipMemset(m->m1, 0, m->n_params * 4));
-  MLP_HIP(hipMemset(m->m2, 0, m->n_params * 4));
-  to_bf16_kernel<<<grid_for(m->n_params), 256, 0, m->stream>>>(m->p, m->pb, m->n_params);
+  // on the library's (non-blocking) stream, ordered before the BF16 refresh
+  MLP_HIP(hipMemsetAsync(m->m1, 0, m->n_params * 4, m->stream));
+  MLP_HIP(hipMemsetAsync(m->m2, 0, m->n_params * 4, m->stream));
+  MLP_HIP(hipMemsetAsync(m->pb, 0, m->n_bf16 * sizeof(__bf16), m->stream));  // W1 row padding
+  to_bf16_kernel<<<grid_for(m->n_params), 256, 0, m->stream>>>(m->p, m->pb, m->bmap, m->n_params);
   MLP_HIP(hipStreamSynchronize(m->stream));
   *out = m;
   return GCM_OK;
@@ -727,7 +784,7 @@ int gcm_mlp_set_params(gcm_mlp* m, const float* in) {
   MLP_HIP(hipSetDevice(m->device));
   MLP_HIP(hipStreamSynchronize(m->stream));
   MLP_HIP(hipMemcpy(m->p, in, m->n_params * 4, hipMemcpyHostToDevice));
-  to_bf16_kernel<<<grid_for(m->n_params), 256, 0, m->stream>>>(m->p, m->pb, m->n_params);
+  to_bf16_kernel<<<grid_for(m->n_params), 256, 0, m->stream>>>(m->p, m->pb, m->bmap, m->n_params);
   MLP_HIP(hipStreamSynchronize(m->stream));
   return GCM_OK;
 }
@@ -876,7 +933,7 @@ int gcm_mlp_train(gcm_mlp* m, int epochs, int batch, float lr, uint64_t seed,
       m->adam_t++;
       const float c1 = 1.0f - std::pow(b1, (float)m->adam_t);
       const float c2 = 1.0f - std::pow(b2, (float)m->adam_t);
-      adam_kernel<<<grid_for(m->n_params), 256, 0, m->stream>>>(m->p, m->g, m->m1, m->m2, m->pb,
+      adam_kernel<<<grid_for(m->n_params), 256, 0, m->stream>>>(m->p, m->g, m->m1, m->m2, m->pb, m->bmap,
                                                                 m->n_params, lr, b1, b2, eps, c1, c2);
       MLP_HIP(hipGetLastError());
     }
