@@ -63,7 +63,8 @@ struct GemmArgs {
   // element (k, n) of B: B[n * ldb + k] when B is K-contiguous, else B[k * ldb + n]
   const void* B;
   int64_t ldb;
-  int vec;   // both operands meet the alignment of the unchecked vector loads
+  int vec;    // both operands meet the alignment of the unchecked vector loads
+  int yfast;  // tile order: M tiles fastest (B the larger operand), else N tiles
   float* C;  // C[m * ldc + n]; split z of a split-K launch writes C + z * split_stride
   int64_t ldc, split_stride;
   int k_chunk;        // K range of one split, a multiple of TK
@@ -225,8 +226,25 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
   const int tid = threadIdx.x;
   const int lane = tid % 64, wave = tid / 64;
   const int wm = wave / 2, wn = wave % 2;
-  const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
-  const int kb = blockIdx.z * g.k_chunk;
+  // Workgroups are dispatched to the 8 XCDs round-robin in linear order; each
+  // XCD gets a contiguous range of tiles, ordered so that the tiles sharing
+  // the larger operand's tiles share that XCD's L2.
+  const int gx = gridDim.x, gy = gridDim.y;
+  const int nblk = gx * gy * gridDim.z;
+  const int lin = blockIdx.x + gx * (blockIdx.y + gy * blockIdx.z);
+  const int q = nblk / 8, rem = nblk % 8, xcd = lin % 8;
+  const int t = xcd * q + min(xcd, rem) + lin / 8;
+  int bx, by;
+  if (g.yfast) {
+    by = t % gy;
+    bx = (t / gy) % gx;
+  } else {
+    bx = t % gx;
+    by = (t / gx) % gy;
+  }
+  const int bz = t / (gx * gy);
+  const int m0 = by * TM, n0 = bx * TN;
+  const int kb = bz * g.k_chunk;
   const int ke = min(g.K, kb + g.k_chunk);
   f32x16 acc = {0};
   SA sa, sa2;
@@ -284,7 +302,7 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
     mfma_step(As[0], Bs[0], acc, wm, wn, lane);
     __syncthreads();
   }
-  float* C = g.C + (int64_t)blockIdx.z * g.split_stride;
+  float* C = g.C + (int64_t)bz * g.split_stride;
   const int col = n0 + wn * 32 + lane % 32;
   const float bcol = (g.bias && col < g.N) ? g.bias[col] : 0.0f;  // one load per lane
 #pragma unroll
@@ -606,6 +624,7 @@ int gemm(gcm_mlp* m, int M, int N, int K, const void* A, int64_t lda, int a_kc, 
   const int chunk_steps = (steps + splits - 1) / splits;
   splits = (steps + chunk_steps - 1) / chunk_steps;
   GemmArgs g{M, N, K, A, lda, B, ldb, vec_ok(A, lda, a_kc, a_bf16, M) && vec_ok(B, ldb, b_kc, b_bf16, N),
+             (int64_t)N * K * (b_bf16 ? 2 : 4) > (int64_t)M * K * (a_bf16 ? 2 : 4),
              C, ldc, 0, chunk_steps * TK, bias, Z, Z != nullptr};
   const int sel = (a_kc ? 8 : 0) | (a_bf16 ? 4 : 0) | (b_kc ? 2 : 0) | (b_bf16 ? 1 : 0);
   dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM, splits);
