@@ -117,6 +117,38 @@ int main() {
       return 1;
     }
   }
+  // the toggles, timers and parameter indices of the reference's interface
+  if (gcm::CacheManager::HasCUDA()) {
+    std::printf("HasCUDA must be false\n");
+    return 1;
+  }
+  if (cm.ParameterIndex(0) != 0 || cm.ParameterIndex(v.n_parameters) >= 0 ||
+      cm.ParameterIndex(-1) >= 0) {
+    std::printf("ParameterIndex mismatch\n");
+    return 1;
+  }
+  {
+    // with the histogram copy off, the future leaves the last copy untouched
+    cm.Fill(nominal).get();
+    const std::vector<double> before = cm.SumWeights();
+    const bool was = cm.SetIsHistContentCopyEnabled(false);
+    cm.SetEnableDebugPrintouts(true);
+    cm.Fill(th0).get();
+    cm.SetEnableDebugPrintouts(false);
+    const bool unchanged = cm.SumWeights() == before;
+    cm.SetIsHistContentCopyEnabled(was);
+    cm.Fill(th0).get();
+    const bool refreshed = cm.SumWeights() != before;
+    const auto& ft = cm.GetCacheFillTimer();
+    const auto& pt = cm.GetPullFromDeviceTimer();
+    std::printf("fill timer: %ld fills, last %.3f ms, mean of last 10 %.3f ms; pull %.3f ms\n",
+                ft.Count(), ft.LastMs(), ft.AverageMs(), pt.AverageMs());
+    if (!was || !unchanged || !refreshed || ft.Count() != pt.Count() || ft.Count() < 10 ||
+        !(ft.AverageMs() > 0)) {
+      std::printf("toggle/timer check failed\n");
+      return 1;
+    }
+  }
   std::printf("resident %.1f MiB, nominal LLH check %.3g\n",
               cm.GetResidentMemory() / 1048576.0, total);
   gcm_synth_destroy(wl);

@@ -2,14 +2,17 @@
 // include/gcm.h with the shape of the reference's Cache::Manager
 // (src/CacheManager/include/CacheManager.h:56-410): Build, Fill returning a
 // deferred std::future<bool>, PropagateParameters, CopyHistogramContents,
-// CopyEventWeights, GetResidentMemory and the static toggles.  GUNDAM's
+// CopyEventWeights, ValidateHistogramContents, GetResidentMemory, the timers,
+// ParameterIndex and the toggles (per object here: there is no singleton).  GUNDAM's
 // Propagator can hold one of these instead of the singleton; the tables it
 // passes are the EventDialCache content laid out as gcm_tables.
 #pragma once
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <functional>
 #include <future>
 #include <stdexcept>
@@ -34,6 +37,32 @@ inline void check(int rc, const char* where) {
     throw Error(rc, std::string(where) + ": " + gcm_last_error());
 }
 
+// Running timer over the last N intervals, the shape of the reference's
+// GenericToolbox::Time::AveragedTimer<10> behind GetCacheFillTimer and
+// GetPullFromDeviceTimer (CacheManager.h:128-136).
+template <int N>
+class AveragedTimer {
+ public:
+  void add(double ms) {
+    last_ = ms;
+    ring_[count_ % N] = ms;
+    ++count_;
+  }
+  double LastMs() const { return last_; }
+  double AverageMs() const {
+    const long n = std::min<long>(count_, N);
+    double s = 0;
+    for (long i = 0; i < n; ++i) s += ring_[i];
+    return n ? s / n : 0.0;
+  }
+  long Count() const { return count_; }
+
+ private:
+  double ring_[N] = {};
+  double last_ = 0;
+  long count_ = 0;
+};
+
 class CacheManager {
  public:
   explicit CacheManager(int device = 0, int precision = GCM_PRECISION_FP64) {
@@ -44,11 +73,36 @@ class CacheManager {
   CacheManager(const CacheManager&) = delete;
   CacheManager& operator=(const CacheManager&) = delete;
 
-  // Cache::Manager::HasGPU (CacheManager.h:73-78)
+  // Cache::Manager::HasGPU (CacheManager.h:75-78)
   static bool HasGPU() {
     int n = 0;
     return gcm_device_count(&n) == GCM_OK && n > 0;
   }
+  // Cache::Manager::HasCUDA (CacheManager.h:70-73): this build is HIP on
+  // gfx950, with no CUDA.
+  static bool HasCUDA() { return false; }
+
+  // SetIsHistContentCopyEnabled (CacheManager.h:109-115): whether the future of
+  // Fill copies the histograms back; returns the previous setting.
+  bool SetIsHistContentCopyEnabled(bool enable) {
+    const bool old = hist_copy_;
+    hist_copy_ = enable;
+    return old;
+  }
+  // SetEnableDebugPrintouts (CacheManager.h:125-126): one line per completed
+  // Fill (enqueue and wait times, total -2 ln L of point 0) on stdout.
+  void SetEnableDebugPrintouts(bool enable) { debug_ = enable; }
+
+  // GetCacheFillTimer / GetPullFromDeviceTimer (CacheManager.h:128-136): the
+  // time Fill spends uploading and enqueueing, and the time its future's get()
+  // waits for the device and copies the results back.
+  const AveragedTimer<10>& GetCacheFillTimer() const { return fill_timer_; }
+  const AveragedTimer<10>& GetPullFromDeviceTimer() const { return pull_timer_; }
+
+  // ParameterIndex (CacheManager.h:159-163): parameters are addressed by
+  // their position in the theta vector the caller passes to Fill, so the
+  // index of parameter p is p itself (negative if it is not one of them).
+  int ParameterIndex(int p) const { return (p >= 0 && p < n_par_) ? p : -1; }
 
   void SetLaunchConfig(const gcm_launch_config& cfg) {
     check(gcm_set_launch_config(ctx_, &cfg), "gcm_set_launch_config");
@@ -67,7 +121,7 @@ class CacheManager {
   }
   bool IsBuilt() const { return built_; }
 
-  // SetIsEventWeightCopyEnabled (CacheManager.h:104-109)
+  // SetIsEventWeightCopyEnabled (CacheManager.h:117-123)
   bool SetIsEventWeightCopyEnabled(bool enable) {
     const bool old = weight_copy_;
     check(gcm_enable_weights(ctx_, enable ? 1 : 0), "gcm_enable_weights");
@@ -114,15 +168,25 @@ class CacheManager {
   // the copy is enabled.
   std::future<bool> Fill(const std::vector<double>& theta, int k_points = 1,
                          const std::vector<double>* table_values = nullptr) {
+    const auto t0 = Clock::now();
     check(gcm_set_parameters(ctx_, theta.data(), n_par_, k_points), "gcm_set_parameters");
     if (table_values && table_len_)
       check(gcm_set_table_values(ctx_, table_values->data(), table_len_, k_points),
             "gcm_set_table_values");
     check(gcm_fill(ctx_, nullptr), "gcm_fill");
+    fill_timer_.add(ms_since(t0));
     return std::async(std::launch::deferred, [this] {
+      const auto t1 = Clock::now();
       check(gcm_sync(ctx_), "gcm_sync");
       if (hist_copy_) CopyHistogramContents();
       if (weight_copy_) CopyEventWeights();
+      pull_timer_.add(ms_since(t1));
+      if (debug_) {
+        double llh = 0;
+        check(gcm_get_llh(ctx_, &llh, nullptr, nullptr), "gcm_get_llh");
+        std::printf("[gcm] fill %ld: enqueue %.3f ms, wait+copy %.3f ms, -2lnL %.10g\n",
+                    pull_timer_.Count(), fill_timer_.LastMs(), pull_timer_.LastMs(), llh);
+      }
       return true;
     });
   }
@@ -222,8 +286,14 @@ class CacheManager {
  private:
   gcm_context* ctx_ = nullptr;
   bool built_ = false;
+  using Clock = std::chrono::steady_clock;
+  static double ms_since(Clock::time_point t) {
+    return std::chrono::duration<double, std::milli>(Clock::now() - t).count();
+  }
   bool hist_copy_ = true;  // fIsHistContentCopyEnabled default true
   bool weight_copy_ = false;
+  bool debug_ = false;
+  AveragedTimer<10> fill_timer_, pull_timer_;
   int max_points_ = 1;
   int n_bins_ = 0, n_par_ = 0;
   int64_t n_events_ = 0, table_len_ = 0;
