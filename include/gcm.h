@@ -12,7 +12,10 @@
  *     thrown across the ABI and nothing calls exit().
  *   - Plain pointers and sizes only.  Host pointers unless a name ends in
  *     _device.  All floating point is IEEE binary64 at the boundary.
- *   - One context owns one set of device-resident tables on one GPU.
+ *   - One context owns one set of device-resident tables on one GPU.  A
+ *     context (and a gcm_mlp handle) is driven by one host thread at a time;
+ *     different contexts share no mutable state and may be used from
+ *     different threads at once.
  *
  * Dial payload layouts (one record per dial, in gcm_tables.dial_data).  They
  * are the reference's own record layouts so the facade can pass
