@@ -452,7 +452,7 @@ def mlp_kernels(args, log, n_in=300, hidden=512, n_out=6000, rows=16384, batch=2
     return {"network": f"{n_in}-{hidden}-{hidden}-{n_out}", "batch": batch,
             "steps": steps, "us_per_step": el / steps * 1e6, "tflops": tflops,
             "peak_tflops": MFMA_BF16_PEAK_TFLOPS, "frac": tflops / MFMA_BF16_PEAK_TFLOPS,
-            "bound": ("launch latency and L2/MALL operand re-reads: ~21 launches per "
+            "bound": ("launch latency and L2/MALL operand re-reads: 16 launches per "
                       "5.3-GFLOP step, 64x64 tiles")}
 
 

@@ -51,7 +51,8 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 // registers while the current one is multiplied, one barrier per step.
 constexpr int TM = 64, TN = 64, TK = 64;
 constexpr int LDK = TK + 8;
-constexpr int kMaxSplit = 32;
+constexpr int kMaxSplit = 16;
+constexpr int kMinSplitSteps = 16;  // split only long reductions (K >= 1024)
 
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
@@ -332,6 +333,7 @@ __global__ void splitk_reduce_kernel(const float* __restrict__ ws, int splits, i
        i += (int64_t)gridDim.x * blockDim.x) {
     const int m = (int)(i / N), c = (int)(i % N);
     float v = ws[i];
+#pragma unroll 8
     for (int s = 1; s < splits; ++s) v += ws[s * n + i];
     if (bias) v += bias[c];
     const int64_t o = (int64_t)m * ldc + c;
@@ -349,27 +351,6 @@ __global__ void gelu_backward_kernel(float* __restrict__ d, const float* __restr
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     d[i] *= gelu_grad(z[i]);
-}
-
-// dY = 2 (Y - T) / (M N); per-block partial sums of the squared error.
-__global__ void mse_kernel(const float* __restrict__ y, const float* __restrict__ t,
-                           float* __restrict__ dy, int64_t n, float scale,
-                           float* __restrict__ partial) {
-  __shared__ float s[256];
-  float acc = 0.0f;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const float e = y[i] - t[i];
-    dy[i] = 2.0f * e * scale;
-    acc += e * e;
-  }
-  s[threadIdx.x] = acc;
-  __syncthreads();
-  for (int st = 128; st > 0; st >>= 1) {
-    if (threadIdx.x < st) s[threadIdx.x] += s[threadIdx.x + st];
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) partial[blockIdx.x] = s[0];
 }
 
 // db[n] = sum over rows of d[m][n]: block (64 columns x 16 row slices); each
@@ -393,13 +374,64 @@ colsum_kernel(const float* __restrict__ d, int M, int N, float* __restrict__ db)
   }
 }
 
-__global__ void gather_rows_kernel(const float* __restrict__ src, const int32_t* __restrict__ idx,
-                                   int rows, int cols, float* __restrict__ dst) {
-  const int64_t n = (int64_t)rows * cols;
+// The loss head of the backward pass, laid out like colsum_kernel:
+// dY = 2 (Y - T) / (M N), db[n] = the column sums of dY, and the block's sum
+// of squared errors in partial[blockIdx.x] (each in a fixed order).
+__global__ void __launch_bounds__(64 * kColsumSlices)
+mse_colsum_kernel(const float* __restrict__ y, const float* __restrict__ t,
+                  float* __restrict__ dy, int M, int N, float scale, float* __restrict__ db,
+                  float* __restrict__ partial) {
+  __shared__ float part[kColsumSlices][65];
+  __shared__ float sq[kColsumSlices][65];
+  const int c = blockIdx.x * 64 + threadIdx.x;
+  float s = 0.0f, e2 = 0.0f;
+  if (c < N) {
+#pragma unroll 4
+    for (int m = threadIdx.y; m < M; m += kColsumSlices) {
+      const int64_t i = (int64_t)m * N + c;
+      const float e = y[i] - t[i];
+      const float g = 2.0f * e * scale;
+      dy[i] = g;
+      s += g;
+      e2 += e * e;
+    }
+  }
+  part[threadIdx.y][threadIdx.x] = s;
+  sq[threadIdx.y][threadIdx.x] = e2;
+  __syncthreads();
+  if (threadIdx.y == 0) {
+    float a = 0.0f, q = 0.0f;
+    for (int i = 0; i < kColsumSlices; ++i) {
+      a += part[i][threadIdx.x];
+      q += sq[i][threadIdx.x];
+    }
+    if (c < N) db[c] = a;
+    part[0][threadIdx.x] = q;
+  }
+  __syncthreads();
+  if (threadIdx.y == 0 && threadIdx.x == 0) {
+    float q = 0.0f;
+    for (int i = 0; i < 64; ++i) q += part[0][i];
+    partial[blockIdx.x] = q;
+  }
+}
+
+// Rows idx[0 .. rows) of the resident inputs and targets into the minibatch
+// buffers, in one launch.
+__global__ void gather_rows_kernel(const float* __restrict__ sx, const float* __restrict__ st,
+                                   const int32_t* __restrict__ idx, int rows, int cx, int ct,
+                                   float* __restrict__ dx, float* __restrict__ dt) {
+  const int64_t nx = (int64_t)rows * cx, n = nx + (int64_t)rows * ct;
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const int r = (int)(i / cols), c = (int)(i % cols);
-    dst[i] = src[(int64_t)idx[r] * cols + c];
+    if (i < nx) {
+      const int r = (int)(i / cx), c = (int)(i % cx);
+      dx[i] = sx[(int64_t)idx[r] * cx + c];
+    } else {
+      const int64_t j = i - nx;
+      const int r = (int)(j / ct), c = (int)(j % ct);
+      dt[j] = st[(int64_t)idx[r] * ct + c];
+    }
   }
 }
 
@@ -606,11 +638,12 @@ hipError_t launch_gemm(int sel, dim3 grid, hipStream_t stream, const GemmArgs& g
 // C[M x N] = A . B (+ bias, GELU with the pre-activation in Z), operand
 // layouts as in GemmArgs.  With gz, C is dense (ldc == N) and is multiplied by
 // gelu'(gz) elementwise (the backward pass through a GELU layer).  Output
-// tilings of fewer than 128 tiles are split along K, towards 1024 blocks, into
-// up to 32 chunks of at least two K steps (a block's K steps are bound by load
-// latency, so more blocks in flight finish sooner); the partials go to a
-// workspace and are summed in split order, so the result does not depend on
-// scheduling.
+// tilings of fewer than 128 tiles with at least 16 K steps are split along K,
+// towards 1024 blocks, into up to 16 chunks of at least two K steps (a
+// block's K steps are bound by load latency, so more blocks in flight finish
+// sooner; shorter reductions gain less than the extra reduction launch
+// costs); the partials go to a workspace and are summed in split order, so
+// the result does not depend on scheduling.
 int gemm(gcm_mlp* m, int M, int N, int K, const void* A, int64_t lda, int a_kc, int a_bf16,
          const void* B, int64_t ldb, int b_kc, int b_bf16, float* C, int64_t ldc,
          const float* bias, float* Z, const float* gz = nullptr) {
@@ -619,7 +652,7 @@ int gemm(gcm_mlp* m, int M, int N, int K, const void* A, int64_t lda, int a_kc, 
   const int tiles = ((M + TM - 1) / TM) * ((N + TN - 1) / TN);
   const int steps = std::max(1, (K + TK - 1) / TK);
   int splits = 1;
-  if (tiles < 128 && steps >= 4)
+  if (tiles < 128 && steps >= kMinSplitSteps)
     splits = std::max(1, std::min({steps / 2, (1024 + tiles - 1) / tiles, kMaxSplit}));
   const int chunk_steps = (steps + splits - 1) / splits;
   splits = (steps + chunk_steps - 1) / chunk_steps;
@@ -669,17 +702,17 @@ int forward(gcm_mlp* m, int rows) {
 // Loss and gradients into m->g for the rows in m->x / m->t.
 int backward(gcm_mlp* m, int rows, double* loss) {
   const int64_t ny = (int64_t)rows * m->n_out;
-  const int nb = grid_for(ny);
-  mse_kernel<<<nb, 256, 0, m->stream>>>(m->y, m->t, m->dy, ny, 1.0f / (float)ny, m->partial);
-  MLP_HIP(hipGetLastError());
+  const int nb = (m->n_out + 63) / 64;  // mse_colsum blocks, one partial each
   const __bf16* w = m->pb;
   float* g = m->g;
   const dim3 cs_block(64, kColsumSlices);
+  mse_colsum_kernel<<<nb, cs_block, 0, m->stream>>>(m->y, m->t, m->dy, rows, m->n_out,
+                                                    1.0f / (float)ny, g + m->off_b3, m->partial);
+  MLP_HIP(hipGetLastError());
   // dW3 = dY^T h2 : (n_out x rows) . (rows x H)
   int rc = gemm(m, m->n_out, m->H, rows, m->dy, m->n_out, 0, 0, m->h2, m->H, 0, 0,
                 g + m->off_w3, m->H, nullptr, nullptr);
   if (rc) return rc;
-  colsum_kernel<<<(m->n_out + 63) / 64, cs_block, 0, m->stream>>>(m->dy, rows, m->n_out, g + m->off_b3);
   // dH2 = (dY W3) * gelu'(z2) : (rows x n_out) . (n_out x H)
   rc = gemm(m, rows, m->H, m->n_out, m->dy, m->n_out, 1, 0, w + m->bmap.bw3, m->H, 0, 1, m->d2,
             m->H, nullptr, nullptr, m->z2);
@@ -745,7 +778,7 @@ int gcm_mlp_create(int device, int n_in, int n_hidden, int n_out, uint64_t seed,
   step(m->alloc(&m->m1, m->n_params));
   step(m->alloc(&m->m2, m->n_params));
   step(m->alloc(&m->pb, m->n_bf16));
-  step(m->alloc(&m->partial, 4096));
+  step(m->alloc(&m->partial, (size_t)(n_out + 63) / 64));  // one per mse_colsum block
   if (rc == GCM_OK && hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess)
     rc = fail(GCM_ERR_HIP, "stream creation failed");
   if (rc != GCM_OK) {
@@ -938,8 +971,8 @@ int gcm_mlp_train(gcm_mlp* m, int epochs, int batch, float lr, uint64_t seed,
     int nbatch = 0, nsampled = 0;
     for (int64_t r0 = 0; r0 + batch <= m->n_data; r0 += batch, ++nbatch) {
       const int32_t* idx = m->order + r0;
-      gather_rows_kernel<<<grid_for((int64_t)batch * m->n_in), 256, 0, m->stream>>>(m->dx, idx, batch, m->n_in, m->x);
-      gather_rows_kernel<<<grid_for((int64_t)batch * m->n_out), 256, 0, m->stream>>>(m->dt, idx, batch, m->n_out, m->t);
+      gather_rows_kernel<<<grid_for((int64_t)batch * (m->n_in + m->n_out)), 256, 0, m->stream>>>(
+          m->dx, m->dt, idx, batch, m->n_in, m->n_out, m->x, m->t);
       if ((rc = forward(m, batch))) return rc;
       // the loss is fetched (a synchronisation) on every 16th minibatch only
       const bool want = loss_out != nullptr && (nbatch % 16) == 0;
