@@ -49,6 +49,31 @@ def test_mlp_forward_and_gradients_match_fp64(shape, rows):
         assert _rel(a, b) < 3e-2, _rel(a, b)
 
 
+def test_mlp_predict_row_chunks_and_row_counts():
+    """predict over 1, 33 and 8193 rows against the FP64 restatement.  With
+    1100 inputs the first layer's reduction is split along K for the small
+    row counts (so the split workspace grows between calls) and not for the
+    8192-row chunk; 8193 rows cross the chunk boundary.  Rows of the large
+    call equal the same rows predicted alone (no mix-up of rows across
+    chunks or splits; the two paths sum in different orders, hence the
+    tolerance)."""
+    print("mlp predict chunks", flush=True)
+    rng = np.random.default_rng(6)
+    n_in, h, n_out = 1100, 64, 96
+    mlp = MLP(n_in, h, n_out, seed=9)
+    params = [a.astype(np.float64) for a in mlp.split(mlp.params().astype(np.float64))]
+    x = rng.standard_normal((8193, n_in)).astype(np.float32)
+    for rows in (1, 33, 8193):
+        y = mlp.predict(x[:rows])
+        y_ref, _ = mlp_oracle.forward(params, x[:rows].astype(np.float64))
+        assert _rel(y, y_ref) < 3e-2, (rows, _rel(y, y_ref))
+    big = mlp.predict(x)
+    for r in (0, 8191, 8192):
+        one = mlp.predict(x[r:r + 1])
+        np.testing.assert_allclose(big[r], one[0], rtol=1e-3, atol=1e-4)
+    mlp.close()
+
+
 def test_mlp_training_reduces_loss():
     print("mlp training", flush=True)
     rng = np.random.default_rng(4)
