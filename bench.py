@@ -459,20 +459,31 @@ def mlp_kernels(args, log, n_in=300, hidden=512, n_out=6000, rows=16384, batch=2
 def posterior_agreement(args, log):
     """Posterior means of the delayed-acceptance chains (surrogate sub-chains
     of args.c5_sub steps) against batched exact chains on the 100k-event
-    model (20 parameters), each run for args.post_seconds from the same
-    burn-in states: per-parameter difference of the means in units of the
-    exact chains' posterior sigma, with its Monte Carlo standard error from
-    batch means (20 batches per chain set)."""
+    model (20 parameters; --post-workload 2: the 5M-event model, 300
+    parameters, device table refresh), each run for args.post_seconds from the
+    same burn-in states, in segments of at most a minute with a progress line
+    after each: per-parameter difference of the means in units of the exact
+    chains' posterior sigma, with its Monte Carlo standard error from batch
+    means (20 batches per chain set)."""
     from gundam_amd import CacheManager, Workload
     from gundam_amd.mcmc import (ExactTarget, SurrogateTarget, adapt_cov, laplace_scales,
                                  run_batched_exact, run_delayed, var_ratio)
     from gundam_amd.surrogate import Surrogate
 
-    wl = Workload(1, seed=args.seed + 1)
+    wk = 2 if args.post_workload == 2 else 1
+    wl = Workload(wk, seed=args.seed + 1)
     t = wl.tables
     wl.bin_events_on_device(0)
-    cm = CacheManager(0)
+    if wk == 2:  # the launch shape the autotuner picks for this event set
+        cm = CacheManager(0, block_threads=384, tile_words=3840, blocks_per_cu=2,
+                          ring_buffers=2)
+    else:
+        cm = CacheManager(0)
     cm.build(t, max_points=64)
+    if t.table_len:
+        from gundam_amd import _native as N
+        cm.add_table_update(N.TABLE_UPDATE_COS, N.TABLE_UPDATE_COS_KERNEL, wl.table_jobs())
+    log(f"posterior: workload {wk} built, {t.n_events} events, {t.n_parameters} parameters")
     mode = wl.nominal()
     cm.propagate_parameters(mode)
     data, w2 = (a.copy() for a in cm.hist(0))
@@ -490,14 +501,32 @@ def posterior_agreement(args, log):
     X, Y = ex.recorded()
     cov = adapt_cov(burn.samples[200:], sc)
     sur = Surrogate(d, t.n_bins, hidden=512, seed=args.seed + 3)
-    sur.fit(X, Y, epochs=200, batch=256, lr=1e-3)
+    sur.fit(X, Y, epochs=300 if wk == 2 else 200, batch=256, lr=1e-3)
     surr = SurrogateTarget(sur, data, var_ratio(data, w2), wl.prior_mean, wl.prior_sigma)
+    log(f"posterior: burn-in, {X.shape[0]} training pairs and the surrogate done")
     start = burn.samples[-1]
     T = args.post_seconds
     big = 10 ** 8
-    res = {"exact": run_batched_exact(ex, start.copy(), cov, big, rng, max_seconds=T),
-           "delayed": run_delayed(ex, surr, start.copy(), cov, big, rng, n_sub=args.c5_sub,
-                                  max_seconds=T)}
+
+    class Joined:  # the segments of one chain set, joined
+        def __init__(self, samples):
+            self.samples = samples
+
+    def segmented(name, run):
+        parts, th, done = [], start.copy(), 0.0
+        while done < T:
+            r = run(th, min(60.0, T - done))
+            parts.append(r.samples)
+            th = r.samples[-1].copy()
+            done += r.seconds
+            log(f"posterior: {name} chains {done:.0f}/{T:.0f} s, "
+                f"{sum(p.shape[0] for p in parts)} iterations")
+        return Joined(np.concatenate(parts))
+
+    res = {"exact": segmented("exact", lambda th, sec: run_batched_exact(
+               ex, th, cov, big, rng, max_seconds=sec)),
+           "delayed": segmented("delayed", lambda th, sec: run_delayed(
+               ex, surr, th, cov, big, rng, n_sub=args.c5_sub, max_seconds=sec))}
 
     def batch_stats(smp, nb=20):
         smp = smp[smp.shape[0] // 10:]  # drop 10% as further burn-in
@@ -511,7 +540,9 @@ def posterior_agreement(args, log):
     diff = (m_da - m_ex) / sigma
     se = np.sqrt(se_ex ** 2 + se_da ** 2) / sigma
     out = {
-        "workload": "100k events, 20 parameters, 400 bins, Barlow-Beeston OA2021 (Asimov)",
+        "workload": ("100k events, 20 parameters, 400 bins, Barlow-Beeston OA2021 (Asimov)"
+                     if wk == 1 else
+                     "5M events, 300 parameters, 6000 bins, Barlow-Beeston OA2021 (Asimov)"),
         "chains": C_, "seconds_per_method": T, "n_sub": args.c5_sub,
         "iterations": {k: int(r.samples.shape[0]) for k, r in res.items()},
         "max_abs_mean_diff_sigma": float(np.max(np.abs(diff))),
@@ -569,6 +600,8 @@ def main():
     ap.add_argument("--only-posterior", action="store_true",
                     help="run only the posterior-agreement study (100k-event model)")
     ap.add_argument("--post-seconds", type=float, default=60.0)
+    ap.add_argument("--post-workload", type=int, default=1, choices=[1, 2],
+                    help="posterior study on the 100k-event (1) or 5M-event (2) model")
     ap.add_argument("--only-mlp", action="store_true",
                     help="run only the surrogate network's training-kernel timing")
     args = ap.parse_args()
