@@ -495,11 +495,11 @@ def posterior_agreement(args, log):
     C_ = 32
     ex.record = True
     burn = run_batched_exact(ex, mode + sc * rng.standard_normal((C_, d)),
-                             (2.38 ** 2 / d) * np.diag(sc ** 2), 400, rng)
+                             (2.38 ** 2 / d) * np.diag(sc ** 2), args.post_burn, rng)
     ex(mode + sc * rng.standard_normal((max(0, 16000 - ex.points), d)))
     ex.record = False
     X, Y = ex.recorded()
-    cov = adapt_cov(burn.samples[200:], sc)
+    cov = adapt_cov(burn.samples[burn.samples.shape[0] // 2:], sc)
     sur = Surrogate(d, t.n_bins, hidden=512, seed=args.seed + 3)
     sur.fit(X, Y, epochs=300 if wk == 2 else 200, batch=256, lr=1e-3)
     surr = SurrogateTarget(sur, data, var_ratio(data, w2), wl.prior_mean, wl.prior_sigma)
@@ -550,6 +550,14 @@ def posterior_agreement(args, log):
         "max_mc_standard_error_sigma": float(np.max(se)),
         "params_within_0p05_sigma": int(np.sum(np.abs(diff) < 0.05)),
         "max_abs_z": float(np.max(np.abs(diff / se))),
+        "burn_in_iterations": int(args.post_burn),
+        # median over parameters of the second half's spread over the first
+        # half's (after the 10% drop): > 1 while a chain set still widens
+        "width_growth": {k: float(np.median(
+            r.samples[r.samples.shape[0] // 10:][(r.samples.shape[0] * 9 // 10) // 2:]
+            .reshape(-1, d).std(0) /
+            r.samples[r.samples.shape[0] // 10:][:(r.samples.shape[0] * 9 // 10) // 2]
+            .reshape(-1, d).std(0))) for k, r in res.items()},
         "std_ratio_delayed_over_exact": [
             float(x) for x in np.round(res["delayed"].samples[res["delayed"].samples.shape[0] // 10:]
                                        .reshape(-1, d).std(0) / sigma, 4)],
@@ -600,6 +608,8 @@ def main():
     ap.add_argument("--only-posterior", action="store_true",
                     help="run only the posterior-agreement study (100k-event model)")
     ap.add_argument("--post-seconds", type=float, default=60.0)
+    ap.add_argument("--post-burn", type=int, default=400,
+                    help="batched exact burn-in iterations of the posterior study")
     ap.add_argument("--post-workload", type=int, default=1, choices=[1, 2],
                     help="posterior study on the 100k-event (1) or 5M-event (2) model")
     ap.add_argument("--only-mlp", action="store_true",
