@@ -449,9 +449,27 @@ def mlp_kernels(args, log, n_in=300, hidden=512, n_out=6000, rows=16384, batch=2
     tflops = flop * batch * steps / el / 1e12
     log(f"surrogate MLP training: {el / steps * 1e6:.1f} us per step of {batch} rows, "
         f"{tflops:.1f} TFLOP/s ({tflops / MFMA_BF16_PEAK_TFLOPS:.2%} of the BF16 MFMA peak)")
+    # the same kernels on 4096-row minibatches: what they reach once a launch
+    # carries enough work (the training itself uses 256-row minibatches)
+    big_rows, big_batch, big_epochs = 16384, 4096, 5
+    m = MLP(n_in, hidden, n_out, seed=args.seed + 5)
+    m.set_data(x[:big_rows], tgt[:big_rows])
+    m.train(1, big_batch, 1e-3, 0)
+    t0 = time.perf_counter()
+    m.train(big_epochs, big_batch, 1e-3, 1)
+    el_big = time.perf_counter() - t0
+    m.close()
+    steps_big = big_epochs * (big_rows // big_batch)
+    tflops_big = flop * big_batch * steps_big / el_big / 1e12
+    log(f"surrogate MLP kernels at {big_batch}-row minibatches: "
+        f"{el_big / steps_big * 1e6:.1f} us per step, {tflops_big:.1f} TFLOP/s "
+        f"({tflops_big / MFMA_BF16_PEAK_TFLOPS:.2%})")
     return {"network": f"{n_in}-{hidden}-{hidden}-{n_out}", "batch": batch,
             "steps": steps, "us_per_step": el / steps * 1e6, "tflops": tflops,
             "peak_tflops": MFMA_BF16_PEAK_TFLOPS, "frac": tflops / MFMA_BF16_PEAK_TFLOPS,
+            "large_batch": {"batch": big_batch, "steps": steps_big,
+                            "us_per_step": el_big / steps_big * 1e6, "tflops": tflops_big,
+                            "frac": tflops_big / MFMA_BF16_PEAK_TFLOPS},
             "bound": ("launch latency and L2/MALL operand re-reads: 16 launches per "
                       "5.3-GFLOP step, 64x64 tiles")}
 
