@@ -59,22 +59,14 @@ def build_synth() -> str:
 
 
 def build_hbm_copy() -> str:
+    """The copy kernel of known size whose FETCH_SIZE / WRITE_SIZE readings
+    calibrate the fill kernel's measured HBM traffic (bench.py's
+    roofline.traffic)."""
     out = os.path.join(HERE, "hbm_copy")
     tmp = out + ".tmp"
     _run([HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang",
           "-target-feature", "-Xclang", "-packed-fp32-ops", "-o", tmp,
           os.path.join(CSRC, "hbm_copy.hip")])
-    os.replace(tmp, out)
-    return out
-
-
-def build_lds_dma_stream() -> str:
-    """The LDS-DMA streaming calibration program (profiles/ uses its output)."""
-    out = os.path.join(HERE, "lds_dma_stream")
-    tmp = out + ".tmp"
-    _run([HIPCC, "--offload-arch=gfx950", "-O3", "-std=c++17", "-Xclang",
-          "-target-feature", "-Xclang", "-packed-fp32-ops", "-o", tmp,
-          os.path.join(CSRC, "lds_dma_stream.hip")])
     os.replace(tmp, out)
     return out
 
@@ -95,7 +87,6 @@ def build_all() -> None:
     build_gcm()
     build_table_update()
     build_hbm_copy()
-    build_lds_dma_stream()
     build_facade_demo()
 
 

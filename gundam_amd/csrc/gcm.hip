@@ -162,9 +162,12 @@ __global__ void prologue_kernel(const double* __restrict__ theta, int n_par,
                                 double* __restrict__ tab_in,
                                 const double* __restrict__ prior_mean,
                                 const double* __restrict__ prior_sigma,
-                                double* __restrict__ penalty) {
+                                double* __restrict__ penalty, int* __restrict__ flags) {
   const int k = blockIdx.x;
   const double* th = theta + (size_t)k * n_par;
+  // the NaN-weight flag reports this evaluation only (cleared in-stream, so
+  // that graph replays and fills that are never synchronised start clean)
+  if (k == 0 && threadIdx.x == 0) *flags = 0;
   for (int j = threadIdx.x; j < n_tab_idx; j += blockDim.x)
     tab_in[(size_t)k * n_tab_idx + j] = input_value(th, in_par, in_mlo, in_mrange, tab_idx[j]);
   for (int i = threadIdx.x; i < n_inputs; i += blockDim.x) {
@@ -1645,7 +1648,8 @@ int gcm_set_likelihood(gcm_context* c, const gcm_llh_config* cfg,
   if (cfg->bb_no_update_weights && !cfg->nominal_err)
     return fail(GCM_ERR_ARG, "bb_no_update_weights needs nominal_err");
   GCM_HIP(hipSetDevice(c->device));
-  GCM_HIP(hipStreamSynchronize(c->stream));
+  // a fill enqueued on a caller stream may still read the buffers replaced here
+  GCM_HIP(hipStreamSynchronize(c->last_stream ? c->last_stream : c->stream));
   c->llh.type = cfg->type;
   c->llh.use_poisson = cfg->use_poisson_likelihood;
   // BarlowBeestonBanff2022.h:85-88: hard limit on the fractional error
@@ -1885,7 +1889,8 @@ static int enqueue_fill(gcm_context* c, hipStream_t s, int K, bool copies,
                                     c->d_in_flags, c->n_inputs, c->d_xin,
                                     c->d_tab_in_idx, n_tab_idx, c->d_tab_in,
                                     c->llh_set ? c->d_prior_mean : nullptr,
-                                    c->llh_set ? c->d_prior_sigma : nullptr, c->d_penalty);
+                                    c->llh_set ? c->d_prior_sigma : nullptr, c->d_penalty,
+                                    c->d_flags);
   GCM_HIP(hipGetLastError());
   if (c->n_pen_sets > 0) {
     penalty_sets_kernel<<<K, 256, 0, s>>>(c->d_theta, c->n_par, c->n_pen_sets, c->d_pen_meta,
@@ -2052,8 +2057,7 @@ int gcm_sync(gcm_context* c) {
   if (c->built) {
     int* flag = reinterpret_cast<int*>(c->h_res + 3 * (size_t)c->max_points);
     if (*flag) {
-      *flag = 0;
-      GCM_HIP(hipMemset(c->d_flags, 0, sizeof(int)));
+      *flag = 0;  // the next fill's prologue clears the device flag
       return fail(GCM_ERR_NUMERIC, "NaN event weight");
     }
   }

@@ -29,9 +29,15 @@ _FUNCS: Dict[str, Callable] = {
 
 
 def _to_python(src: str) -> str:
+    if "~" in src:
+        raise FormulaError(f"unsupported operator '~' in {src!r}")
     s = re.sub(r"\[([A-Za-z_][A-Za-z0-9_.]*)\]", lambda m: "__v_" + m.group(1).replace(".", "__dot__"), src)
     s = s.replace("&&", " and ").replace("||", " or ")
-    s = re.sub(r"!(?!=)", " not ", s)
+    # TFormula's '!' is a unary operator that binds tighter than arithmetic
+    # and comparisons ('!a == b' is '(!a) == b'); Python's unary '~' has that
+    # precedence (Python's 'not' binds looser than '=='), and the evaluator
+    # reads it as logical not
+    s = re.sub(r"!(?!=)", "~", s)
     return s
 
 
@@ -60,7 +66,7 @@ def evaluate(src: str, column: Callable[[str], np.ndarray], n: int) -> np.ndarra
                 return -v
             if isinstance(node.op, ast.UAdd):
                 return v
-            if isinstance(node.op, ast.Not):
+            if isinstance(node.op, (ast.Not, ast.Invert)):
                 return (v == 0).astype(np.float64)
         if isinstance(node, ast.BinOp):
             a, b = ev(node.left), ev(node.right)

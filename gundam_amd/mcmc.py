@@ -29,6 +29,7 @@ from typing import Callable, Optional
 
 import numpy as np
 
+from . import _native as N
 from .cache_manager import CacheManager
 from .surrogate import Surrogate, barlow_llh
 
@@ -42,6 +43,7 @@ class ExactTarget:
         self.table_values = table_values
         self.calls = 0
         self.points = 0
+        self.nan_batches = 0      # launches that reported a NaN event weight
         self.record = record      # keep (theta, bin contents) of every point
         self.rec_theta: list = []
         self.rec_sums: list = []
@@ -55,8 +57,16 @@ class ExactTarget:
             self.cm.set_parameters(th)
             if self.table_values is not None and self.cm.table_len:
                 self.cm.set_table_values(self.table_values(th))
-            self.cm.fill()
-            out[i:i + th.shape[0]] = -0.5 * self.cm.llh()[0][:th.shape[0]]
+            try:
+                self.cm.fill().get()
+            except N.GcmError as e:
+                # a NaN event weight (CacheWeights.cpp:83-86): the points it
+                # touches have a NaN likelihood, which the chains reject below
+                if e.code != N.GCM_ERR_NUMERIC:
+                    raise
+                self.nan_batches += 1
+            llh = self.cm.llh()[0][:th.shape[0]]
+            out[i:i + th.shape[0]] = np.where(np.isnan(llh), -np.inf, -0.5 * llh)
             if self.record:
                 self.rec_theta.append(th.copy())
                 self.rec_sums.append(np.stack([self.cm.hist(k)[0]

@@ -324,7 +324,20 @@ class Propagator:
         if not self.parameters:
             raise ConfigError("no enabled parameter")
 
+    # dial-set keys of the reference (DialCollection.cpp:30-60) that change
+    # which events a dial applies to or what it returns, and are not built
+    # here: a configuration that uses them is rejected rather than loaded with
+    # different weights
+    _UNSUPPORTED_DIAL_KEYS = ("parametersBinningPath", "binning", "binningFilePath",
+                              "definitionRange", "mirrorDefinitionRange", "dialsFilePath",
+                              "dialsList", "dialsTreePath", "dialsDefinitions", "tableConfig")
+
     def _add_dial_set(self, p: Parameter, dd: dict, p2: Optional[Parameter] = None):
+        bad = [k for k in self._UNSUPPORTED_DIAL_KEYS if k in dd]
+        if bad:
+            raise ConfigError(f"parameter {p.name}: dial-set key(s) {', '.join(bad)} "
+                              f"not supported (binned dials, definition ranges and "
+                              f"dial files are not built)")
         t = str(_get(dd, "dialType", "type", "dialsType", default="")).lower()
         leaf = _get(dd, "dialLeafName", "treeExpression")
         if (t == "surface") != (p2 is not None):
@@ -355,8 +368,9 @@ class Propagator:
                 raise ConfigError(f"parameter {p.name}: mirrorHighEdge < mirrorLowEdge")
             mir_lo, mir_r = lo, hi - lo
         flags = 0 if dd.get("allowDialExtrapolation", True) else N.INPUT_NO_EXTRAPOLATION
-        # DialCollection.cpp:453,915-917: minDialResponse defaults to 0
-        mn = dd.get("minDialResponse", 0.0)
+        # DialCollection.cpp:453,915-917: minDialResponse defaults to 0;
+        # minimumSplineResponse is its alias (DialCollection.cpp:42)
+        mn = _get(dd, "minDialResponse", "minimumSplineResponse", default=0.0)
         mx = dd.get("maxDialResponse", None)
         self._inputs.append((p.index, flags, mir_lo, mir_r,
                              nan if mn is None else float(mn), nan if mx is None else float(mx)))

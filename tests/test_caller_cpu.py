@@ -203,3 +203,52 @@ def test_config_errors(tmp_path):
     with pytest.raises(ConfigError):
         Propagator(bad, str(tmp_path), build=False, binner=pyoracle.bin_events)
     assert pc["dataSetList"][0]["name"] == "TestSample"
+
+
+def test_formula_not_binds_like_tformula():
+    """TFormula's '!' is a tight unary operator: '!a == b' is '(!a) == b' and
+    '!x > 0' is '(!x) > 0' (Python's 'not' would bind looser than both)."""
+    cols = {"a": np.array([0.0, 1.0, 2.0, 0.0]), "b": np.array([1.0, 0.0, 0.0, 0.0]),
+            "x": np.array([-1.0, 0.0, 3.0, 1.0])}
+    col = cols.__getitem__
+    np.testing.assert_array_equal(evaluate("!x > 0", col, 4), [0.0, 1.0, 0.0, 0.0])
+    np.testing.assert_array_equal(evaluate("!(x > 0)", col, 4), [1.0, 1.0, 0.0, 0.0])
+    np.testing.assert_array_equal(evaluate("![a] == [b]", col, 4), [1.0, 1.0, 1.0, 0.0])
+    np.testing.assert_array_equal(evaluate("![a] == 0", col, 4), [0.0, 1.0, 1.0, 0.0])
+    np.testing.assert_array_equal(evaluate("!a * 3", col, 4), [3.0, 0.0, 0.0, 3.0])
+    np.testing.assert_array_equal(evaluate("!!a", col, 4), [0.0, 1.0, 1.0, 0.0])
+    np.testing.assert_array_equal(evaluate("a != b", col, 4), [1.0, 1.0, 1.0, 0.0])
+    with pytest.raises(FormulaError):
+        evaluate("~a", col, 4)
+
+
+def _first_dial_set(cfg):
+    return cfg["fitterEngineConfig"]["propagatorConfig"]["parameterSetListConfig"][1][
+        "parameterDefinitions"][0]["dialSetDefinitions"][0]
+
+
+def test_min_spline_response_alias(tmp_path):
+    """minimumSplineResponse is read as minDialResponse (DialCollection.cpp:42)."""
+    path = caller_fixture.write(str(tmp_path), n_mc=50, n_data=10)
+    with open(path) as f:
+        cfg = yaml.safe_load(f)
+    ds = _first_dial_set(cfg)
+    del ds["minDialResponse"]
+    ds["minimumSplineResponse"] = 0.35
+    prop = Propagator(cfg, str(tmp_path), build=False, binner=pyoracle.bin_events)
+    assert prop.tables.inputs["min_response"][4] == 0.35
+
+
+@pytest.mark.parametrize("key,value", [
+    ("parametersBinningPath", "bins.txt"), ("binning", "bins.txt"),
+    ("binningFilePath", "bins.txt"), ("definitionRange", [-1.0, 1.0]),
+    ("mirrorDefinitionRange", [-1.0, 1.0]), ("dialsFilePath", "d.root")])
+def test_unsupported_dial_keys_are_rejected(tmp_path, key, value):
+    """Binned dials, definition ranges and dial files would change which
+    events a dial weights; a configuration using them is refused."""
+    path = caller_fixture.write(str(tmp_path), n_mc=50, n_data=10)
+    with open(path) as f:
+        cfg = yaml.safe_load(f)
+    _first_dial_set(cfg)[key] = value
+    with pytest.raises(ConfigError, match=key):
+        Propagator(cfg, str(tmp_path), build=False, binner=pyoracle.bin_events)

@@ -378,6 +378,19 @@ def test_nan_weight_is_reported_once():
     cm.set_parameters(np.array([2.0, 1.0]))
     cm.fill().get()
     np.testing.assert_array_equal(cm.hist(0)[0], [100.0, 50.0, 100.0, 50.0])
+    # a NaN fill that is never synchronised leaves nothing behind: the next
+    # evaluation clears the flag in-stream (eager and graph replay alike)
+    for graph in (False, True):
+        cm.enable_graph(graph)
+        cm.set_parameters(np.array([np.nan, 1.0]))
+        cm.fill()
+        cm.set_parameters(np.array([3.0, 1.0]))
+        cm.fill().get()
+        np.testing.assert_array_equal(cm.hist(0)[0], [150.0, 50.0, 150.0, 50.0])
+        cm.set_parameters(np.array([np.nan, 1.0]))
+        with pytest.raises(N.GcmError) as e:
+            cm.fill().get()
+        assert e.value.code == N.GCM_ERR_NUMERIC
     cm.close()
 
 
