@@ -21,11 +21,14 @@ def _rel(a, b):
     return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30)
 
 
-# The last two shapes take the split-K paths: at (300, 512, 600) every GEMM
-# of the step is split 2-5 ways along K; at (24, 64, 3000) the 3000-long
-# reduction of the output layer's input gradient is split sixteen ways.
+# (300, 512, 600) and (24, 64, 3000) take the split-K paths: at the first
+# every GEMM of the step is split 2-5 ways along K; at the second the
+# 3000-long reduction of the output layer's input gradient is split sixteen
+# ways.  (300, 512, 6000) is the 5M-event model's network (300 parameters,
+# 6000 bins) at the training minibatch of 256 rows.
 @pytest.mark.parametrize("shape,rows", [((20, 64, 40), 100), ((37, 96, 130), 257),
-                                        ((300, 512, 600), 256), ((24, 64, 3000), 256)])
+                                        ((300, 512, 600), 256), ((24, 64, 3000), 256),
+                                        ((300, 512, 6000), 256)])
 def test_mlp_forward_and_gradients_match_fp64(shape, rows):
     print("mlp grad check", shape, rows, flush=True)
     n_in, h, n_out = shape
