@@ -165,6 +165,7 @@ GCM_SIGNATURES = {
     "gcm_mlp_train": (C.c_int, [_P, C.c_int, C.c_int, C.c_float, C.c_uint64, _P]),
     "gcm_mlp_set_head": (C.c_int, [_P, _P, _P, _P, _P]),
     "gcm_mlp_predict_llh": (C.c_int, [_P, _P, _I64, _P]),
+    "gcm_mlp_predict_llh_noalloc": (C.c_int, [_P, _P, _I64, _P, _P]),
 }
 
 SYNTH_SIGNATURES = {

@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <string>
 #include <type_traits>
@@ -534,10 +535,41 @@ head_llh_kernel(const float* __restrict__ y, int n_out, const double* __restrict
 
 }  // namespace
 
+// Scratch of gcm_mlp_predict_llh_noalloc: the activations and split-K
+// partials of one batch live in this fixed array (one copy per GPU, held
+// until the process ends) instead of in memory the library allocates.  In
+// host code the name is not a device address: kernels get the address that
+// hipGetSymbolAddress returns for the current device (noalloc_scratch below).
+constexpr size_t kNoAllocScratchBytes = size_t(16) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device`, resolved once per device with
+// that device current.
+int noalloc_scratch(int device, unsigned char** out) {
+  if (device < 0 || device >= kMaxDevices) return fail(GCM_ERR_ARG, "device index too large");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(GCM_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<unsigned char*>(ptr);
+  }
+  *out = g_scratch_ptr[device];
+  return GCM_OK;
+}
+}  // namespace
+
 struct gcm_mlp {
   int device = 0;
   int n_in = 0, H = 0, n_out = 0;
   hipStream_t stream = nullptr;
+  unsigned char* scratch = nullptr;  // this device's noalloc scratch array
   int64_t n_params = 0;
   int64_t off_w1 = 0, off_b1 = 0, off_w2 = 0, off_b2 = 0, off_w3 = 0, off_b3 = 0;
   float *p = nullptr, *g = nullptr, *m1 = nullptr, *m2 = nullptr;
@@ -644,59 +676,96 @@ hipError_t launch_gemm(int sel, dim3 grid, hipStream_t stream, const GemmArgs& g
 // sooner; shorter reductions gain less than the extra reduction launch
 // costs); the partials go to a workspace and are summed in split order, so
 // the result does not depend on scheduling.
-int gemm(gcm_mlp* m, int M, int N, int K, const void* A, int64_t lda, int a_kc, int a_bf16,
-         const void* B, int64_t ldb, int b_kc, int b_bf16, float* C, int64_t ldc,
-         const float* bias, float* Z, const float* gz = nullptr) {
-  if (M <= 0 || N <= 0) return GCM_OK;
-  if (gz && ldc != N) return fail(GCM_ERR_ARG, "gemm: GELU backward needs a dense output");
+// Split count of an M x N x K product and its K steps per split.
+void gemm_splits(int M, int N, int K, int* splits_out, int* chunk_out) {
   const int tiles = ((M + TM - 1) / TM) * ((N + TN - 1) / TN);
   const int steps = std::max(1, (K + TK - 1) / TK);
   int splits = 1;
   if (tiles < 128 && steps >= kMinSplitSteps)
     splits = std::max(1, std::min({steps / 2, (1024 + tiles - 1) / tiles, kMaxSplit}));
   const int chunk_steps = (steps + splits - 1) / splits;
-  splits = (steps + chunk_steps - 1) / chunk_steps;
+  *splits_out = (steps + chunk_steps - 1) / chunk_steps;
+  *chunk_out = chunk_steps;
+}
+
+// Split-K partial floats an M x N x K product needs (0 when not split).
+size_t gemm_ws_floats(int M, int N, int K) {
+  int splits, chunk;
+  gemm_splits(M, N, K, &splits, &chunk);
+  return splits > 1 ? (size_t)splits * M * N : 0;
+}
+
+// Where a GEMM runs: a stream and a fixed split-K workspace that is never
+// grown (the allocation-free path).  Without one, the handle's stream and its
+// workspace, grown on demand.
+struct Exec {
+  hipStream_t stream;
+  float* ws;
+  size_t ws_cap;  // floats
+};
+
+int gemm(gcm_mlp* m, int M, int N, int K, const void* A, int64_t lda, int a_kc, int a_bf16,
+         const void* B, int64_t ldb, int b_kc, int b_bf16, float* C, int64_t ldc,
+         const float* bias, float* Z, const float* gz = nullptr, const Exec* ex = nullptr) {
+  if (M <= 0 || N <= 0) return GCM_OK;
+  if (gz && ldc != N) return fail(GCM_ERR_ARG, "gemm: GELU backward needs a dense output");
+  const hipStream_t stream = ex ? ex->stream : m->stream;
+  int splits, chunk_steps;
+  gemm_splits(M, N, K, &splits, &chunk_steps);
   GemmArgs g{M, N, K, A, lda, B, ldb, vec_ok(A, lda, a_kc, a_bf16, M) && vec_ok(B, ldb, b_kc, b_bf16, N),
              (int64_t)N * K * (b_bf16 ? 2 : 4) > (int64_t)M * K * (a_bf16 ? 2 : 4),
              C, ldc, 0, chunk_steps * TK, bias, Z, Z != nullptr};
   const int sel = (a_kc ? 8 : 0) | (a_bf16 ? 4 : 0) | (b_kc ? 2 : 0) | (b_bf16 ? 1 : 0);
   dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM, splits);
   if (splits == 1) {
-    MLP_HIP(launch_gemm(sel, grid, m->stream, g));
+    MLP_HIP(launch_gemm(sel, grid, stream, g));
     if (gz) {
-      gelu_backward_kernel<<<grid_for((int64_t)M * N), 256, 0, m->stream>>>(C, gz, (int64_t)M * N);
+      gelu_backward_kernel<<<grid_for((int64_t)M * N), 256, 0, stream>>>(C, gz, (int64_t)M * N);
       MLP_HIP(hipGetLastError());
     }
     return GCM_OK;
   }
   const int64_t mn = (int64_t)M * N;
-  int rc = m->ensure_ws((size_t)splits * mn);
-  if (rc) return rc;
-  g.C = m->ws;
+  float* ws = nullptr;
+  if (ex) {
+    if ((size_t)splits * mn > ex->ws_cap) return fail(GCM_ERR_ARG, "gemm: workspace too small");
+    ws = ex->ws;
+  } else {
+    int rc = m->ensure_ws((size_t)splits * mn);
+    if (rc) return rc;
+    ws = m->ws;
+  }
+  g.C = ws;
   g.ldc = N;
   g.split_stride = mn;
   g.bias = nullptr;
   g.Z = nullptr;
   g.gelu = 0;
-  MLP_HIP(launch_gemm(sel, grid, m->stream, g));
-  splitk_reduce_kernel<<<grid_for(mn), 256, 0, m->stream>>>(m->ws, splits, M, N, C, ldc, bias, Z,
-                                                            Z != nullptr, gz);
+  MLP_HIP(launch_gemm(sel, grid, stream, g));
+  splitk_reduce_kernel<<<grid_for(mn), 256, 0, stream>>>(ws, splits, M, N, C, ldc, bias, Z,
+                                                         Z != nullptr, gz);
   MLP_HIP(hipGetLastError());
   return GCM_OK;
 }
 
-// Forward on rows already in m->x: h1 = gelu(x W1^T + b1), h2 = gelu(h1 W2^T
-// + b2), y = h2 W3^T + b3.
-int forward(gcm_mlp* m, int rows) {
+// Forward: h1 = gelu(x W1^T + b1), h2 = gelu(h1 W2^T + b2), y = h2 W3^T +
+// b3, with the pre-activations in z1, z2.
+int forward(gcm_mlp* m, int rows, const float* x, float* z1, float* h1, float* z2, float* h2,
+            float* y, const Exec* ex) {
   const __bf16* w = m->pb;
-  int rc = gemm(m, rows, m->H, m->n_in, m->x, m->n_in, 1, 0, w, m->bmap.n_in_pad, 1, 1, m->h1,
-                m->H, m->p + m->off_b1, m->z1);
+  int rc = gemm(m, rows, m->H, m->n_in, x, m->n_in, 1, 0, w, m->bmap.n_in_pad, 1, 1, h1,
+                m->H, m->p + m->off_b1, z1, nullptr, ex);
   if (rc) return rc;
-  rc = gemm(m, rows, m->H, m->H, m->h1, m->H, 1, 0, w + m->bmap.bw2, m->H, 1, 1, m->h2, m->H,
-            m->p + m->off_b2, m->z2);
+  rc = gemm(m, rows, m->H, m->H, h1, m->H, 1, 0, w + m->bmap.bw2, m->H, 1, 1, h2, m->H,
+            m->p + m->off_b2, z2, nullptr, ex);
   if (rc) return rc;
-  return gemm(m, rows, m->n_out, m->H, m->h2, m->H, 1, 0, w + m->bmap.bw3, m->H, 1, 1, m->y,
-              m->n_out, m->p + m->off_b3, nullptr);
+  return gemm(m, rows, m->n_out, m->H, h2, m->H, 1, 0, w + m->bmap.bw3, m->H, 1, 1, y,
+              m->n_out, m->p + m->off_b3, nullptr, nullptr, ex);
+}
+
+// Forward on rows already in m->x, into the handle's buffers.
+int forward(gcm_mlp* m, int rows) {
+  return forward(m, rows, m->x, m->z1, m->h1, m->z2, m->h2, m->y, nullptr);
 }
 
 // Loss and gradients into m->g for the rows in m->x / m->t.
@@ -781,6 +850,7 @@ int gcm_mlp_create(int device, int n_in, int n_hidden, int n_out, uint64_t seed,
   step(m->alloc(&m->partial, (size_t)(n_out + 63) / 64));  // one per mse_colsum block
   if (rc == GCM_OK && hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess)
     rc = fail(GCM_ERR_HIP, "stream creation failed");
+  step(noalloc_scratch(device, &m->scratch));
   if (rc != GCM_OK) {
     for (void* q : m->allocs) (void)hipFree(q);
     delete m;
@@ -904,6 +974,50 @@ int gcm_mlp_predict_llh(gcm_mlp* m, const float* x, int64_t rows, double* chi2) 
     MLP_HIP(hipMemcpyAsync(chi2 + r0, m->chi2, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, m->stream));
   }
   MLP_HIP(hipStreamSynchronize(m->stream));
+  return GCM_OK;
+}
+
+// gcm_mlp_predict_llh without allocation: device input and output, every
+// intermediate in this GPU's `scratch` array (through the address resolved
+// at gcm_mlp_create), enqueued on `stream` without synchronising.  Layout of
+// the scratch, each part 256-byte aligned: z1, h1, z2, h2 [rows][H], y
+// [rows][n_out], then the split-K partials of the largest split GEMM.  The
+// same kernels with the same split decisions as gcm_mlp_predict_llh, so the
+// results are the same for the same rows.  Nothing in the scratch needs to
+// start at zero (every kernel writes what it later reads).
+int gcm_mlp_predict_llh_noalloc(gcm_mlp* m, const float* x_device, int64_t rows,
+                                double* chi2_device, void* stream_ptr) {
+  if (!m || rows < 0 || (rows && (!x_device || !chi2_device)))
+    return fail(GCM_ERR_ARG, "invalid arguments");
+  if (!m->head) return fail(GCM_ERR_STATE, "gcm_mlp_set_head first");
+  if (!m->scratch) return fail(GCM_ERR_STATE, "no scratch address for this device");
+  if (rows == 0) return GCM_OK;
+  auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+  const int r = (int)std::min<int64_t>(rows, INT32_MAX);
+  const size_t act = up((size_t)r * m->H * 4), out = up((size_t)r * m->n_out * 4);
+  const size_t ws = std::max({gemm_ws_floats(r, m->H, m->n_in), gemm_ws_floats(r, m->H, m->H),
+                              gemm_ws_floats(r, m->n_out, m->H)});
+  const size_t need = 4 * act + out + up(ws * 4);
+  if (rows > INT32_MAX || need > kNoAllocScratchBytes)
+    return fail(GCM_ERR_ARG, "gcm_mlp_predict_llh_noalloc: " + std::to_string(rows) +
+                                 " rows need " + std::to_string(need) + " bytes of scratch, more than " +
+                                 std::to_string(kNoAllocScratchBytes));
+  MLP_HIP(hipSetDevice(m->device));
+  unsigned char* base = m->scratch;
+  float* z1 = reinterpret_cast<float*>(base);
+  float* h1 = reinterpret_cast<float*>(base + act);
+  float* z2 = reinterpret_cast<float*>(base + 2 * act);
+  float* h2 = reinterpret_cast<float*>(base + 3 * act);
+  float* y = reinterpret_cast<float*>(base + 4 * act);
+  const Exec ex{stream_ptr ? (hipStream_t)stream_ptr : m->stream,
+                reinterpret_cast<float*>(base + 4 * act + out), ws};
+  int rc = forward(m, r, x_device, z1, h1, z2, h2, y, &ex);
+  if (rc) return rc;
+  const double* h = m->head;
+  const int64_t no = m->n_out;
+  head_llh_kernel<<<r, kHeadBlock, 0, ex.stream>>>(y, m->n_out, h, h + no, h + 2 * no, h + 3 * no,
+                                                   chi2_device);
+  MLP_HIP(hipGetLastError());
   return GCM_OK;
 }
 

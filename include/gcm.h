@@ -358,6 +358,21 @@ int gcm_mlp_train(gcm_mlp* m, int epochs, int batch, float lr, uint64_t seed,
 int gcm_mlp_set_head(gcm_mlp* m, const double* mean, const double* scale,
                      const double* data, const double* var_ratio);
 int gcm_mlp_predict_llh(gcm_mlp* m, const float* x, int64_t rows, double* chi2);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under hipGraph stream capture, tight serving and MCMC loops):
+ * gcm_mlp_predict_llh on device arrays -- x_device [rows][n_in] floats,
+ * chi2_device [rows] doubles -- enqueued on `stream` (a hipStream_t; NULL:
+ * the handle's stream) without synchronising, and without allocating: every
+ * intermediate (activations, split-K partials) lives in one fixed 16 MiB
+ * device array per GPU, held until the process ends.  The rows must fit in
+ * it: 4 x (4 n_hidden + n_out) bytes per row (each part rounded up to
+ * 256 B) plus the split-K partials of long reductions; larger calls return
+ * GCM_ERR_ARG (about 500 rows for 300 -> 512 -> 512 -> 6000).  Results equal
+ * gcm_mlp_predict_llh's for the same rows.  All calls on one GPU share the
+ * array: calls that overlap in time on one GPU (other streams, other
+ * handles, other threads) race; serialise them. */
+int gcm_mlp_predict_llh_noalloc(gcm_mlp* m, const float* x_device, int64_t rows,
+                                double* chi2_device, void* stream);
 
 #ifdef __cplusplus
 }
