@@ -33,7 +33,7 @@ def build_gcm() -> str:
     out = os.path.join(HERE, "libgcm.so")
     tmp = out + ".tmp"
     _run([HIPCC, *HIP_FLAGS, "-o", tmp, os.path.join(CSRC, "gcm.hip"),
-          os.path.join(CSRC, "mlp.hip")])
+          os.path.join(CSRC, "gcm_points.hip"), os.path.join(CSRC, "mlp.hip")])
     os.replace(tmp, out)
     return out
 

@@ -314,12 +314,16 @@ struct TableEntry<float> {
 };
 
 // Evaluate one dial.  x is the mirrored input value; flags are the input's
-// GCM_INPUT_* flags; tab points at this point's table vector.  Returns the
-// raw response, before the supervisor clamp.
-template <typename R>
+// GCM_INPUT_* flags; the point's table value at slot s is tab[s] (the
+// point's own table vector), or with kInterleaved tab[s * k_stride] (the
+// point-interleaved tables of a K-point launch, k_stride = K).  Returns the raw response, before the supervisor
+// clamp.
+template <typename R, bool kInterleaved = false>
 __device__ __forceinline__ double eval_dial(int type, int n, const R* d,
                                            double x_in, int flags,
-                                           const double* __restrict__ tab) {
+                                           const double* __restrict__ tab,
+                                           int64_t k_stride = 1) {
+  const int64_t tstride = kInterleaved ? k_stride : 1;
   R x = (R)x_in;
   const bool no_extrap = (flags & GCM_INPUT_NO_EXTRAPOLATION) != 0;
   switch (type) {
@@ -371,9 +375,15 @@ __device__ __forceinline__ double eval_dial(int type, int n, const R* d,
       TableEntry<R>::get(d, 0, &slot, &f);
       if (sizeof(R) == 8) {
         const double fd = (double)f;
-        return fd * tab[slot] + (1.0 - fd) * tab[slot + 1];
+        if constexpr (kInterleaved)
+          return fd * tab[slot * tstride] + (1.0 - fd) * tab[(slot + 1) * tstride];
+        else
+          return fd * tab[slot] + (1.0 - fd) * tab[slot + 1];
       }
-      return (double)(f * (float)tab[slot] + (1.0f - f) * (float)tab[slot + 1]);
+      if constexpr (kInterleaved)
+        return (double)(f * (float)tab[slot * tstride] + (1.0f - f) * (float)tab[(slot + 1) * tstride]);
+      else
+        return (double)(f * (float)tab[slot] + (1.0f - f) * (float)tab[slot + 1]);
     }
     case GCM_DIAL_KRIGED: {  // Kriged.cpp:14-21
       // The table gathers are issued eight at a time (they hit L2 or HBM);
@@ -389,7 +399,10 @@ __device__ __forceinline__ double eval_dial(int type, int n, const R* d,
           float w = 0.0f;
           if (j0 + q < n) TableEntry<R>::get(d, j0 + q, &slot, &w);
           wv[q] = w;
-          tv[q] = (j0 + q < n) ? tab[slot] : 0.0;
+          if constexpr (kInterleaved)
+            tv[q] = (j0 + q < n) ? tab[(int64_t)slot * tstride] : 0.0;
+          else
+            tv[q] = (j0 + q < n) ? tab[slot] : 0.0;
         }
 #pragma unroll
         for (int q = 0; q < 8; ++q) {

@@ -1,0 +1,328 @@
+// fill_common.hpp -- definitions shared by the device code of gcm.hip (one
+// point per lane of work: fill_kernel) and gcm_points.hip (one lane per
+// point: fill_points_kernel): the device layout of events, tiles and input
+// records, the fill arguments, the LDS-DMA helpers, and the per-bin
+// likelihood.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <string>
+
+#include "../../include/gcm.h"
+
+namespace gcm {
+
+int fail(int code, const std::string& msg);  // gcm.hip
+
+#define GCM_HIP(call)                                                        \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return fail(GCM_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int kWave = 64;
+constexpr int kMaxTileEvents = 64;
+// Launches of at least kPointsMin points evaluate with one lane per point
+// (fill_points_kernel); each tile's events are split into kRanges contiguous
+// ranges, one per evaluator wave, each with its own partial slots.
+constexpr int kPointsMin = 16;
+constexpr int kRanges = 8;
+constexpr int kDefaultBlock = 256;
+constexpr int kDefaultTileWords = 2048;  // 16 KiB of records per tile
+constexpr int kDefaultBlocksPerCU = 3;
+constexpr int kFinalizeBlock = 256;
+constexpr int kTotalBlock = 1024;  // threads of the per-point LLH sum
+constexpr int kFinalizeMaxBlocks = 8192;  // waves in flight for the per-bin pass
+constexpr size_t kInputsLdsMax = 16384;  // stage input records in LDS up to this
+constexpr size_t kLdsPerCU = 160 * 1024;   // gfx950 LDS per compute unit
+constexpr size_t kHeaderBytes = 32;      // tile header copy in each LDS buffer
+constexpr int kMaxDynamicLds = 160 * 1024;  // gfx950 LDS per CU
+
+// One binned event, in bin-sorted order: base weight, global bin, and its
+// dials as (first event-major slot in the tile, count).
+struct EventRec {
+  double base;
+  int32_t bin;
+  uint32_t dials;  // slot | count << 16
+};
+static_assert(sizeof(EventRec) == 16, "event record is 16 bytes");
+
+struct TileHdr {
+  uint32_t first_event;
+  uint32_t first_dial;
+  uint64_t rec_begin;  // in record words of the precision's type
+  uint32_t rec_words;
+  uint16_t n_events;
+  uint16_t n_dials;
+  uint32_t first_partial;
+  uint32_t grid_begin;  // first 2-D dial in the tile's type-sorted order
+};
+static_assert(sizeof(TileHdr) == 32, "tile header is 32 bytes");
+
+// Per-point record of one dial input: the mirrored value and the response
+// supervisor bounds (unset bounds are -+inf).
+struct InputRec {
+  double x;
+  double lo;
+  double hi;
+  int32_t flags;
+  int32_t pad;
+};
+static_assert(sizeof(InputRec) == 32, "input record is 32 bytes");
+
+// Response supervisor and flags shared by the inputs of one class.  Dial
+// descriptors carry their input's class (4 bits), so that when the inputs'
+// x values are staged in LDS an evaluator lane reads x and its class record
+// from LDS and nothing from global memory; class kEscapeClass (workloads
+// with more than kInputClasses - 1 distinct (lo, hi, flags)) and the
+// unstaged case read the input's full record from global memory.
+struct ClassRec {
+  double lo;
+  double hi;
+  int32_t flags;
+  int32_t pad;
+  double pad2;
+};
+static_assert(sizeof(ClassRec) == 32, "class record is 32 bytes");
+constexpr int kInputClasses = 16;
+constexpr int kEscapeClass = kInputClasses - 1;
+constexpr size_t kClassBytes = kInputClasses * sizeof(ClassRec);
+
+// Knuth two-sum: s + e == a + b exactly.
+__device__ __forceinline__ void two_sum(double a, double b, double& s,
+                                        double& e) {
+  s = a + b;
+  const double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+
+struct FillArgs {
+  const ClassRec* cls;  // [kInputClasses]
+  const TileHdr* tiles;
+  int64_t n_tiles;
+  const uint64_t* desc;
+  const void* rec;
+  const EventRec* ev;
+  const int32_t* ev_result;
+  const InputRec* xin;
+  int n_inputs;
+  const double* tab;
+  int64_t table_len;
+  double cap;
+  int k_points;
+  double* psum;
+  double* psum2;
+  double* pcomp;
+  double* pcomp2;
+  int64_t n_partials;
+  double* weights_out;
+  int* flags;
+  int rec_bytes;   // LDS record capacity per buffer (multiple of 16 bytes)
+  int max_dials;   // LDS descriptor/response capacity (multiple of 128)
+  int inputs_lds;  // input records staged in LDS (K * n_inputs * 32 bytes)
+};
+
+// Cache policy of the tile stream: non-temporal, so the streamed records do
+// not evict the per-point lookup tables and input records from L2.
+#ifndef GCM_STREAM_POLICY
+#define GCM_STREAM_POLICY 2
+#endif
+constexpr int kStreamPolicy = GCM_STREAM_POLICY;
+
+typedef __attribute__((address_space(3))) void* lds_void_ptr;
+typedef __attribute__((address_space(1))) void* global_void_ptr;
+
+// Block barrier that waits only for this wave's LDS traffic (not for its
+// global stores or, in the producer, for anything it has not waited on).
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+// Producer wave: copy `bytes` (a multiple of 16) from global memory straight
+// into LDS with 16-byte-per-lane DMA loads; lane i of a chunk lands at
+// dst + chunk + 16 i, and lanes past the end are masked off.
+__device__ __forceinline__ void dma_to_lds(const unsigned char* src,
+                                           unsigned char* dst, int bytes,
+                                           int lane) {
+  for (int j = 0; j < bytes; j += 1024) {
+    if (j + lane * 16 < bytes)
+      __builtin_amdgcn_global_load_lds((global_void_ptr)(src + j + lane * 16),
+                                       (lds_void_ptr)(dst + j), 16, 0, kStreamPolicy);
+  }
+}
+
+template <typename R>
+__device__ __forceinline__ void dma_tile(const TileHdr& h, const unsigned char* rec,
+                                         const unsigned char* desc,
+                                         const EventRec* ev, unsigned char* buf,
+                                         int rec_bytes, int max_dials, int lane) {
+  dma_to_lds(rec + h.rec_begin * sizeof(R), buf,
+             (int)((h.rec_words * sizeof(R) + 15) / 16 * 16), lane);
+  dma_to_lds(desc + (size_t)h.first_dial * 8, buf + rec_bytes,
+             (h.n_dials + 1) / 2 * 16, lane);
+  dma_to_lds(reinterpret_cast<const unsigned char*>(ev + h.first_event),
+             buf + rec_bytes + (size_t)max_dials * 8, h.n_events * 16, lane);
+}
+
+// The tile header travels with the tile (evaluators and the event wave read
+// it from LDS instead of waiting on a global load after each barrier).
+__device__ __forceinline__ void dma_header(const TileHdr* hdr, unsigned char* buf,
+                                           size_t hdr_off, int lane) {
+  dma_to_lds(reinterpret_cast<const unsigned char*>(hdr), buf + hdr_off,
+             (int)sizeof(TileHdr), lane);
+}
+
+struct LlhParams {
+  int32_t type;
+  int32_t use_poisson;
+  double frac_limit;
+  double expected_min;
+  int32_t lsq_poisson;
+  int32_t no_update_weights;
+  const double* nominal_err;     // [n_bins] or null
+  const double* extra_fraction;  // [n_bins] or null
+  const uint8_t* ignore;         // [n_bins] or null
+};
+
+// Per-bin -2 ln L of bin b from (data, sum w, sqrt(sum w^2)) as
+// JointProbabilityBase::eval(SamplePair, bin) hands them over
+// (JointProbabilityBase.h:37-43).  Where the reference throws (LogThrowIf)
+// the result is NaN.
+__device__ inline double llh_bin(const LlhParams& c, int b, double data, double pred,
+                          double err) {
+  const double inf = __builtin_huge_val();
+  const double nan = __builtin_nan("");
+  const double dmin = 2.2250738585072014e-308;  // numeric_limits<double>::min
+  const double deps = 2.220446049250313e-16;    // numeric_limits<double>::epsilon
+  // JointProbabilityBase.h:74-77: bins flagged zero at the prior
+  if (c.ignore && c.ignore[b]) return 0.0;
+  switch (c.type) {
+    case GCM_LLH_POISSON:  // PoissonLogLikelihood.h:19-53
+      if (pred <= 0) return inf;
+      if (data <= 0) return 2.0 * pred;
+      return 2.0 * (pred - data + data * log(data / pred));
+    case GCM_LLH_BARLOW_OA2021: {  // BarlowBeestonBanff2022.h:92-294
+      double mcuncert = (c.no_update_weights && c.nominal_err) ? c.nominal_err[b] : err;
+      mcuncert *= mcuncert;
+      if (!isfinite(mcuncert) || mcuncert < 0.0) return inf;
+      if (pred <= c.expected_min) {
+        if (pred <= 0.0 && data > 0.0) return inf;
+        return 0.0;
+      }
+      double newmc = pred;
+      double penalty = 0.0;
+      if (mcuncert > 0.0) {
+        const double fractional = fmin(sqrt(mcuncert) / pred, c.frac_limit);
+        const double temp = pred * fractional * fractional - 1;
+        const double temp2 = temp * temp + 4 * data * fractional * fractional;
+        if (temp2 < 0) return inf;
+        const double beta = (-1 * temp + sqrt(temp2)) / 2.;
+        newmc = fmax(pred * beta, dmin);
+        penalty = (beta - 1) * (beta - 1) / (2 * fractional * fractional);
+      }
+      double stat = 0;
+      if (data <= deps) {
+        stat = newmc;
+      } else if (newmc > dmin) {
+        stat = newmc - data + data * (log(data) - log(newmc));
+      } else {
+        stat = dmin - data + data * (log(data) - log(dmin));
+      }
+      double chisq = 2.0 * stat;
+      if (!c.use_poisson) chisq += 2.0 * penalty;
+      return chisq;
+    }
+    case GCM_LLH_BARLOW_BEESTON: {  // BarlowBeeston.h:33-62 (err as given)
+      const double dataVal = fmax(data, 0.0);
+      const double predVal = fmax(pred, sqrt(dmin));
+      const double mcUncert = fmax(err, dmin);
+      const double rel_var = mcUncert / (predVal * predVal);
+      const double bq = (predVal * rel_var) - 1;
+      const double cq = 4 * dataVal * rel_var;
+      const double beta = (-bq + sqrt(bq * bq + cq)) / 2.0;
+      const double mc_hat = predVal * beta;
+      double chi2 = 0.0;
+      if (dataVal <= 0.0) {
+        chi2 = 2 * mc_hat;
+        chi2 += (beta - 1) * (beta - 1) / rel_var;
+      } else {
+        chi2 = 2 * (mc_hat - dataVal);
+        if (dataVal > 0.0) chi2 += 2 * dataVal * (log(dataVal) - log(mc_hat));
+        chi2 += (beta - 1) * (beta - 1) / rel_var;
+      }
+      return chi2;
+    }
+    case GCM_LLH_BARLOW_OA2020:      // BarlowBeestonBanff2020.h:18-85
+    case GCM_LLH_BARLOW_OA2021_SFGD: {  // BarlowBeestonBanff2022Sfgd.h:20-134
+      double fractional;
+      if (c.type == GCM_LLH_BARLOW_OA2020) {
+        fractional = sqrt(err) / pred;
+      } else {
+        const double sfg = c.extra_fraction ? err + pred * c.extra_fraction[b] : err;
+        fractional = sfg / pred;
+      }
+      const double temp = pred * fractional * fractional - 1;
+      const double temp2 = temp * temp + 4 * data * fractional * fractional;
+      if (temp2 < 0) return nan;
+      const double beta = (-1 * temp + sqrt(temp2)) / 2.;
+      const double newmc = pred * beta;
+      const double penalty = (beta - 1) * (beta - 1) / (2 * fractional * fractional);
+      double stat = 0;
+      if (data == 0) stat = newmc;
+      else if (newmc > 0) stat = newmc - data + data * log(data / newmc);
+      double chisq = 0.;
+      if (c.type == GCM_LLH_BARLOW_OA2020 && isnan(penalty)) return inf;
+      if (pred > 0.0 && data > 0.0) chisq += 2.0 * (stat + penalty);
+      else if (pred > 0.0) chisq += 2.0 * (stat + penalty);
+      if (c.type == GCM_LLH_BARLOW_OA2020 && isnan(chisq)) return nan;
+      return chisq;
+    }
+    case GCM_LLH_CHI_SQUARED: {  // ChiSquared.h:19-29
+      if (pred == 0) return inf;
+      const double d = pred - data;
+      return d * d / pred;
+    }
+    case GCM_LLH_LEAST_SQUARES: {  // LeastSquares.h:45-52
+      double v = data - pred;
+      v = v * v;
+      if (c.lsq_poisson && data > 1.0) v /= 0.5 * data;
+      return v;
+    }
+    default:
+      return nan;
+  }
+}
+
+// Wave-level sum of (s, c) pairs with a fixed butterfly shape: the result is
+// the same on every run.  With kComp the pairs are combined by two-sum.
+// ---- the K-point path (gcm_points.hip) ----------------------------------
+struct PointsArgs {
+  const double* xT;         // [n_inputs][K]
+  const double* tabT;       // [table_len][K]
+  const uint32_t* tile_pr;  // [n_tiles] first range partial slot of each tile
+  double* prs;              // [n_pr][K] sums of w
+  double* prs2;             // [n_pr][K] sums of w^2
+  double* prc;              // [n_pr][K] compensations (FP32 mode)
+  double* prc2;
+};
+
+// LDS bytes of one fill_points_kernel block for the tile layout in a.
+size_t points_lds_bytes(const FillArgs& a, int nb);
+// Allow fill_points_kernel its dynamic LDS (once per process is enough).
+int points_set_lds_attrs();
+// Enqueue fill_points_kernel over the tiles; finalize_points_kernel over the
+// bins; transpose_tables_kernel (tab [K][len] -> tabT [len][K]).
+int launch_points(const FillArgs& a, const PointsArgs& pa, bool fp32, bool grid, int nb,
+                  int blocks, size_t lds, hipStream_t s);
+int launch_finalize_points(const PointsArgs& pa, bool fp32, const uint32_t* pbeg, int n_bins,
+                           int k_points, const double* data, const LlhParams& cfg,
+                           double* hist, double* hist2, double* llh_terms, hipStream_t s);
+int launch_transpose_tables(const double* tab, int64_t len, int k_points, double* tabT,
+                            hipStream_t s);
+
+}  // namespace gcm

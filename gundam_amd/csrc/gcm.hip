@@ -41,42 +41,9 @@
 #include "../../include/gcm.h"
 #include "../../include/gcm_table_update.h"
 #include "dial_eval.hpp"
+#include "fill_common.hpp"
 
 namespace gcm {
-
-constexpr int kWave = 64;
-constexpr int kMaxTileEvents = 64;
-constexpr int kDefaultBlock = 256;
-constexpr int kDefaultTileWords = 2048;  // 16 KiB of records per tile
-constexpr int kDefaultBlocksPerCU = 3;
-constexpr int kFinalizeBlock = 256;
-constexpr int kTotalBlock = 1024;  // threads of the per-point LLH sum
-constexpr int kFinalizeMaxBlocks = 8192;  // waves in flight for the per-bin pass
-constexpr size_t kInputsLdsMax = 16384;  // stage input records in LDS up to this
-constexpr size_t kLdsPerCU = 160 * 1024;   // gfx950 LDS per compute unit
-constexpr size_t kHeaderBytes = 32;      // tile header copy in each LDS buffer
-constexpr int kMaxDynamicLds = 160 * 1024;  // gfx950 LDS per CU
-
-// One binned event, in bin-sorted order: base weight, global bin, and its
-// dials as (first event-major slot in the tile, count).
-struct EventRec {
-  double base;
-  int32_t bin;
-  uint32_t dials;  // slot | count << 16
-};
-static_assert(sizeof(EventRec) == 16, "event record is 16 bytes");
-
-struct TileHdr {
-  uint32_t first_event;
-  uint32_t first_dial;
-  uint64_t rec_begin;  // in record words of the precision's type
-  uint32_t rec_words;
-  uint16_t n_events;
-  uint16_t n_dials;
-  uint32_t first_partial;
-  uint32_t grid_begin;  // first 2-D dial in the tile's type-sorted order
-};
-static_assert(sizeof(TileHdr) == 32, "tile header is 32 bytes");
 
 thread_local std::string g_error;
 
@@ -85,45 +52,10 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
-#define GCM_HIP(call)                                                        \
-  do {                                                                       \
-    hipError_t e_ = (call);                                                  \
-    if (e_ != hipSuccess)                                                    \
-      return fail(GCM_ERR_HIP, std::string(#call ": ") + hipGetErrorString(e_)); \
-  } while (0)
 
 // ---------------------------------------------------------------------------
 // Kernels
 // ---------------------------------------------------------------------------
-
-// Per-point record of one dial input: the mirrored value and the response
-// supervisor bounds (unset bounds are -+inf).
-struct InputRec {
-  double x;
-  double lo;
-  double hi;
-  int32_t flags;
-  int32_t pad;
-};
-static_assert(sizeof(InputRec) == 32, "input record is 32 bytes");
-
-// Response supervisor and flags shared by the inputs of one class.  Dial
-// descriptors carry their input's class (4 bits), so that when the inputs'
-// x values are staged in LDS an evaluator lane reads x and its class record
-// from LDS and nothing from global memory; class kEscapeClass (workloads
-// with more than kInputClasses - 1 distinct (lo, hi, flags)) and the
-// unstaged case read the input's full record from global memory.
-struct ClassRec {
-  double lo;
-  double hi;
-  int32_t flags;
-  int32_t pad;
-  double pad2;
-};
-static_assert(sizeof(ClassRec) == 32, "class record is 32 bytes");
-constexpr int kInputClasses = 16;
-constexpr int kEscapeClass = kInputClasses - 1;
-constexpr size_t kClassBytes = kInputClasses * sizeof(ClassRec);
 
 constexpr int kPenaltyChunk = 256;
 constexpr int kTableUpdateBlock = 128;  // threads per (job, point) table update
@@ -147,7 +79,8 @@ __device__ __forceinline__ double input_value(const double* th, const int32_t* i
   return v;
 }
 
-// Per point k (one block each, 256 threads): mirrored input values, the
+// Per point k (one block each, 256 threads): mirrored input values (and, for
+// K-point launches of fill_points_kernel, their input-major copy xT), the
 // inputs of the device table-update jobs (tab_in[k][j] = value of input
 // tab_idx[j]) and the prior penalty.
 __global__ void prologue_kernel(const double* __restrict__ theta, int n_par,
@@ -162,7 +95,8 @@ __global__ void prologue_kernel(const double* __restrict__ theta, int n_par,
                                 double* __restrict__ tab_in,
                                 const double* __restrict__ prior_mean,
                                 const double* __restrict__ prior_sigma,
-                                double* __restrict__ penalty, int* __restrict__ flags) {
+                                double* __restrict__ penalty, int* __restrict__ flags,
+                                double* __restrict__ xT) {
   const int k = blockIdx.x;
   const double* th = theta + (size_t)k * n_par;
   // the NaN-weight flag reports this evaluation only (cleared in-stream, so
@@ -179,6 +113,7 @@ __global__ void prologue_kernel(const double* __restrict__ theta, int n_par,
     r.flags = in_flags[i];
     r.pad = 0;
     xin[(size_t)k * n_inputs + i] = r;
+    if (xT) xT[(size_t)i * gridDim.x + k] = v;  // input-major copy for fill_points_kernel
   }
   // LikelihoodInterface.cpp:186-206 with a diagonal covariance: the terms are
   // formed in parallel and summed in parameter order by one thread.
@@ -265,90 +200,6 @@ __global__ void penalty_sets_kernel(const double* __restrict__ theta, int n_par,
   if (threadIdx.x == 0) penalty[k] = pen;
 }
 
-
-// Knuth two-sum: s + e == a + b exactly.
-__device__ __forceinline__ void two_sum(double a, double b, double& s,
-                                        double& e) {
-  s = a + b;
-  const double bb = s - a;
-  e = (a - (s - bb)) + (b - bb);
-}
-
-struct FillArgs {
-  const ClassRec* cls;  // [kInputClasses]
-  const TileHdr* tiles;
-  int64_t n_tiles;
-  const uint64_t* desc;
-  const void* rec;
-  const EventRec* ev;
-  const int32_t* ev_result;
-  const InputRec* xin;
-  int n_inputs;
-  const double* tab;
-  int64_t table_len;
-  double cap;
-  int k_points;
-  double* psum;
-  double* psum2;
-  double* pcomp;
-  double* pcomp2;
-  int64_t n_partials;
-  double* weights_out;
-  int* flags;
-  int rec_bytes;   // LDS record capacity per buffer (multiple of 16 bytes)
-  int max_dials;   // LDS descriptor/response capacity (multiple of 128)
-  int inputs_lds;  // input records staged in LDS (K * n_inputs * 32 bytes)
-};
-
-// Cache policy of the tile stream: non-temporal, so the streamed records do
-// not evict the per-point lookup tables and input records from L2.
-#ifndef GCM_STREAM_POLICY
-#define GCM_STREAM_POLICY 2
-#endif
-constexpr int kStreamPolicy = GCM_STREAM_POLICY;
-
-typedef __attribute__((address_space(3))) void* lds_void_ptr;
-typedef __attribute__((address_space(1))) void* global_void_ptr;
-
-// Block barrier that waits only for this wave's LDS traffic (not for its
-// global stores or, in the producer, for anything it has not waited on).
-__device__ __forceinline__ void lds_barrier() {
-  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
-}
-
-// Producer wave: copy `bytes` (a multiple of 16) from global memory straight
-// into LDS with 16-byte-per-lane DMA loads; lane i of a chunk lands at
-// dst + chunk + 16 i, and lanes past the end are masked off.
-__device__ __forceinline__ void dma_to_lds(const unsigned char* src,
-                                           unsigned char* dst, int bytes,
-                                           int lane) {
-  for (int j = 0; j < bytes; j += 1024) {
-    if (j + lane * 16 < bytes)
-      __builtin_amdgcn_global_load_lds((global_void_ptr)(src + j + lane * 16),
-                                       (lds_void_ptr)(dst + j), 16, 0, kStreamPolicy);
-  }
-}
-
-template <typename R>
-__device__ __forceinline__ void dma_tile(const TileHdr& h, const unsigned char* rec,
-                                         const unsigned char* desc,
-                                         const EventRec* ev, unsigned char* buf,
-                                         int rec_bytes, int max_dials, int lane) {
-  dma_to_lds(rec + h.rec_begin * sizeof(R), buf,
-             (int)((h.rec_words * sizeof(R) + 15) / 16 * 16), lane);
-  dma_to_lds(desc + (size_t)h.first_dial * 8, buf + rec_bytes,
-             (h.n_dials + 1) / 2 * 16, lane);
-  dma_to_lds(reinterpret_cast<const unsigned char*>(ev + h.first_event),
-             buf + rec_bytes + (size_t)max_dials * 8, h.n_events * 16, lane);
-}
-
-// The tile header travels with the tile (evaluators and the event wave read
-// it from LDS instead of waiting on a global load after each barrier).
-__device__ __forceinline__ void dma_header(const TileHdr* hdr, unsigned char* buf,
-                                           size_t hdr_off, int lane) {
-  dma_to_lds(reinterpret_cast<const unsigned char*>(hdr), buf + hdr_off,
-             (int)sizeof(TileHdr), lane);
-}
 
 // The fill kernel.  Wave roles in a block of C + NB waves:
 //   waves [0, C)   evaluate every dial of the current tile (type-sorted
@@ -589,129 +440,6 @@ static const void* fill_fn(bool fp32, bool inputs_lds, int nb) {
                                : (const void*)fill_kernel<double, false, false, 2, kGrid>);
 }
 
-struct LlhParams {
-  int32_t type;
-  int32_t use_poisson;
-  double frac_limit;
-  double expected_min;
-  int32_t lsq_poisson;
-  int32_t no_update_weights;
-  const double* nominal_err;     // [n_bins] or null
-  const double* extra_fraction;  // [n_bins] or null
-  const uint8_t* ignore;         // [n_bins] or null
-};
-
-// Per-bin -2 ln L of bin b from (data, sum w, sqrt(sum w^2)) as
-// JointProbabilityBase::eval(SamplePair, bin) hands them over
-// (JointProbabilityBase.h:37-43).  Where the reference throws (LogThrowIf)
-// the result is NaN.
-__device__ double llh_bin(const LlhParams& c, int b, double data, double pred,
-                          double err) {
-  const double inf = __builtin_huge_val();
-  const double nan = __builtin_nan("");
-  const double dmin = 2.2250738585072014e-308;  // numeric_limits<double>::min
-  const double deps = 2.220446049250313e-16;    // numeric_limits<double>::epsilon
-  // JointProbabilityBase.h:74-77: bins flagged zero at the prior
-  if (c.ignore && c.ignore[b]) return 0.0;
-  switch (c.type) {
-    case GCM_LLH_POISSON:  // PoissonLogLikelihood.h:19-53
-      if (pred <= 0) return inf;
-      if (data <= 0) return 2.0 * pred;
-      return 2.0 * (pred - data + data * log(data / pred));
-    case GCM_LLH_BARLOW_OA2021: {  // BarlowBeestonBanff2022.h:92-294
-      double mcuncert = (c.no_update_weights && c.nominal_err) ? c.nominal_err[b] : err;
-      mcuncert *= mcuncert;
-      if (!isfinite(mcuncert) || mcuncert < 0.0) return inf;
-      if (pred <= c.expected_min) {
-        if (pred <= 0.0 && data > 0.0) return inf;
-        return 0.0;
-      }
-      double newmc = pred;
-      double penalty = 0.0;
-      if (mcuncert > 0.0) {
-        const double fractional = fmin(sqrt(mcuncert) / pred, c.frac_limit);
-        const double temp = pred * fractional * fractional - 1;
-        const double temp2 = temp * temp + 4 * data * fractional * fractional;
-        if (temp2 < 0) return inf;
-        const double beta = (-1 * temp + sqrt(temp2)) / 2.;
-        newmc = fmax(pred * beta, dmin);
-        penalty = (beta - 1) * (beta - 1) / (2 * fractional * fractional);
-      }
-      double stat = 0;
-      if (data <= deps) {
-        stat = newmc;
-      } else if (newmc > dmin) {
-        stat = newmc - data + data * (log(data) - log(newmc));
-      } else {
-        stat = dmin - data + data * (log(data) - log(dmin));
-      }
-      double chisq = 2.0 * stat;
-      if (!c.use_poisson) chisq += 2.0 * penalty;
-      return chisq;
-    }
-    case GCM_LLH_BARLOW_BEESTON: {  // BarlowBeeston.h:33-62 (err as given)
-      const double dataVal = fmax(data, 0.0);
-      const double predVal = fmax(pred, sqrt(dmin));
-      const double mcUncert = fmax(err, dmin);
-      const double rel_var = mcUncert / (predVal * predVal);
-      const double bq = (predVal * rel_var) - 1;
-      const double cq = 4 * dataVal * rel_var;
-      const double beta = (-bq + sqrt(bq * bq + cq)) / 2.0;
-      const double mc_hat = predVal * beta;
-      double chi2 = 0.0;
-      if (dataVal <= 0.0) {
-        chi2 = 2 * mc_hat;
-        chi2 += (beta - 1) * (beta - 1) / rel_var;
-      } else {
-        chi2 = 2 * (mc_hat - dataVal);
-        if (dataVal > 0.0) chi2 += 2 * dataVal * (log(dataVal) - log(mc_hat));
-        chi2 += (beta - 1) * (beta - 1) / rel_var;
-      }
-      return chi2;
-    }
-    case GCM_LLH_BARLOW_OA2020:      // BarlowBeestonBanff2020.h:18-85
-    case GCM_LLH_BARLOW_OA2021_SFGD: {  // BarlowBeestonBanff2022Sfgd.h:20-134
-      double fractional;
-      if (c.type == GCM_LLH_BARLOW_OA2020) {
-        fractional = sqrt(err) / pred;
-      } else {
-        const double sfg = c.extra_fraction ? err + pred * c.extra_fraction[b] : err;
-        fractional = sfg / pred;
-      }
-      const double temp = pred * fractional * fractional - 1;
-      const double temp2 = temp * temp + 4 * data * fractional * fractional;
-      if (temp2 < 0) return nan;
-      const double beta = (-1 * temp + sqrt(temp2)) / 2.;
-      const double newmc = pred * beta;
-      const double penalty = (beta - 1) * (beta - 1) / (2 * fractional * fractional);
-      double stat = 0;
-      if (data == 0) stat = newmc;
-      else if (newmc > 0) stat = newmc - data + data * log(data / newmc);
-      double chisq = 0.;
-      if (c.type == GCM_LLH_BARLOW_OA2020 && isnan(penalty)) return inf;
-      if (pred > 0.0 && data > 0.0) chisq += 2.0 * (stat + penalty);
-      else if (pred > 0.0) chisq += 2.0 * (stat + penalty);
-      if (c.type == GCM_LLH_BARLOW_OA2020 && isnan(chisq)) return nan;
-      return chisq;
-    }
-    case GCM_LLH_CHI_SQUARED: {  // ChiSquared.h:19-29
-      if (pred == 0) return inf;
-      const double d = pred - data;
-      return d * d / pred;
-    }
-    case GCM_LLH_LEAST_SQUARES: {  // LeastSquares.h:45-52
-      double v = data - pred;
-      v = v * v;
-      if (c.lsq_poisson && data > 1.0) v /= 0.5 * data;
-      return v;
-    }
-    default:
-      return nan;
-  }
-}
-
-// Wave-level sum of (s, c) pairs with a fixed butterfly shape: the result is
-// the same on every run.  With kComp the pairs are combined by two-sum.
 template <bool kComp>
 __device__ __forceinline__ void wave_sum(double& s, double& c) {
 #pragma unroll
@@ -914,6 +642,15 @@ struct gcm_context {
   EventRec* d_ev = nullptr;
   int32_t* d_ev_result = nullptr;
   uint32_t* d_bin_pbeg = nullptr;
+  // K-point launches (K >= kPointsMin, fill_points_kernel): range partial
+  // slots, point-interleaved inputs and tables
+  int64_t n_pr = 0;
+  uint32_t* d_tile_pr = nullptr;    // [n_tiles]
+  uint32_t* d_bin_prbeg = nullptr;  // [n_bins + 1]
+  double *d_prs = nullptr, *d_prs2 = nullptr, *d_prc = nullptr, *d_prc2 = nullptr;  // [n_pr][K]
+  double* d_xT = nullptr;           // [n_inputs][K]
+  double* d_tabT = nullptr;         // [table_len][K]
+  bool points_ready = false;
   double *d_psum = nullptr, *d_psum2 = nullptr, *d_pcomp = nullptr,
          *d_pcomp2 = nullptr;
   int32_t *d_in_par = nullptr, *d_in_flags = nullptr;
@@ -1254,6 +991,29 @@ int build_layout(gcm_context* c, const gcm_tables* t) {
   }
   if (np > UINT32_MAX) return fail(GCM_ERR_UNSUPPORTED, "too many partial slots");
   for (int b = 0; b < t->n_bins; ++b) bin_np[b + 1] += bin_np[b];
+  // K-point launches (fill_points_kernel): one partial slot per distinct bin
+  // per range of a tile (kRanges contiguous event ranges, range r starting at
+  // event r n / kRanges), numbered tile by tile and range by range
+  std::vector<uint32_t> tile_pr((size_t)std::max<int64_t>(T, 1), 0);
+  std::vector<uint32_t> bin_npr(t->n_bins + 1, 0);
+  int64_t npr = 0;
+  for (int64_t ti = 0; ti < T; ++ti) {
+    tile_pr[ti] = (uint32_t)npr;
+    const int n = tiles[ti].n_events;
+    int prev = -1;
+    for (int j = 0; j < n; ++j) {
+      const int b = t->bin[order[tiles[ti].first_event + j]];
+      bool start = false;
+      for (int r = 0; r < kRanges; ++r) start = start || (r * n / kRanges == j);
+      if (start || b != prev) {
+        npr++;
+        bin_npr[b + 1]++;
+      }
+      prev = b;
+    }
+  }
+  if (npr > UINT32_MAX) return fail(GCM_ERR_UNSUPPORTED, "too many partial slots");
+  for (int b = 0; b < t->n_bins; ++b) bin_npr[b + 1] += bin_npr[b];
 
   // fill records, descriptors and event arrays, tiles in parallel
   std::vector<R> rec((size_t)std::max<int64_t>(total_words, 1), R(0));
@@ -1373,6 +1133,9 @@ int build_layout(gcm_context* c, const gcm_tables* t) {
   GCM_TRY(upload(c, &c->d_ev, ev.data(), ev.size()));
   GCM_TRY(upload(c, &c->d_ev_result, ev_result.data(), ev_result.size()));
   GCM_TRY(upload(c, &c->d_bin_pbeg, bin_np.data(), bin_np.size()));
+  c->n_pr = npr;
+  GCM_TRY(upload(c, &c->d_tile_pr, tile_pr.data(), tile_pr.size()));
+  GCM_TRY(upload(c, &c->d_bin_prbeg, bin_npr.data(), bin_npr.size()));
   return GCM_OK;
 }
 
@@ -1543,6 +1306,17 @@ int gcm_build(gcm_context* c, const gcm_tables* t, int32_t max_points) {
   if (c->precision == GCM_PRECISION_FP32) {
     step(c->alloc(&c->d_pcomp, K * c->n_partials));
     step(c->alloc(&c->d_pcomp2, K * c->n_partials));
+  }
+  if (max_points >= kPointsMin) {
+    step(c->alloc(&c->d_prs, K * std::max<int64_t>(1, c->n_pr)));
+    step(c->alloc(&c->d_prs2, K * std::max<int64_t>(1, c->n_pr)));
+    if (c->precision == GCM_PRECISION_FP32) {
+      step(c->alloc(&c->d_prc, K * std::max<int64_t>(1, c->n_pr)));
+      step(c->alloc(&c->d_prc2, K * std::max<int64_t>(1, c->n_pr)));
+    }
+    step(c->alloc(&c->d_xT, K * std::max(1, c->n_inputs)));
+    step(c->alloc(&c->d_tabT, K * std::max<int64_t>(1, c->table_len)));
+    c->points_ready = true;
   }
   step(c->alloc(&c->d_hist, K * c->n_bins));
   step(c->alloc(&c->d_hist2, K * c->n_bins));
@@ -1867,6 +1641,7 @@ static int prepare_launch(gcm_context* c) {
     const void* f = grid ? fill_fn<true>(fp32, inl, nb) : fill_fn<false>(fp32, inl, nb);
     GCM_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, kMaxDynamicLds));
   }
+  GCM_TRY(points_set_lds_attrs());
   c->lds_attr_set = true;
   return GCM_OK;
 }
@@ -1883,6 +1658,8 @@ static int enqueue_fill(gcm_context* c, hipStream_t s, int K, bool copies,
       GCM_HIP(hipMemcpyAsync(c->d_tab, c->h_tab, (size_t)K * c->table_len * sizeof(double),
                              hipMemcpyHostToDevice, s));
   }
+  // K-point launches of at least kPointsMin points: one lane per point
+  const bool points = K >= kPointsMin && c->points_ready;
   const int n_tab_idx = c->tab_updates.empty() ? 0 : (int)c->tab_in_idx.size();
   prologue_kernel<<<K, 256, 0, s>>>(c->d_theta, c->n_par, c->d_in_par, c->d_in_mlo,
                                     c->d_in_mrange, c->d_in_min, c->d_in_max,
@@ -1890,7 +1667,7 @@ static int enqueue_fill(gcm_context* c, hipStream_t s, int K, bool copies,
                                     c->d_tab_in_idx, n_tab_idx, c->d_tab_in,
                                     c->llh_set ? c->d_prior_mean : nullptr,
                                     c->llh_set ? c->d_prior_sigma : nullptr, c->d_penalty,
-                                    c->d_flags);
+                                    c->d_flags, points ? c->d_xT : nullptr);
   GCM_HIP(hipGetLastError());
   if (c->n_pen_sets > 0) {
     penalty_sets_kernel<<<K, 256, 0, s>>>(c->d_theta, c->n_par, c->n_pen_sets, c->d_pen_meta,
@@ -1952,8 +1729,16 @@ static int enqueue_fill(gcm_context* c, hipStream_t s, int K, bool copies,
                  (staged_pad + ring_lds) * (size_t)c->launch.blocks_per_cu <= kLdsPerCU;
   const size_t in_pad = a.inputs_lds ? staged_pad : (kClassBytes + 1023) / 1024 * 1024;
   const size_t lds = in_pad + ring_lds;
+  if (points && c->table_len) GCM_TRY(launch_transpose_tables(c->d_tab, c->table_len, K, c->d_tabT, s));
   if (tev) GCM_HIP(hipEventRecord(tev[0], s));
-  if (c->n_tiles > 0) {
+  if (points && c->n_tiles > 0) {
+    const int nb = c->launch.ring_buffers;
+    const size_t plds = points_lds_bytes(a, nb);
+    const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(4, (int64_t)(kLdsPerCU / plds)));
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(c->n_tiles, per_cu * c->n_cu));
+    PointsArgs pa{c->d_xT, c->d_tabT, c->d_tile_pr, c->d_prs, c->d_prs2, c->d_prc, c->d_prc2};
+    GCM_TRY(launch_points(a, pa, fp32, c->has_grid, nb, grid, plds, s));
+  } else if (c->n_tiles > 0) {
     const int64_t want = (int64_t)c->launch.blocks_per_cu * c->n_cu;
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(c->n_tiles, want));
     const int nb = c->launch.ring_buffers;
@@ -1969,7 +1754,12 @@ static int enqueue_fill(gcm_context* c, hipStream_t s, int K, bool copies,
   // the per-bin pass over K x n_bins items, then one block per point
   const int fb = (int)std::max<int64_t>(1, std::min<int64_t>(
       c->fin_blocks, ((int64_t)K * c->n_bins + kFinalizeBlock / kWave - 1) / (kFinalizeBlock / kWave)));
-  if (fp32)
+  if (points) {
+    PointsArgs pa{c->d_xT, c->d_tabT, c->d_tile_pr, c->d_prs, c->d_prs2, c->d_prc, c->d_prc2};
+    GCM_TRY(launch_finalize_points(pa, fp32, c->d_bin_prbeg, c->n_bins, K,
+                                   c->llh_set ? c->d_data : nullptr, c->llh, c->d_hist,
+                                   c->d_hist2, c->d_llh_part, s));
+  } else if (fp32)
     finalize_bins_kernel<true><<<fb, kFinalizeBlock, 0, s>>>(
         c->d_psum, c->d_psum2, c->d_pcomp, c->d_pcomp2, c->n_partials, c->d_bin_pbeg,
         c->n_bins, K, c->llh_set ? c->d_data : nullptr, c->llh, c->d_hist, c->d_hist2,

@@ -5,7 +5,8 @@ and spline sizes including the largest knot counts, the reweight cap, one
 event, and an event too large for a tile (a clean build error).
 
 Bars as in the parity tests: weights bit-exact, sums within 1e-10 relative,
-LLH within 1e-10 relative."""
+LLH within 1e-10 relative.  Every case runs one point per launch, a 4-point
+launch and a 20-point launch (the one-lane-per-point kernel)."""
 import numpy as np
 import pytest
 
@@ -94,6 +95,34 @@ def _check(t, thetas, max_points=4):
         _, s_ref, _ = orc.propagate(pts[k], want_weights=False)
         np.testing.assert_allclose(cm.hist(k)[0], s_ref, rtol=1e-10, atol=0)
     cm.close()
+    # 20 points per launch: the one-lane-per-point kernel (lanes 20-63 idle);
+    # point 0's event weights bit-exact, every point's sums and LLH to 1e-10
+    base = np.array(thetas)
+    rng = np.random.default_rng(7)
+    pts = base[np.arange(20) % len(base)] + (np.arange(20) >= len(base))[:, None] * \
+        0.1 * rng.standard_normal((20, base.shape[1]))
+    cmp = CacheManager(0)
+    cmp.build(t, max_points=20)
+    cmp.set_event_weight_copy(True)
+    cmp.set_likelihood("poisson", data)
+    cmp.set_parameters(pts)
+    cmp.fill().get()
+    w = cmp.copy_event_weights(np.full(t.n_events, np.nan))
+    st = cmp.llh()[1]
+    for k in range(20):
+        w_ref, s_ref, e_ref = orc.propagate(pts[k])
+        if k == 0:
+            binned = t.bin >= 0
+            np.testing.assert_array_equal(w[binned], w_ref[binned])
+        s, s2 = cmp.hist(k)
+        np.testing.assert_allclose(s, s_ref, rtol=1e-10, atol=0)
+        np.testing.assert_allclose(np.sqrt(s2), e_ref, rtol=1e-10, atol=0)
+        _, st_ref, _ = pyoracle.llh("poisson", data, s_ref, e_ref)
+        if np.isfinite(st_ref):
+            assert abs(st[k] - st_ref) <= 1e-10 * max(1.0, abs(st_ref))
+        else:
+            assert st[k] == st_ref
+    cmp.close()
     orc.close()
 
 

@@ -417,3 +417,89 @@ def test_validate_histogram_contents():
     assert not cm.validate_histogram_contents(s_ref, err)
     cm.close()
     orc.close()
+
+
+@pytest.mark.parametrize("workload,n_events,K", [(0, 0, 16), (2, 20000, 64), (3, 0, 70)])
+def test_points_kernel_matches_oracle(workload, n_events, K):
+    """K >= 16 points per launch run one lane per point (fill_points_kernel):
+    every point's bin sums, sqrt(sum w^2) and LLH against the oracle evaluated
+    point by point (1e-10), point 0's event weights bit-exact.  Workload 0
+    holds every dial type (2-D dials, mirrors, caps, shifts, second inputs);
+    K = 70 takes two chunks of lanes (64 + 6)."""
+    print(f"points kernel workload={workload} n_events={n_events} K={K}", flush=True)
+    wl, t = _setup(workload, n_events)
+    orc = pyoracle.Oracle(t, n_threads=8)
+    cm = CacheManager(0)
+    cm.build(t, max_points=K)
+    cm.set_event_weight_copy(True)
+    _, data, _ = orc.propagate(wl.nominal(), want_weights=False) if not t.table_len else \
+        (None, np.full(t.n_bins, 3.0), None)
+    kind = "poisson" if workload in (1, 3) else "barlow"
+    cm.set_likelihood(kind, data, wl.prior_mean, wl.prior_sigma)
+    pts = wl.random_points(K, 0.5, seed=23)
+    tabs = wl.table_values(pts) if t.table_len else None
+    cm.set_parameters(pts)
+    if tabs is not None:
+        cm.set_table_values(tabs)
+    cm.fill().get()
+    w = cm.copy_event_weights(np.full(t.n_events, np.nan))
+    tot = cm.llh()[0].copy()
+    binned = t.bin >= 0
+    for k in range(K):
+        if tabs is not None:
+            orc.set_table_values(tabs[k])
+        w_ref, s_ref, e_ref = orc.propagate(pts[k], want_weights=(k == 0))
+        if k == 0:
+            np.testing.assert_array_equal(w[binned], w_ref[binned])
+        s, s2 = cm.hist(k)
+        np.testing.assert_allclose(s, s_ref, rtol=RTOL_SUM, atol=0)
+        np.testing.assert_allclose(np.sqrt(s2), e_ref, rtol=RTOL_SUM, atol=0)
+        tot_ref, _, _ = pyoracle.llh(kind, data, s_ref, e_ref, pts[k], wl.prior_mean,
+                                     wl.prior_sigma)
+        _close(tot[k], tot_ref)
+    cm.close()
+    orc.close()
+
+
+def test_points_kernel_against_single_points_and_fp32():
+    """The one-lane-per-point kernel against one-point launches of the same
+    tables: event weights equal bit for bit (same products in the same
+    order), bin sums within 1e-12 relative (the two kernels add a bin's
+    events in different orders: serial per (tile, range) slot here, a tree
+    per tile there); and the FP32 mode's 32-point launch within the FP32
+    bounds of test_fp32_mode_error_bounds against the FP64 one."""
+    print("points kernel vs single, fp32", flush=True)
+    wl, t = _setup(2, 30000)
+    pts = wl.random_points(32, 0.5, seed=29)
+    tabs = wl.table_values(pts)
+    cms = {}
+    for prec in ("fp64", "fp32"):
+        cms[prec] = CacheManager(0, precision=prec)
+        cms[prec].build(t, max_points=32)
+        cms[prec].set_event_weight_copy(True)
+        cms[prec].set_likelihood("barlow", np.full(t.n_bins, 3.0), wl.prior_mean,
+                                 wl.prior_sigma)
+        cms[prec].set_parameters(pts)
+        cms[prec].set_table_values(tabs)
+        cms[prec].fill().get()
+    cm = cms["fp64"]
+    hk = [cm.hist(k) for k in range(32)]
+    h32 = [cms["fp32"].hist(k) for k in range(32)]
+    w_batch = cm.copy_event_weights(np.full(t.n_events, np.nan))
+    binned = t.bin >= 0
+    for k in range(32):
+        s64, _ = hk[k]
+        nz = s64 != 0
+        assert np.max(np.abs(h32[k][0][nz] - s64[nz]) / s64[nz]) < 1e-4
+    for k in (0, 7, 31):
+        cm.set_parameters(pts[k])
+        cm.set_table_values(tabs[k])
+        cm.fill().get()
+        s, s2 = cm.hist(0)
+        np.testing.assert_allclose(hk[k][0], s, rtol=1e-12, atol=0)
+        np.testing.assert_allclose(hk[k][1], s2, rtol=1e-12, atol=0)
+        if k == 0:
+            w = cm.copy_event_weights(np.full(t.n_events, np.nan))
+            np.testing.assert_array_equal(w_batch[binned], w[binned])
+    for c in cms.values():
+        c.close()
