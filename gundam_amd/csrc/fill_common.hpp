@@ -298,8 +298,6 @@ __device__ inline double llh_bin(const LlhParams& c, int b, double data, double 
   }
 }
 
-// Wave-level sum of (s, c) pairs with a fixed butterfly shape: the result is
-// the same on every run.  With kComp the pairs are combined by two-sum.
 // ---- the K-point path (gcm_points.hip) ----------------------------------
 struct PointsArgs {
   const double* xT;         // [n_inputs][K]
@@ -311,15 +309,33 @@ struct PointsArgs {
   double* prc2;
 };
 
+// The per-bin pass of a K-point launch in two levels, so that no wave adds
+// more than `chunk` slots one after another (a bin's slot count grows with
+// its events): level 1 sums runs of at most `chunk` consecutive slots of one
+// bin, and finishes a bin that is a single run; level 2 adds a longer bin's
+// run sums in run order and finishes it.  Every bin has at least one run
+// (an empty bin a run of no slots), so every bin is written.
+struct FinalizePlan {
+  const uint4* runs;  // [n_runs] {first slot, slots, bin, 0 | 1 + item}
+  int64_t n_runs;
+  const uint4* long_bins;  // [n_long] {bin, first item, items, 0}
+  int64_t n_long;
+  double* ips;  // [items][K] the run sums of the long bins
+  double* ips2;
+  double* ipc;  // compensations (FP32 mode)
+  double* ipc2;
+};
+
 // LDS bytes of one fill_points_kernel block for the tile layout in a.
 size_t points_lds_bytes(const FillArgs& a, int nb);
 // Allow fill_points_kernel its dynamic LDS (once per process is enough).
 int points_set_lds_attrs();
-// Enqueue fill_points_kernel over the tiles; finalize_points_kernel over the
-// bins; transpose_tables_kernel (tab [K][len] -> tabT [len][K]).
+// Enqueue fill_points_kernel over the tiles (a persistent grid of the blocks
+// that fit on n_cu CUs at once); finalize_points_kernel over the bins;
+// transpose_tables_kernel (tab [K][len] -> tabT [len][K]).
 int launch_points(const FillArgs& a, const PointsArgs& pa, bool fp32, bool grid, int nb,
-                  int blocks, size_t lds, hipStream_t s);
-int launch_finalize_points(const PointsArgs& pa, bool fp32, const uint32_t* pbeg, int n_bins,
+                  int n_cu, size_t lds, hipStream_t s);
+int launch_finalize_points(const PointsArgs& pa, const FinalizePlan& fp, bool fp32, int n_bins,
                            int k_points, const double* data, const LlhParams& cfg,
                            double* hist, double* hist2, double* llh_terms, hipStream_t s);
 int launch_transpose_tables(const double* tab, int64_t len, int k_points, double* tabT,

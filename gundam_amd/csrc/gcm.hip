@@ -440,6 +440,8 @@ static const void* fill_fn(bool fp32, bool inputs_lds, int nb) {
                                : (const void*)fill_kernel<double, false, false, 2, kGrid>);
 }
 
+// Wave-level sum of (s, c) pairs with a fixed butterfly shape: the result is
+// the same on every run.  With kComp the pairs are combined by two-sum.
 template <bool kComp>
 __device__ __forceinline__ void wave_sum(double& s, double& c) {
 #pragma unroll
@@ -646,8 +648,12 @@ struct gcm_context {
   // slots, point-interleaved inputs and tables
   int64_t n_pr = 0;
   uint32_t* d_tile_pr = nullptr;    // [n_tiles]
-  uint32_t* d_bin_prbeg = nullptr;  // [n_bins + 1]
   double *d_prs = nullptr, *d_prs2 = nullptr, *d_prc = nullptr, *d_prc2 = nullptr;  // [n_pr][K]
+  // the two-level per-bin pass (FinalizePlan): runs, long bins, run sums
+  uint4* d_fin_runs = nullptr;
+  uint4* d_fin_long = nullptr;
+  int64_t n_fin_runs = 0, n_fin_long = 0, n_fin_items = 0;
+  double *d_ips = nullptr, *d_ips2 = nullptr, *d_ipc = nullptr, *d_ipc2 = nullptr;  // [items][K]
   double* d_xT = nullptr;           // [n_inputs][K]
   double* d_tabT = nullptr;         // [table_len][K]
   bool points_ready = false;
@@ -1014,6 +1020,30 @@ int build_layout(gcm_context* c, const gcm_tables* t) {
   }
   if (npr > UINT32_MAX) return fail(GCM_ERR_UNSUPPORTED, "too many partial slots");
   for (int b = 0; b < t->n_bins; ++b) bin_npr[b + 1] += bin_npr[b];
+  // The two-level plan of their per-bin pass: runs of at most `chunk` slots,
+  // chunk about the square root of the longest bin's slot count (a multiple
+  // of 8, at least 16), so that neither level adds a long chain of slots.
+  uint32_t max_slots = 0;
+  for (int b = 0; b < t->n_bins; ++b) max_slots = std::max(max_slots, bin_npr[b + 1] - bin_npr[b]);
+  uint32_t chunk = 16;
+  while ((uint64_t)chunk * chunk < max_slots) chunk += 8;
+  std::vector<uint4> fin_runs, fin_long;
+  fin_runs.reserve(t->n_bins);
+  uint32_t fin_items = 0;
+  for (int b = 0; b < t->n_bins; ++b) {
+    const uint32_t p0 = bin_npr[b], n = bin_npr[b + 1] - p0;
+    if (n <= chunk) {
+      fin_runs.push_back(make_uint4(p0, n, (uint32_t)b, 0));
+      continue;
+    }
+    const uint32_t m = (n + chunk - 1) / chunk;
+    fin_long.push_back(make_uint4((uint32_t)b, fin_items, m, 0));
+    for (uint32_t i = 0; i < m; ++i) {
+      const uint32_t f = p0 + i * chunk;
+      fin_runs.push_back(make_uint4(f, std::min(chunk, p0 + n - f), (uint32_t)b, fin_items + i + 1));
+    }
+    fin_items += m;
+  }
 
   // fill records, descriptors and event arrays, tiles in parallel
   std::vector<R> rec((size_t)std::max<int64_t>(total_words, 1), R(0));
@@ -1135,7 +1165,11 @@ int build_layout(gcm_context* c, const gcm_tables* t) {
   GCM_TRY(upload(c, &c->d_bin_pbeg, bin_np.data(), bin_np.size()));
   c->n_pr = npr;
   GCM_TRY(upload(c, &c->d_tile_pr, tile_pr.data(), tile_pr.size()));
-  GCM_TRY(upload(c, &c->d_bin_prbeg, bin_npr.data(), bin_npr.size()));
+  c->n_fin_runs = (int64_t)fin_runs.size();
+  c->n_fin_long = (int64_t)fin_long.size();
+  c->n_fin_items = fin_items;
+  GCM_TRY(upload(c, &c->d_fin_runs, fin_runs.data(), fin_runs.size()));
+  GCM_TRY(upload(c, &c->d_fin_long, fin_long.data(), fin_long.size()));
   return GCM_OK;
 }
 
@@ -1313,6 +1347,12 @@ int gcm_build(gcm_context* c, const gcm_tables* t, int32_t max_points) {
     if (c->precision == GCM_PRECISION_FP32) {
       step(c->alloc(&c->d_prc, K * std::max<int64_t>(1, c->n_pr)));
       step(c->alloc(&c->d_prc2, K * std::max<int64_t>(1, c->n_pr)));
+    }
+    step(c->alloc(&c->d_ips, K * std::max<int64_t>(1, c->n_fin_items)));
+    step(c->alloc(&c->d_ips2, K * std::max<int64_t>(1, c->n_fin_items)));
+    if (c->precision == GCM_PRECISION_FP32) {
+      step(c->alloc(&c->d_ipc, K * std::max<int64_t>(1, c->n_fin_items)));
+      step(c->alloc(&c->d_ipc2, K * std::max<int64_t>(1, c->n_fin_items)));
     }
     step(c->alloc(&c->d_xT, K * std::max(1, c->n_inputs)));
     step(c->alloc(&c->d_tabT, K * std::max<int64_t>(1, c->table_len)));
@@ -1734,10 +1774,8 @@ static int enqueue_fill(gcm_context* c, hipStream_t s, int K, bool copies,
   if (points && c->n_tiles > 0) {
     const int nb = c->launch.ring_buffers;
     const size_t plds = points_lds_bytes(a, nb);
-    const int64_t per_cu = std::max<int64_t>(1, std::min<int64_t>(4, (int64_t)(kLdsPerCU / plds)));
-    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(c->n_tiles, per_cu * c->n_cu));
     PointsArgs pa{c->d_xT, c->d_tabT, c->d_tile_pr, c->d_prs, c->d_prs2, c->d_prc, c->d_prc2};
-    GCM_TRY(launch_points(a, pa, fp32, c->has_grid, nb, grid, plds, s));
+    GCM_TRY(launch_points(a, pa, fp32, c->has_grid, nb, c->n_cu, plds, s));
   } else if (c->n_tiles > 0) {
     const int64_t want = (int64_t)c->launch.blocks_per_cu * c->n_cu;
     const int grid = (int)std::max<int64_t>(1, std::min<int64_t>(c->n_tiles, want));
@@ -1756,7 +1794,9 @@ static int enqueue_fill(gcm_context* c, hipStream_t s, int K, bool copies,
       c->fin_blocks, ((int64_t)K * c->n_bins + kFinalizeBlock / kWave - 1) / (kFinalizeBlock / kWave)));
   if (points) {
     PointsArgs pa{c->d_xT, c->d_tabT, c->d_tile_pr, c->d_prs, c->d_prs2, c->d_prc, c->d_prc2};
-    GCM_TRY(launch_finalize_points(pa, fp32, c->d_bin_prbeg, c->n_bins, K,
+    FinalizePlan fp{c->d_fin_runs, c->n_fin_runs, c->d_fin_long, c->n_fin_long,
+                    c->d_ips,      c->d_ips2,     c->d_ipc,      c->d_ipc2};
+    GCM_TRY(launch_finalize_points(pa, fp, fp32, c->n_bins, K,
                                    c->llh_set ? c->d_data : nullptr, c->llh, c->d_hist,
                                    c->d_hist2, c->d_llh_part, s));
   } else if (fp32)

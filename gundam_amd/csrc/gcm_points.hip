@@ -11,6 +11,17 @@
 
 namespace gcm {
 
+// Lane j's value, broadcast to the wave (j uniform).
+__device__ __forceinline__ uint32_t lane_u32(uint32_t v, int j) {
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, j);
+}
+
+__device__ __forceinline__ double lane_f64(double v, int j) {
+  const uint64_t b = (uint64_t)__double_as_longlong(v);
+  const uint64_t lo = lane_u32((uint32_t)b, j), hi = lane_u32((uint32_t)(b >> 32), j);
+  return __longlong_as_double((long long)(lo | hi << 32));
+}
+
 // K points per launch with one lane per point (K >= kPointsMin).  The tiles
 // stream into the same LDS ring as in fill_kernel; each tile's events are cut
 // into kRanges contiguous ranges, one per evaluator wave.  A wave walks its
@@ -26,8 +37,11 @@ namespace gcm {
 // landed, writes the tile's event-major index (dial-list position -> the
 // type-sorted descriptor) into the buffer, so no evaluator searches.
 
+// Launch bounds: the block's kRanges + NB - 1 waves, and registers for five
+// waves per SIMD, so that two blocks share a CU.
 template <typename R, bool kComp, int NB, bool kGrid>
-__global__ void __launch_bounds__(1024, 1) fill_points_kernel(FillArgs a, PointsArgs pa) {
+__global__ void __launch_bounds__((kRanges + NB - 1) * kWave, 5)
+    fill_points_kernel(FillArgs a, PointsArgs pa) {
   constexpr int P = NB - 1;  // producer waves
   constexpr int W = kRanges;  // evaluator waves
   extern __shared__ __align__(16) unsigned char lds[];
@@ -158,70 +172,63 @@ __global__ void __launch_bounds__(1024, 1) fill_points_kernel(FillArgs a, Points
         }
       };
       if (e0 < e1) {
-        // the range's dials are the event-major slots [S0, S1), walked as
-        // one stream; the descriptor and the points' x of the dial two ahead
-        // are fetched while the current one is evaluated
+        // the range's dials are the event-major slots [S0, S1), taken 64 at
+        // a time: lane j reads the descriptor of slot sb + j and its input's
+        // supervisor bounds, and the walk over the batch broadcasts them
+        // with readlane; the points' x of the next dial is loaded while the
+        // current one is evaluated
         const int S0 = (int)(s_ev[e0].dials & 0xffff);
         const EventRec last = s_ev[e1 - 1];
         const int S1 = (int)(last.dials & 0xffff) + (int)(last.dials >> 16);
-        struct Item {
-          uint32_t lo, hi;
-          double x;
-        };
-        auto fetch = [&](int sl) {
-          const uint64_t d = s_desc[s_inv[sl]];
-          Item it;
-          it.lo = __builtin_amdgcn_readfirstlane((uint32_t)d);
-          it.hi = __builtin_amdgcn_readfirstlane((uint32_t)(d >> 32));
-          it.x = pa.xT[(int64_t)(it.lo & 0xffff) * K + kk];
-          return it;
-        };
-        Item ia{}, ib{};
-        if (S0 < S1) ia = fetch(S0);
-        if (S0 + 1 < S1) ib = fetch(S0 + 1);
         int e = e0;
         EventRec er = s_ev[e];
         int ev_end = (int)(er.dials & 0xffff) + (int)(er.dials >> 16);
         double temp = 1;
-        for (int sl = S0; sl < S1; ++sl) {
-          while (sl >= ev_end) {  // events that end here (dialless ones too)
-            finish_event(e, er, temp);
-            temp = 1;
-            er = s_ev[++e];
-            ev_end = (int)(er.dials & 0xffff) + (int)(er.dials >> 16);
+        for (int sb = S0; sb < S1; sb += kWave) {
+          const int nbat = S1 - sb < kWave ? S1 - sb : kWave;
+          const uint64_t dv = s_desc[s_inv[sb + (lane < nbat ? lane : 0)]];
+          const uint32_t vlo = (uint32_t)dv, vhi = (uint32_t)(dv >> 32);
+          const int vcl = (int)((vlo >> 28) & 0xf);
+          // the supervisor does not depend on the point: the class record,
+          // or for the escape class the input's own record
+          double blo = s_cls[vcl].lo, bhi = s_cls[vcl].hi;
+          int bfl = s_cls[vcl].flags;
+          if (vcl == kEscapeClass) {
+            const InputRec* ir = a.xin + (vlo & 0xffff);
+            blo = ir->lo;
+            bhi = ir->hi;
+            bfl = ir->flags;
           }
-          const Item it = ia;
-          ia = ib;
-          if (sl + 2 < S1) ib = fetch(sl + 2);
-          const int in = (int)(it.lo & 0xffff);
-          const int type = (int)((it.lo >> 16) & 0xf);
-          const int nk = (int)((it.lo >> 20) & 0xff);
-          const int cl = (int)((it.lo >> 28) & 0xf);
-          const int off = (int)(it.hi >> 16);
-          double lo, hi;
-          int flags;
-          if (cl != kEscapeClass) {
-            lo = s_cls[cl].lo;
-            hi = s_cls[cl].hi;
-            flags = s_cls[cl].flags;
-          } else {  // the supervisor does not depend on the point
-            const InputRec ir = a.xin[in];
-            lo = ir.lo;
-            hi = ir.hi;
-            flags = ir.flags;
+          double xn = pa.xT[(int64_t)(lane_u32(vlo, 0) & 0xffff) * K + kk];
+          for (int j = 0; j < nbat; ++j) {
+            const int sl = sb + j;
+            while (sl >= ev_end) {  // events that end here (dialless ones too)
+              finish_event(e, er, temp);
+              temp = 1;
+              er = s_ev[++e];
+              ev_end = (int)(er.dials & 0xffff) + (int)(er.dials >> 16);
+            }
+            const uint32_t dlo = lane_u32(vlo, j), dhi = lane_u32(vhi, j);
+            const double x = xn;
+            if (j + 1 < nbat) xn = pa.xT[(int64_t)(lane_u32(vlo, j + 1) & 0xffff) * K + kk];
+            const int type = (int)((dlo >> 16) & 0xf);
+            const int nk = (int)((dlo >> 20) & 0xff);
+            const int off = (int)(dhi >> 16);
+            const int flags = (int)lane_u32((uint32_t)bfl, j);
+            double r;
+            if (kGrid && (type == GCM_DIAL_BILINEAR || type == GCM_DIAL_BICUBIC)) {
+              const int in2 = __builtin_amdgcn_readfirstlane((int)s_rec[off]);
+              const double x2 = pa.xT[(int64_t)in2 * K + kk];
+              r = eval_dial2d<R>(type, s_rec + off + 1, x, x2, flags);
+            } else {
+              r = eval_dial<R, true>(type, nk, s_rec + off, x, flags, pa.tabT + kk, K);
+            }
+            // DialResponseSupervisor::process (DialResponseSupervisor.cpp:10-18)
+            const double lo = lane_f64(blo, j), hi = lane_f64(bhi, j);
+            if (r < lo) r = lo;
+            else if (r > hi) r = hi;
+            temp *= r;
           }
-          double r;
-          if (kGrid && (type == GCM_DIAL_BILINEAR || type == GCM_DIAL_BICUBIC)) {
-            const int in2 = __builtin_amdgcn_readfirstlane((int)s_rec[off]);
-            const double x2 = pa.xT[(int64_t)in2 * K + kk];
-            r = eval_dial2d<R>(type, s_rec + off + 1, it.x, x2, flags);
-          } else {
-            r = eval_dial<R, true>(type, nk, s_rec + off, it.x, flags, pa.tabT + kk, K);
-          }
-          // DialResponseSupervisor::process (DialResponseSupervisor.cpp:10-18)
-          if (r < lo) r = lo;
-          else if (r > hi) r = hi;
-          temp *= r;
         }
         for (; e < e1; ++e) {  // the last event with dials, and dialless ones after it
           finish_event(e, er, temp);
@@ -242,67 +249,123 @@ __global__ void __launch_bounds__(1024, 1) fill_points_kernel(FillArgs a, Points
   }
 }
 
-// The per-bin pass of a K-point launch of fill_points_kernel: one wave per
-// (bin, chunk of 64 points), one lane per point; the bin's range slots
-// [pbeg[b], pbeg[b + 1]) are added in slot order (coalesced [slot][K] loads,
-// with Knuth two-sums in the FP32 mode), then the bin's LLH term per point.
+// Rows [p0, p1) of the [row][K] arrays v, v2 (and their compensations vc,
+// vc2 with kComp) added for point k in row order onto (s, s2, c, c2): eight
+// rows' loads in flight, coalesced over the points, Knuth two-sums with
+// kComp.
+template <bool kComp>
+__device__ __forceinline__ void add_rows(const double* __restrict__ v,
+                                         const double* __restrict__ v2,
+                                         const double* __restrict__ vc,
+                                         const double* __restrict__ vc2, uint32_t p0,
+                                         uint32_t p1, int K, int k, double& s, double& s2,
+                                         double& c, double& c2) {
+  for (uint32_t pb = p0; pb < p1; pb += 8) {
+    double x[8], x2[8], xc[8], xc2[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const bool ok = pb + u < p1;
+      const size_t q = (size_t)(ok ? pb + u : p0) * K + k;
+      x[u] = ok ? v[q] : 0.0;
+      x2[u] = ok ? v2[q] : 0.0;
+      if (kComp) {
+        xc[u] = ok ? vc[q] : 0.0;
+        xc2[u] = ok ? vc2[q] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      if (pb + u >= p1) break;
+      if (kComp) {
+        double t, e;
+        two_sum(s, x[u], t, e);
+        s = t;
+        c += e + xc[u];
+        two_sum(s2, x2[u], t, e);
+        s2 = t;
+        c2 += e + xc2[u];
+      } else {
+        s += x[u];
+        s2 += x2[u];
+      }
+    }
+  }
+}
+
+// A bin's sums for point k, and its LLH term.
+template <bool kComp>
+__device__ __forceinline__ void finish_bin(int b, int k, int n_bins, double s, double s2,
+                                           double c, double c2, const double* data,
+                                           const LlhParams& cfg, double* hist, double* hist2,
+                                           double* llh_terms) {
+  if (kComp) {
+    s = s + c;
+    s2 = s2 + c2;
+  }
+  hist[(size_t)k * n_bins + b] = s;
+  hist2[(size_t)k * n_bins + b] = s2;
+  llh_terms[(size_t)k * n_bins + b] = data ? llh_bin(cfg, b, data[b], s, sqrt(s2)) : 0.0;
+}
+
+// Level 1 of the per-bin pass of a K-point launch: one wave per (run, chunk
+// of 64 points), one lane per point; the run's range slots are added in slot
+// order; a bin of one run is finished here, a longer bin's run sum is stored
+// for level 2.
 template <bool kComp>
 __global__ void __launch_bounds__(kFinalizeBlock)
-    finalize_points_kernel(const double* __restrict__ prs, const double* __restrict__ prs2,
-                           const double* __restrict__ prc, const double* __restrict__ prc2,
-                           const uint32_t* __restrict__ pbeg, int n_bins, int k_points,
-                           const double* __restrict__ data, LlhParams cfg,
-                           double* __restrict__ hist, double* __restrict__ hist2,
-                           double* __restrict__ llh_terms) {
+    finalize_runs_kernel(PointsArgs pa, FinalizePlan fp, int n_bins, int k_points,
+                         const double* __restrict__ data, LlhParams cfg,
+                         double* __restrict__ hist, double* __restrict__ hist2,
+                         double* __restrict__ llh_terms) {
   const int lane = threadIdx.x % kWave;
   const int K = k_points;
   const int n_chunks = (K + kWave - 1) / kWave;
   const int64_t n_waves = (int64_t)gridDim.x * (blockDim.x / kWave);
-  const int64_t items = (int64_t)n_chunks * n_bins;
+  const int64_t items = (int64_t)n_chunks * fp.n_runs;
   for (int64_t it = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
        it < items; it += n_waves) {
-    const int b = (int)(it % n_bins), k = (int)(it / n_bins) * kWave + lane;
+    const int64_t r = it % fp.n_runs;
+    const int k = (int)(it / fp.n_runs) * kWave + lane;
     if (k >= K) continue;
+    const uint4 run = fp.runs[r];
     double s = 0.0, s2 = 0.0, c = 0.0, c2 = 0.0;
-    const uint32_t p0 = pbeg[b], p1 = pbeg[b + 1];
-    // eight slots' loads in flight, added in slot order
-    for (uint32_t pb = p0; pb < p1; pb += 8) {
-      double v[8], v2[8], vc[8], vc2[8];
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        const bool ok = pb + u < p1;
-        const size_t q = (size_t)(ok ? pb + u : p0) * K + k;
-        v[u] = ok ? prs[q] : 0.0;
-        v2[u] = ok ? prs2[q] : 0.0;
-        if (kComp) {
-          vc[u] = ok ? prc[q] : 0.0;
-          vc2[u] = ok ? prc2[q] : 0.0;
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < 8; ++u) {
-        if (pb + u >= p1) break;
-        if (kComp) {
-          double t, e;
-          two_sum(s, v[u], t, e);
-          s = t;
-          c += e + vc[u];
-          two_sum(s2, v2[u], t, e);
-          s2 = t;
-          c2 += e + vc2[u];
-        } else {
-          s += v[u];
-          s2 += v2[u];
-        }
+    add_rows<kComp>(pa.prs, pa.prs2, pa.prc, pa.prc2, run.x, run.x + run.y, K, k, s, s2, c, c2);
+    if (run.w == 0) {
+      finish_bin<kComp>((int)run.z, k, n_bins, s, s2, c, c2, data, cfg, hist, hist2, llh_terms);
+    } else {
+      const size_t q = (size_t)(run.w - 1) * K + k;
+      fp.ips[q] = s;
+      fp.ips2[q] = s2;
+      if (kComp) {
+        fp.ipc[q] = c;
+        fp.ipc2[q] = c2;
       }
     }
-    if (kComp) {
-      s = s + c;
-      s2 = s2 + c2;
-    }
-    hist[(size_t)k * n_bins + b] = s;
-    hist2[(size_t)k * n_bins + b] = s2;
-    llh_terms[(size_t)k * n_bins + b] = data ? llh_bin(cfg, b, data[b], s, sqrt(s2)) : 0.0;
+  }
+}
+
+// Level 2: one wave per (long bin, chunk of 64 points); the bin's run sums
+// are added in run order, then the bin is finished.
+template <bool kComp>
+__global__ void __launch_bounds__(kFinalizeBlock)
+    finalize_long_bins_kernel(FinalizePlan fp, int n_bins, int k_points,
+                              const double* __restrict__ data, LlhParams cfg,
+                              double* __restrict__ hist, double* __restrict__ hist2,
+                              double* __restrict__ llh_terms) {
+  const int lane = threadIdx.x % kWave;
+  const int K = k_points;
+  const int n_chunks = (K + kWave - 1) / kWave;
+  const int64_t n_waves = (int64_t)gridDim.x * (blockDim.x / kWave);
+  const int64_t items = (int64_t)n_chunks * fp.n_long;
+  for (int64_t it = (int64_t)blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+       it < items; it += n_waves) {
+    const int64_t j = it % fp.n_long;
+    const int k = (int)(it / fp.n_long) * kWave + lane;
+    if (k >= K) continue;
+    const uint4 lb = fp.long_bins[j];
+    double s = 0.0, s2 = 0.0, c = 0.0, c2 = 0.0;
+    add_rows<kComp>(fp.ips, fp.ips2, fp.ipc, fp.ipc2, lb.y, lb.y + lb.z, K, k, s, s2, c, c2);
+    finish_bin<kComp>((int)lb.x, k, n_bins, s, s2, c, c2, data, cfg, hist, hist2, llh_terms);
   }
 }
 
@@ -345,9 +408,15 @@ int points_set_lds_attrs() {
 }
 
 int launch_points(const FillArgs& a, const PointsArgs& pa, bool fp32, bool grid, int nb,
-                  int blocks, size_t lds, hipStream_t s) {
+                  int n_cu, size_t lds, hipStream_t s) {
   const int threads = (kRanges + nb - 1) * kWave;
   const void* fn = grid ? points_fn<true>(fp32, nb) : points_fn<false>(fp32, nb);
+  // a persistent grid: as many blocks as are resident at once (registers
+  // and LDS), never more than there are tiles
+  int per_cu = 0;
+  GCM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, threads, lds));
+  const int blocks = (int)std::max<int64_t>(
+      1, std::min<int64_t>(a.n_tiles, (int64_t)std::max(1, per_cu) * n_cu));
   FillArgs aa = a;
   PointsArgs pp = pa;
   void* args[] = {&aa, &pp};
@@ -356,21 +425,32 @@ int launch_points(const FillArgs& a, const PointsArgs& pa, bool fp32, bool grid,
   return GCM_OK;
 }
 
-int launch_finalize_points(const PointsArgs& pa, bool fp32, const uint32_t* pbeg, int n_bins,
+int launch_finalize_points(const PointsArgs& pa, const FinalizePlan& fp, bool fp32, int n_bins,
                            int k_points, const double* data, const LlhParams& cfg,
                            double* hist, double* hist2, double* llh_terms, hipStream_t s) {
-  const int64_t items = (int64_t)((k_points + kWave - 1) / kWave) * n_bins;
-  const int pb = (int)std::max<int64_t>(1, std::min<int64_t>(
-      kFinalizeMaxBlocks, (items + kFinalizeBlock / kWave - 1) / (kFinalizeBlock / kWave)));
+  const int64_t n_chunks = (k_points + kWave - 1) / kWave;
+  auto blocks = [](int64_t waves) {
+    return (int)std::max<int64_t>(1, std::min<int64_t>(
+        kFinalizeMaxBlocks, (waves + kFinalizeBlock / kWave - 1) / (kFinalizeBlock / kWave)));
+  };
+  const int b1 = blocks(n_chunks * fp.n_runs);
   if (fp32)
-    finalize_points_kernel<true><<<pb, kFinalizeBlock, 0, s>>>(
-        pa.prs, pa.prs2, pa.prc, pa.prc2, pbeg, n_bins, k_points, data, cfg, hist, hist2,
-        llh_terms);
+    finalize_runs_kernel<true><<<b1, kFinalizeBlock, 0, s>>>(pa, fp, n_bins, k_points, data, cfg,
+                                                             hist, hist2, llh_terms);
   else
-    finalize_points_kernel<false><<<pb, kFinalizeBlock, 0, s>>>(
-        pa.prs, pa.prs2, pa.prc, pa.prc2, pbeg, n_bins, k_points, data, cfg, hist, hist2,
-        llh_terms);
+    finalize_runs_kernel<false><<<b1, kFinalizeBlock, 0, s>>>(pa, fp, n_bins, k_points, data,
+                                                              cfg, hist, hist2, llh_terms);
   GCM_HIP(hipGetLastError());
+  if (fp.n_long > 0) {
+    const int b2 = blocks(n_chunks * fp.n_long);
+    if (fp32)
+      finalize_long_bins_kernel<true><<<b2, kFinalizeBlock, 0, s>>>(fp, n_bins, k_points, data,
+                                                                    cfg, hist, hist2, llh_terms);
+    else
+      finalize_long_bins_kernel<false><<<b2, kFinalizeBlock, 0, s>>>(fp, n_bins, k_points, data,
+                                                                     cfg, hist, hist2, llh_terms);
+    GCM_HIP(hipGetLastError());
+  }
   return GCM_OK;
 }
 
