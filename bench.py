@@ -59,14 +59,24 @@ def cpu_model() -> str:
     return "unknown"
 
 
-def cpu_baseline(workload, n_sample, n_full, seed, threads, evals=20):
+def cpu_baseline(workload, n_sample, n_full, seed, threads, evals=20, full_wl=None):
+    """The oracle timed on the host cores.  With full_wl (the bench's own
+    workload, binned), the whole event set; otherwise a regenerated subset of
+    n_sample events, scaled to the full count."""
     from gundam_amd import Workload
     from oracle import pyoracle
 
-    wl = Workload(workload, n_events=n_sample, seed=seed)
-    t = wl.tables
-    t.bin[:] = pyoracle.bin_events(wl.vars, wl.edges)
+    if full_wl is not None:
+        wl = full_wl
+        t = wl.tables
+        n_sample = t.n_events
+    else:
+        wl = Workload(workload, n_events=n_sample, seed=seed)
+        t = wl.tables
+        t.bin[:] = pyoracle.bin_events(wl.vars, wl.edges)
+    t0 = time.perf_counter()
     orc = pyoracle.Oracle(t, n_threads=threads)
+    build_s = time.perf_counter() - t0
     pts = wl.random_points(evals + 3, 0.5, seed=seed + 1)
     tabs = wl.table_values(pts) if t.table_len else None
     _, data, _ = orc.propagate(wl.nominal(), want_weights=False)
@@ -96,14 +106,20 @@ def cpu_baseline(workload, n_sample, n_full, seed, threads, evals=20):
         if i >= 3:
             times1.append(time.perf_counter() - t0)
     o1.close()
+    orc.close()
+    full = n_sample == n_full
     return {
         "workload1_evals_per_s": 1.0 / float(np.median(times1)),
         "value": (1.0 / med) * (n_sample / n_full),
-        "unit": "LLH evals/s (scaled to the full event count)",
+        "unit": "LLH evals/s" if full else "LLH evals/s (scaled to the full event count)",
         "cores": threads,
         "cpu_model": cpu_model(),
         "kind": "port",
-        "sample": (f"workload {workload} restricted to {n_sample} events "
+        "oracle_build_s": build_s,
+        "sample": (f"workload {workload}, all {n_sample} events ({binned} binned), median "
+                   f"of {len(times)} evaluations after 3 warm-ups = {med*1e3:.2f} ms"
+                   if full else
+                   f"workload {workload} restricted to {n_sample} events "
                    f"({binned} binned), median of {len(times)} evaluations "
                    f"after 3 warm-ups = {med*1e3:.2f} ms, scaled linearly by "
                    f"{n_sample}/{n_full} events"),
@@ -225,8 +241,9 @@ def surrogate_da(args, cm, wl, data, err, log, device_tables=False):
     plain chain's squared jump per accepted move: the number of plain-chain
     accepted moves that cover the same distance.  Raw accepted moves/s are
     reported beside it."""
-    from gundam_amd.mcmc import (ExactTarget, SurrogateTarget, adapt_cov, laplace_scales,
-                                 run_batched_exact, run_delayed, run_plain, var_ratio)
+    from gundam_amd.mcmc import (ExactTarget, SurrogateTarget, adapt_cov, ess_batch_means,
+                                 laplace_scales, run_batched_exact, run_delayed, run_plain,
+                                 var_ratio)
     from gundam_amd.surrogate import Surrogate
 
     tv = wl.table_values if wl.tables.table_len and not device_tables else None
@@ -280,14 +297,20 @@ def surrogate_da(args, cm, wl, data, err, log, device_tables=False):
     out = {}
     for k, r in res.items():
         sj = r.sq_jumps(sc) / r.seconds
-        out[k] = {"accepted_per_s": r.accepted_per_s,
+        ess = ess_batch_means(r.samples)
+        kept = r.samples.shape[0] - r.samples.shape[0] // 10
+        out[k] = {"ess_per_s": {"min": float(ess.min() / r.seconds),
+                                "median": float(np.median(ess) / r.seconds)},
+                  "ess_batches_per_chain": max(2, int(np.sqrt(kept))),
+                  "accepted_per_s": r.accepted_per_s,
                   "exact_equivalent_accepted_per_s": sj / jump_per_acc,
                   "exact_evals_per_s": r.exact_evals / r.seconds,
                   "exact_evals_per_accepted": r.exact_evals / max(1, r.accepted),
                   "stage2_acceptance": r.accepted / max(1, r.stage1_passed),
                   "iterations": int(r.samples.shape[0]), "chains": int(r.samples.shape[1]),
                   "seconds": r.seconds}
-        log(f"c5 {k}: {r.accepted_per_s:.1f} accepted/s, "
+        log(f"c5 {k}: ESS/s min {out[k]['ess_per_s']['min']:.2f} median "
+            f"{out[k]['ess_per_s']['median']:.2f}, {r.accepted_per_s:.1f} accepted/s, "
             f"{sj / jump_per_acc:.1f} exact-equivalent/s, "
             f"{r.exact_evals / max(1, r.accepted):.2f} exact evals per accepted move")
     out["da"]["n_sub"] = 1
@@ -314,6 +337,13 @@ def surrogate_da(args, cm, wl, data, err, log, device_tables=False):
                                        "p50_abs_centred": float(np.median(cen)),
                                        "p90_abs_centred": float(np.quantile(cen, 0.9))},
         "methods": out,
+        "gain_ess_per_s": {k: {q: out[k]["ess_per_s"][q] / out["plain"]["ess_per_s"][q]
+                               for q in ("min", "median")}
+                           for k in ("batched_exact", "da", "da_sub")},
+        "ess_method": ("batch means (Flegal & Jones 2010), per parameter over all chains, "
+                       "first 10% of each run dropped, floor(sqrt(n)) batches per chain; "
+                       "runs much shorter than the integrated autocorrelation time "
+                       "overstate ESS"),
         "gain_exact_equivalent": {k: out[k]["exact_equivalent_accepted_per_s"] / base
                                   for k in ("batched_exact", "da", "da_sub")},
         "gain_accepted_per_s": {k: out[k]["accepted_per_s"] / out["plain"]["accepted_per_s"]
@@ -347,8 +377,14 @@ def precision_c4(args, log):
     from gundam_amd.autotune import autotune
     tp = wl.prior_points(1, seed=args.seed + 5)
     for prec in ("fp64", "fp32"):
-        tuned = autotune(t, tp, wl.table_values(tp), precision=prec,
-                         n_events=args.tune_events, reps=3)
+        # FP32 records are half the size: the main workload's FP64 shape
+        # does not carry over, so that mode is always tuned
+        if args.c4_autotune or not args.tile_words or prec == "fp32":
+            tuned = autotune(t, tp, wl.table_values(tp), precision=prec,
+                             n_events=args.tune_events, reps=3)
+        else:
+            tuned = {"tile_words": args.tile_words, "block_threads": args.block,
+                     "blocks_per_cu": args.blocks_per_cu, "ring_buffers": args.ring}
         shapes[prec] = {k: tuned[k] for k in ("tile_words", "block_threads",
                                               "blocks_per_cu", "ring_buffers")}
         t0 = time.time()
@@ -597,7 +633,8 @@ def main():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--batch", type=int, default=64,
                     help="points per launch for the batched figure (0: skip)")
-    ap.add_argument("--cpu-sample", type=int, default=1000000)
+    ap.add_argument("--cpu-sample", type=int, default=0,
+                    help="events of the CPU baseline (0: the whole workload)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--tile-words", type=int, default=0)
@@ -610,17 +647,20 @@ def main():
     ap.add_argument("--tune-events", type=int, default=500000)
     ap.add_argument("--no-c5", action="store_true",
                     help="skip the surrogate delayed-acceptance MCMC leg")
-    ap.add_argument("--c5-pairs", type=int, default=16000)
+    ap.add_argument("--c5-pairs", type=int, default=200000)
     ap.add_argument("--c5-burn", type=int, default=120)
-    ap.add_argument("--c5-epochs", type=int, default=300)
+    ap.add_argument("--c5-epochs", type=int, default=60)
     ap.add_argument("--c5-sub", type=int, default=16)
     ap.add_argument("--c5-seconds", type=float, default=8.0)
     ap.add_argument("--host-tables", action="store_true",
                     help="compute lookup tables with the host callback and upload them "
                          "every step (default: device table refresh)")
-    ap.add_argument("--c4", action="store_true",
-                    help="add the FP32-vs-FP64 precision study on the 20M-event workload")
-    ap.add_argument("--c4-points", type=int, default=1000)
+    ap.add_argument("--no-c4", action="store_true",
+                    help="skip the FP32-vs-FP64 precision study on the 20M-event workload")
+    ap.add_argument("--c4-points", type=int, default=128)
+    ap.add_argument("--c4-autotune", action="store_true",
+                    help="autotune the precision study's launch shape per precision "
+                         "(default: the main workload's shape)")
     ap.add_argument("--only-c4", action="store_true",
                     help="run only the precision study and print its JSON")
     ap.add_argument("--only-posterior", action="store_true",
@@ -759,7 +799,7 @@ def main():
     batched_c3 = None if args.no_c3 else batched_workload3(args, log)
     plain_c1 = None if args.no_c3 else plain_workload1(args, log)
 
-    c4 = precision_c4(args, log) if args.c4 else None
+    c4 = None if args.no_c4 else precision_c4(args, log)
 
     c5 = None
     if not args.no_c5 and args.batch >= 32:
@@ -770,9 +810,10 @@ def main():
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         n_full = t.n_events
-        sample = min(args.cpu_sample, n_full)
+        sample = min(args.cpu_sample, n_full) if args.cpu_sample > 0 else n_full
         log(f"cpu baseline: oracle on {sample} events, {threads} threads")
-        cpu = cpu_baseline(args.workload, sample, n_full, args.seed, threads)
+        cpu = cpu_baseline(args.workload, sample, n_full, args.seed, threads,
+                           full_wl=wl if sample == n_full else None)
 
     traffic = read_traffic(os.path.join(
         ROOT, "profiles", f"fill_kernel_workload{args.workload}_{args.precision}_traffic.txt"))

@@ -79,11 +79,13 @@ __device__ __forceinline__ double input_value(const double* th, const int32_t* i
   return v;
 }
 
-// Per point k (one block each, 256 threads): mirrored input values (and, for
-// K-point launches of fill_points_kernel, their input-major copy xT), the
-// inputs of the device table-update jobs (tab_in[k][j] = value of input
-// tab_idx[j]) and the prior penalty.
-__global__ void prologue_kernel(const double* __restrict__ theta, int n_par,
+// Per point k (one block each, 256 threads): the point's parameters read
+// from the caller's pinned staging buffer theta_src (when given) into theta,
+// then the mirrored input values (and, for K-point launches of
+// fill_points_kernel, their input-major copy xT), the inputs of the device
+// table-update jobs (tab_in[k][j] = value of input tab_idx[j]) and the prior
+// penalty.
+__global__ void prologue_kernel(const double* theta_src, double* theta, int n_par,
                                 const int32_t* __restrict__ in_par,
                                 const double* __restrict__ in_mlo,
                                 const double* __restrict__ in_mrange,
@@ -98,7 +100,13 @@ __global__ void prologue_kernel(const double* __restrict__ theta, int n_par,
                                 double* __restrict__ penalty, int* __restrict__ flags,
                                 double* __restrict__ xT) {
   const int k = blockIdx.x;
-  const double* th = theta + (size_t)k * n_par;
+  double* th = theta + (size_t)k * n_par;
+  if (theta_src) {  // host memory: system-scope loads, so no stale cached copy
+    const double* src = theta_src + (size_t)k * n_par;
+    for (int p = threadIdx.x; p < n_par; p += blockDim.x)
+      th[p] = __hip_atomic_load(src + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __syncthreads();
+  }
   // the NaN-weight flag reports this evaluation only (cleared in-stream, so
   // that graph replays and fills that are never synchronised start clean)
   if (k == 0 && threadIdx.x == 0) *flags = 0;
@@ -513,10 +521,15 @@ __global__ void __launch_bounds__(kFinalizeBlock)
 
 // One block per point: the per-bin LLH terms summed in a fixed order (each
 // thread a strided run in bin order, then a fixed tree), plus the penalty.
+// The results also go to the caller's pinned result block host_res
+// ([total][stat][penalty], M entries each, then the NaN flag), written with
+// system-scope stores, so that reading them after the stream's
+// synchronisation needs no copy.
 __global__ void __launch_bounds__(kTotalBlock)
     finalize_total_kernel(const double* __restrict__ llh_terms, int n_bins,
                           const double* __restrict__ penalty, double* stat_out,
-                          double* pen_out, double* total_out) {
+                          double* pen_out, double* total_out, const int* flags,
+                          double* host_res, int M) {
   __shared__ double s_red[kTotalBlock];
   const int k = blockIdx.x;
   const double* lb = llh_terms + (size_t)k * n_bins;
@@ -530,9 +543,16 @@ __global__ void __launch_bounds__(kTotalBlock)
   }
   if (threadIdx.x == 0) {
     const double stat = s_red[0];
+    const double pen = penalty[k];
     stat_out[k] = stat;
-    pen_out[k] = penalty[k];
-    total_out[k] = stat + penalty[k];
+    pen_out[k] = pen;
+    total_out[k] = stat + pen;
+    __hip_atomic_store(host_res + k, stat + pen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(host_res + M + k, stat, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(host_res + 2 * M + k, pen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    if (k == 0)
+      __hip_atomic_store(reinterpret_cast<int*>(host_res + 3 * M), *flags, __ATOMIC_RELAXED,
+                         __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
@@ -671,13 +691,17 @@ struct gcm_context {
          *d_total = nullptr, *d_penalty = nullptr;
   int* d_flags = nullptr;
   double* d_weights = nullptr;
-  // pinned staging
+  // pinned staging; the parameters and the results are coherent pinned
+  // memory that the prologue reads and finalize_total_kernel writes (their
+  // device addresses h_theta_dev, h_res_dev), so that a fill copies neither
   double* h_theta = nullptr;
   double* h_tab = nullptr;
-  // per-point results {total[K], stat[K], penalty[K], NaN flag}: one device
-  // block (d_res) copied to pinned memory (h_res) at the end of every fill
+  // per-point results {total[K], stat[K], penalty[K], NaN flag}: the device
+  // block d_res and its pinned mirror h_res
   double* d_res = nullptr;
   double* h_res = nullptr;
+  double* h_theta_dev = nullptr;
+  double* h_res_dev = nullptr;
   bool tab_dirty = false, theta_dirty = false;
   hipEvent_t ev_staged = nullptr;  // staging buffers consumed by the device
   hipStream_t last_stream = nullptr;
@@ -1378,11 +1402,15 @@ int gcm_build(gcm_context* c, const gcm_tables* t, int32_t max_points) {
   GCM_HIP(hipMemset(c->d_flags, 0, sizeof(int)));
   GCM_HIP(hipMemset(c->d_tab, 0, K * std::max<int64_t>(1, c->table_len) * sizeof(double)));
   GCM_HIP(hipMemset(c->d_theta, 0, K * c->n_par * sizeof(double)));
-  if (hipHostMalloc((void**)&c->h_theta, K * c->n_par * sizeof(double)) != hipSuccess ||
+  const unsigned coherent = hipHostMallocMapped | hipHostMallocCoherent;
+  if (hipHostMalloc((void**)&c->h_theta, K * c->n_par * sizeof(double), coherent) != hipSuccess ||
       hipHostMalloc((void**)&c->h_tab, K * std::max<int64_t>(1, c->table_len) * sizeof(double)) != hipSuccess ||
-      hipHostMalloc((void**)&c->h_res, (3 * K + 1) * sizeof(double)) != hipSuccess)
+      hipHostMalloc((void**)&c->h_res, (3 * K + 1) * sizeof(double), coherent) != hipSuccess)
     return fail(GCM_ERR_NOMEM, "pinned staging allocation failed");
+  std::memset(c->h_theta, 0, K * c->n_par * sizeof(double));
   std::memset(c->h_res, 0, (3 * K + 1) * sizeof(double));
+  GCM_HIP(hipHostGetDevicePointer((void**)&c->h_theta_dev, c->h_theta, 0));
+  GCM_HIP(hipHostGetDevicePointer((void**)&c->h_res_dev, c->h_res, 0));
   // default likelihood: no data histogram -> stat term 0 until set
   c->built = true;
   return GCM_OK;
@@ -1686,14 +1714,14 @@ static int prepare_launch(gcm_context* c) {
   return GCM_OK;
 }
 
-// Enqueue one evaluation of K points on stream s: the staged uploads (when
-// `copies`), the prologue, fill and finalize kernels, and the timing events
-// (when tev is given).  Used eagerly and under hipGraph stream capture.
+// Enqueue one evaluation of K points on stream s: with `copies` the staged
+// parameters (read by the prologue from pinned memory) and the host-table
+// upload, then the prologue, fill and finalize kernels (the last one writes
+// the results to pinned memory), and the timing events (when tev is given).
+// Used eagerly and under hipGraph stream capture.
 static int enqueue_fill(gcm_context* c, hipStream_t s, int K, bool copies,
                         hipEvent_t* tev) {
   if (copies) {
-    GCM_HIP(hipMemcpyAsync(c->d_theta, c->h_theta, (size_t)K * c->n_par * sizeof(double),
-                           hipMemcpyHostToDevice, s));
     if (c->table_len && c->host_tab)
       GCM_HIP(hipMemcpyAsync(c->d_tab, c->h_tab, (size_t)K * c->table_len * sizeof(double),
                              hipMemcpyHostToDevice, s));
@@ -1701,7 +1729,7 @@ static int enqueue_fill(gcm_context* c, hipStream_t s, int K, bool copies,
   // K-point launches of at least kPointsMin points: one lane per point
   const bool points = K >= kPointsMin && c->points_ready;
   const int n_tab_idx = c->tab_updates.empty() ? 0 : (int)c->tab_in_idx.size();
-  prologue_kernel<<<K, 256, 0, s>>>(c->d_theta, c->n_par, c->d_in_par, c->d_in_mlo,
+  prologue_kernel<<<K, 256, 0, s>>>(copies ? c->h_theta_dev : nullptr, c->d_theta, c->n_par, c->d_in_par, c->d_in_mlo,
                                     c->d_in_mrange, c->d_in_min, c->d_in_max,
                                     c->d_in_flags, c->n_inputs, c->d_xin,
                                     c->d_tab_in_idx, n_tab_idx, c->d_tab_in,
@@ -1811,13 +1839,10 @@ static int enqueue_fill(gcm_context* c, hipStream_t s, int K, bool copies,
         c->d_llh_part);
   GCM_HIP(hipGetLastError());
   finalize_total_kernel<<<K, kTotalBlock, 0, s>>>(c->d_llh_part, c->n_bins, c->d_penalty,
-                                                     c->d_stat, c->d_pen, c->d_total);
+                                                     c->d_stat, c->d_pen, c->d_total,
+                                                     c->d_flags, c->h_res_dev, c->max_points);
   GCM_HIP(hipGetLastError());
   if (tev) GCM_HIP(hipEventRecord(tev[2], s));
-  // results and the NaN flag to pinned memory, so that fetching them after
-  // the stream's synchronisation is a host copy
-  GCM_HIP(hipMemcpyAsync(c->h_res, c->d_res, (3 * (size_t)c->max_points + 1) * sizeof(double),
-                         hipMemcpyDeviceToHost, s));
   return GCM_OK;
 }
 

@@ -159,6 +159,32 @@ class ChainResult:
         return float((d * d).sum())
 
 
+def ess_batch_means(samples: np.ndarray, burn_frac: float = 0.1,
+                    n_batches: Optional[int] = None) -> np.ndarray:
+    """Effective sample size of each parameter by non-overlapping batch means
+    (Flegal & Jones 2010, Ann. Statist. 38:1034).  samples: [steps, chains, P].
+    The first burn_frac of the steps is dropped; each chain's remaining n
+    steps are cut into a batches of b = n // a steps (a = floor(sqrt(n)) by
+    default); sigma^2_BM = b * sum over all chains' batches of (batch mean -
+    pooled mean)^2 / (a C - 1), and ESS = a b C var(x) / sigma^2_BM.  A
+    parameter that never moves gets ESS 0."""
+    x = samples[int(samples.shape[0] * burn_frac):]
+    n, C_, P = x.shape
+    a = n_batches if n_batches else max(2, int(np.sqrt(n)))
+    b = n // a
+    if b < 1:
+        return np.zeros(P)
+    x = x[n - a * b:]  # the last a b steps
+    flat = x.reshape(-1, P)
+    mu = flat.mean(0)
+    var = flat.var(0, ddof=1)
+    bm = x.reshape(a, b, C_, P).mean(1)
+    s2 = b * ((bm - mu) ** 2).sum((0, 1)) / (a * C_ - 1)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        ess = np.where(s2 > 0, a * b * C_ * var / s2, 0.0)
+    return ess
+
+
 def adapt_cov(samples: np.ndarray, scale0: np.ndarray) -> np.ndarray:
     d = samples.shape[-1]
     flat = samples.reshape(-1, d)
