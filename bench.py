@@ -520,8 +520,8 @@ def posterior_agreement(args, log):
     chains' posterior sigma, with its Monte Carlo standard error from batch
     means (20 batches per chain set)."""
     from gundam_amd import CacheManager, Workload
-    from gundam_amd.mcmc import (ExactTarget, SurrogateTarget, adapt_cov, laplace_scales,
-                                 run_batched_exact, run_delayed, var_ratio)
+    from gundam_amd.mcmc import (ExactTarget, SurrogateTarget, adapt_cov, ess_batch_means,
+                                 laplace_scales, run_batched_exact, run_delayed, var_ratio)
     from gundam_amd.surrogate import Surrogate
 
     wk = 2 if args.post_workload == 2 else 1
@@ -602,6 +602,12 @@ def posterior_agreement(args, log):
         "max_abs_mean_diff_sigma": float(np.max(np.abs(diff))),
         "median_abs_mean_diff_sigma": float(np.median(np.abs(diff))),
         "max_mc_standard_error_sigma": float(np.max(se)),
+        # the exact chains' own batch-means standard error of the mean
+        "exact_mc_standard_error_sigma": {"median": float(np.median(se_ex / sigma)),
+                                          "max": float(np.max(se_ex / sigma))},
+        "ess_batch_means": {k: {"min": float(e.min()), "median": float(np.median(e))}
+                            for k, e in ((k, ess_batch_means(r.samples))
+                                         for k, r in res.items())},
         "params_within_0p05_sigma": int(np.sum(np.abs(diff) < 0.05)),
         "max_abs_z": float(np.max(np.abs(diff / se))),
         "burn_in_iterations": int(args.post_burn),

@@ -45,12 +45,16 @@ using gcm::fail;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-// Block tile 64 x 64 with a K step of 64; each of the four waves owns a
-// 32 x 32 sub-tile (one accumulator).  Operand tiles are double-buffered in
-// LDS as [row][k] BF16 with a padded row of 72 elements (144 B, 16-B aligned
-// for the 128-bit operand reads); the K step after next is loaded into
-// registers while the current one is multiplied, one barrier per step.
+// Block tile BM x BN (64 x 64, or 128 x 128 for products with plenty of
+// work) with a K step of 64; the four waves form a 2 x 2 grid, each owning a
+// (BM/2) x (BN/2) sub-tile of 32 x 32 accumulators.  Operand tiles are
+// double-buffered in LDS as [row][k] BF16 with a padded row of 72 elements
+// (144 B, 16-B aligned for the 128-bit operand reads); the K step after next
+// is loaded into registers while the current one is multiplied, one barrier
+// per step.  A 128 x 128 tile reads each operand byte once per 128 outputs
+// instead of 64, halving the operand traffic from L2 per FLOP.
 constexpr int TM = 64, TN = 64, TK = 64;
+constexpr int kBig = 128;  // the large tile's side
 constexpr int LDK = TK + 8;
 constexpr int kMaxSplit = 16;
 constexpr int kMinSplitSteps = 16;  // split only long reductions (K >= 1024)
@@ -67,32 +71,44 @@ struct GemmArgs {
   int64_t ldb;
   int vec;    // both operands meet the alignment of the unchecked vector loads
   int yfast;  // tile order: M tiles fastest (B the larger operand), else N tiles
-  float* C;  // C[m * ldc + n]; split z of a split-K launch writes C + z * split_stride
-  int64_t ldc, split_stride;
+  float* C;   // C[m * ldc + n]
+  int64_t ldc;
+  // split-K launches (gridDim.z > 1): split z writes its partial product to
+  // ws + z * split_stride (dense, ld N); a reduction launch adds them in
+  // split order and applies the epilogue
+  float* ws;
+  int64_t split_stride;
   int k_chunk;        // K range of one split, a multiple of TK
+  // epilogue: + bias[n]; GELU with the pre-activation kept in Z; or, with gz,
+  // the backward pass through a GELU (C *= gelu'(gz), C dense)
   const float* bias;  // [N] or null
   float* Z;           // pre-activation copy (GELU only) or null
   int gelu;
+  const float* gz;
 };
 
-// One 64 (rows) x 64 (k) operand tile held in registers between its global
-// load and its LDS store, for an operand that is K-contiguous (KC) or
-// row-contiguous, FP32 or BF16 (BF).  load() is unchecked along K (full K
-// steps of an aligned operand); load_checked() reads rows >= R and k >= kend
-// as zero.
-//  KC: thread t holds rows t/8 and t/8 + 32, k = 8 (t%8) .. + 7, read as
-//      16-byte vectors along k, stored as one 16-byte LDS write per row.
-//  row-contiguous: thread t holds rows 2 (t%32) and + 1, k = 8 (t/32) .. + 7,
-//      read as 2-element vectors along the rows (one per k), stored
-//      transposed as one 16-byte LDS write per row.
-template <bool KC, bool BF>
+// One ROWS x 64 (k) operand tile (ROWS = 64 or 128) held in registers
+// between its global load and its LDS store, for an operand that is
+// K-contiguous (KC) or row-contiguous, FP32 or BF16 (BF).  load() is
+// unchecked along K (full K steps of an aligned operand); load_checked()
+// reads rows >= R and k >= kend as zero.
+//  KC: thread t holds rows t/8 + 32 i (i < ROWS/32), k = 8 (t%8) .. + 7, read
+//      as 16-byte vectors along k, stored as one 16-byte LDS write per row.
+//  row-contiguous: thread t holds rows 2 (t%32) + e + 64 j (e < 2,
+//      j < ROWS/64), k = 8 (t/32) .. + 7, read as 2-element vectors along
+//      the rows (one per k), stored transposed as one 16-byte LDS write per
+//      row.
+template <bool KC, bool BF, int ROWS>
 struct Stage {
   using E = typename std::conditional<BF, __bf16, float>::type;
+  static constexpr int NI = ROWS / 32;          // KC: rows per thread
+  static constexpr int NJ = ROWS / 64;          // row-contiguous: row pairs per thread
+  static constexpr int NS = KC ? NI : 2 * NJ;   // rows per thread
   // raw loaded values: converted to BF16 (FP32 operands) only when stored,
   // so that the loads stay in flight across the multiply of the previous step
-  float f[2][8];   // FP32: [row][k]
-  bf16x8 hk[2];    // BF16, K-contiguous: [row] -> 8 k
-  bf16x2 hr[8];    // BF16, row-contiguous: [k] -> 2 rows
+  float f[NS][8];      // FP32: [row slot][k]
+  bf16x8 hk[NI];       // BF16, K-contiguous: [row slot] -> 8 k
+  bf16x2 hr[NJ][8];    // BF16, row-contiguous: [pair][k] -> 2 rows
 
   // Rows past R are read as row R - 1 (K-contiguous) or as rows R - 2, R - 1
   // (row-contiguous, R even): they only feed output rows that are not stored.
@@ -100,7 +116,7 @@ struct Stage {
                                        int tid) {
     if constexpr (KC) {
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
+      for (int i = 0; i < NI; ++i) {
         const int r = min(r0 + tid / 8 + 32 * i, R - 1);
         const E* p = base + (int64_t)r * ld + k0 + (tid % 8) * 8;
         if constexpr (BF) {
@@ -113,16 +129,19 @@ struct Stage {
         }
       }
     } else {
-      const int r = min(r0 + 2 * (tid % 32), R - 2);
-      const E* p = base + (int64_t)(k0 + 8 * (tid / 32)) * ld + r;
 #pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        if constexpr (BF) {
-          hr[q] = *reinterpret_cast<const bf16x2*>(p + q * ld);
-        } else {
-          const float2 v = *reinterpret_cast<const float2*>(p + q * ld);
-          f[0][q] = v.x;
-          f[1][q] = v.y;
+      for (int j = 0; j < NJ; ++j) {
+        const int r = min(r0 + 2 * (tid % 32) + 64 * j, R - 2);
+        const E* p = base + (int64_t)(k0 + 8 * (tid / 32)) * ld + r;
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          if constexpr (BF) {
+            hr[j][q] = *reinterpret_cast<const bf16x2*>(p + q * ld);
+          } else {
+            const float2 v = *reinterpret_cast<const float2*>(p + q * ld);
+            f[2 * j][q] = v.x;
+            f[2 * j + 1][q] = v.y;
+          }
         }
       }
     }
@@ -136,7 +155,7 @@ struct Stage {
     const E zero = (E)0.0f;
     if constexpr (KC) {
 #pragma unroll
-      for (int i = 0; i < 2; ++i) {
+      for (int i = 0; i < NI; ++i) {
         const int r = r0 + tid / 8 + 32 * i, k = k0 + (tid % 8) * 8;
         E v[8];
 #pragma unroll
@@ -152,40 +171,43 @@ struct Stage {
         }
       }
     } else {
-      const int r = r0 + 2 * (tid % 32), k = k0 + 8 * (tid / 32);
-      E v[8][2];
 #pragma unroll
-      for (int q = 0; q < 8; ++q)
+      for (int jj = 0; jj < NJ; ++jj) {
+        const int r = r0 + 2 * (tid % 32) + 64 * jj, k = k0 + 8 * (tid / 32);
+        E v[8][2];
 #pragma unroll
-        for (int e = 0; e < 2; ++e) {
-          const bool ok = r + e < R && k + q < kend;
-          v[q][e] = base[ok ? (int64_t)(k + q) * ld + r + e : 0];
-        }
+        for (int q = 0; q < 8; ++q)
 #pragma unroll
-      for (int q = 0; q < 8; ++q)
+          for (int e = 0; e < 2; ++e) {
+            const bool ok = r + e < R && k + q < kend;
+            v[q][e] = base[ok ? (int64_t)(k + q) * ld + r + e : 0];
+          }
 #pragma unroll
-        for (int e = 0; e < 2; ++e) {
-          const bool ok = r + e < R && k + q < kend;
-          if constexpr (BF) hr[q][e] = ok ? v[q][e] : zero;
-          else f[e][q] = ok ? v[q][e] : zero;
-        }
+        for (int q = 0; q < 8; ++q)
+#pragma unroll
+          for (int e = 0; e < 2; ++e) {
+            const bool ok = r + e < R && k + q < kend;
+            if constexpr (BF) hr[jj][q][e] = ok ? v[q][e] : zero;
+            else f[2 * jj + e][q] = ok ? v[q][e] : zero;
+          }
+      }
     }
   }
 
   __device__ __forceinline__ void store(__bf16* S, int tid) const {
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < NS; ++i) {
       bf16x8 v;
       if constexpr (BF && KC) {
         v = hk[i];
       } else {
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
-          if constexpr (BF) v[j] = hr[j][i];
+          if constexpr (BF) v[j] = hr[i / 2][j][i % 2];
           else v[j] = (__bf16)f[i][j];
         }
       }
-      const int r = KC ? tid / 8 + 32 * i : 2 * (tid % 32) + i;
+      const int r = KC ? tid / 8 + 32 * i : 2 * (tid % 32) + (i % 2) + 64 * (i / 2);
       const int k = KC ? (tid % 8) * 8 : 8 * (tid / 32);
       *reinterpret_cast<bf16x8*>(S + r * LDK + k) = v;
     }
@@ -200,29 +222,61 @@ __device__ __forceinline__ float gelu_grad(float z) {
          z * 0.39894228040143268f * expf(-0.5f * z * z);
 }
 
-// One K step (64) of the wave's 32 x 32 sub-tile from the LDS tiles.
-__device__ __forceinline__ void mfma_step(const __bf16* As, const __bf16* Bs, f32x16& acc,
-                                          int wm, int wn, int lane) {
-  const __bf16* as = As + (wm * 32 + lane % 32) * LDK + (lane / 32) * 8;
-  const __bf16* bs = Bs + (wn * 32 + lane % 32) * LDK + (lane / 32) * 8;
+// The GEMM epilogue of element (row, col) of C: bias (bcol = bias[col],
+// loaded by the caller), GELU (the pre-activation kept in Z) or the GELU
+// derivative, then the store.
+__device__ __forceinline__ void finish_store(const GemmArgs& g, int row, int col, float v,
+                                             float bcol) {
+  const int64_t o = (int64_t)row * g.ldc + col;
+  if (g.bias) v += bcol;
+  if (g.gelu) {
+    g.Z[o] = v;
+    v = gelu(v);
+  }
+  if (g.gz) v *= gelu_grad(g.gz[o]);
+  g.C[o] = v;
+}
+
+// One K step (64) of the wave's (BM/2) x (BN/2) sub-tile from the LDS tiles.
+template <int BM, int BN>
+__device__ __forceinline__ void mfma_step(const __bf16* As, const __bf16* Bs,
+                                          f32x16 (&acc)[BM / 64][BN / 64], int wm, int wn,
+                                          int lane) {
+  constexpr int MI = BM / 64, NJ = BN / 64;
+  const __bf16* as = As + (wm * (BM / 2) + lane % 32) * LDK + (lane / 32) * 8;
+  const __bf16* bs = Bs + (wn * (BN / 2) + lane % 32) * LDK + (lane / 32) * 8;
 #pragma unroll
   for (int kk = 0; kk < TK; kk += 16) {
-    const bf16x8 a = *reinterpret_cast<const bf16x8*>(as + kk);
-    const bf16x8 b = *reinterpret_cast<const bf16x8*>(bs + kk);
-    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    bf16x8 a[MI], b[NJ];
+#pragma unroll
+    for (int i = 0; i < MI; ++i) a[i] = *reinterpret_cast<const bf16x8*>(as + i * 32 * LDK + kk);
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) b[j] = *reinterpret_cast<const bf16x8*>(bs + j * 32 * LDK + kk);
+#pragma unroll
+    for (int i = 0; i < MI; ++i)
+#pragma unroll
+      for (int j = 0; j < NJ; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
   }
 }
 
-// Grid (ceil(N / 64), ceil(M / 64), splits); split z covers
+// LDS bytes of a BM x BN tile's double-buffered operands.
+constexpr int gemm_lds_bytes(int BM, int BN) { return 2 * (BM + BN) * LDK * 2; }
+
+// Grid (ceil(N / BN), ceil(M / BM), splits); split z covers
 // k in [z k_chunk, min(K, (z + 1) k_chunk)).  Aligned operands run the full
 // K steps pipelined with unchecked loads (edge tiles by clamping the rows);
 // the K tail and unaligned operands take checked, unpipelined steps.
-template <bool AKC, bool ABF, bool BKC, bool BBF>
+template <int BM, int BN, bool AKC, bool ABF, bool BKC, bool BBF>
 __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
-  __shared__ __align__(16) __bf16 As[2][TM * LDK];
-  __shared__ __align__(16) __bf16 Bs[2][TN * LDK];
-  using SA = Stage<AKC, ABF>;
-  using SB = Stage<BKC, BBF>;
+  extern __shared__ __align__(16) unsigned char gemm_lds[];
+  __bf16* const As[2] = {reinterpret_cast<__bf16*>(gemm_lds),
+                         reinterpret_cast<__bf16*>(gemm_lds) + BM * LDK};
+  __bf16* const Bs[2] = {reinterpret_cast<__bf16*>(gemm_lds) + 2 * BM * LDK,
+                         reinterpret_cast<__bf16*>(gemm_lds) + 2 * BM * LDK + BN * LDK};
+  constexpr int MI = BM / 64, NJ = BN / 64;
+  using SA = Stage<AKC, ABF, BM>;
+  using SB = Stage<BKC, BBF, BN>;
   const auto* A = reinterpret_cast<const typename SA::E*>(g.A);
   const auto* B = reinterpret_cast<const typename SB::E*>(g.B);
   const int tid = threadIdx.x;
@@ -245,10 +299,14 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
     by = (t / gx) % gy;
   }
   const int bz = t / (gx * gy);
-  const int m0 = by * TM, n0 = bx * TN;
+  const int m0 = by * BM, n0 = bx * BN;
   const int kb = bz * g.k_chunk;
   const int ke = min(g.K, kb + g.k_chunk);
-  f32x16 acc = {0};
+  f32x16 acc[MI][NJ];
+#pragma unroll
+  for (int i = 0; i < MI; ++i)
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) acc[i][j] = f32x16{0};
   SA sa, sa2;
   SB sb, sb2;
   const int nfull = (g.vec && ke > kb) ? (ke - kb) / TK : 0;
@@ -271,7 +329,7 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
       sa.load(A, g.lda, g.M, m0, kstep(st + 2), tid);
       sb.load(B, g.ldb, g.N, n0, kstep(st + 2), tid);
       __builtin_amdgcn_sched_barrier(0);  // issue the loads before the multiply
-      mfma_step(As[0], Bs[0], acc, wm, wn, lane);
+      mfma_step<BM, BN>(As[0], Bs[0], acc, wm, wn, lane);
       sa2.store(As[1], tid);
       sb2.store(Bs[1], tid);
       __syncthreads();
@@ -279,18 +337,18 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
       sa2.load(A, g.lda, g.M, m0, kstep(st + 3), tid);
       sb2.load(B, g.ldb, g.N, n0, kstep(st + 3), tid);
       __builtin_amdgcn_sched_barrier(0);
-      mfma_step(As[1], Bs[1], acc, wm, wn, lane);
+      mfma_step<BM, BN>(As[1], Bs[1], acc, wm, wn, lane);
       sa.store(As[0], tid);
       sb.store(Bs[0], tid);
       __syncthreads();
     }
     // one or two full steps left: LDS 0 holds step st, set 2 step st + 1
-    mfma_step(As[0], Bs[0], acc, wm, wn, lane);
+    mfma_step<BM, BN>(As[0], Bs[0], acc, wm, wn, lane);
     if (st + 1 < nfull) {
       sa2.store(As[1], tid);
       sb2.store(Bs[1], tid);
       __syncthreads();
-      mfma_step(As[1], Bs[1], acc, wm, wn, lane);
+      mfma_step<BM, BN>(As[1], Bs[1], acc, wm, wn, lane);
     }
     __syncthreads();
   }
@@ -301,52 +359,47 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
     sa.store(As[0], tid);
     sb.store(Bs[0], tid);
     __syncthreads();
-    mfma_step(As[0], Bs[0], acc, wm, wn, lane);
+    mfma_step<BM, BN>(As[0], Bs[0], acc, wm, wn, lane);
     __syncthreads();
   }
-  float* C = g.C + (int64_t)bz * g.split_stride;
-  const int col = n0 + wn * 32 + lane % 32;
-  const float bcol = (g.bias && col < g.N) ? g.bias[col] : 0.0f;  // one load per lane
+  const int splits = gridDim.z;
+  float* P = g.ws + (int64_t)bz * g.split_stride;
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int row = m0 + wm * 32 + (r / 4) * 8 + (lane / 32) * 4 + r % 4;
-    if (row < g.M && col < g.N) {
-      float v = acc[r];
-      if (g.bias) v += bcol;
-      if (g.gelu) {
-        g.Z[(int64_t)row * g.ldc + col] = v;
-        v = gelu(v);
+  for (int i = 0; i < MI; ++i)
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) {
+      const int col = n0 + wn * (BN / 2) + j * 32 + lane % 32;
+      const float bcol = (g.bias && col < g.N) ? g.bias[col] : 0.0f;  // one load per lane
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int row = m0 + wm * (BM / 2) + i * 32 + (r / 4) * 8 + (lane / 32) * 4 + r % 4;
+        if (row < g.M && col < g.N) {
+          if (splits == 1) finish_store(g, row, col, acc[i][j][r], bcol);
+          else P[(int64_t)row * g.N + col] = acc[i][j][r];
+        }
       }
-      C[(int64_t)row * g.ldc + col] = v;
     }
-  }
 }
 
-// Split-K epilogue: C = sum over s (in order) of ws[s][m][n], then the bias
-// and GELU of the forward pass (Z keeps the pre-activation) or, when gz is
-// given, the GELU derivative of the backward pass (C *= gelu'(gz)).
-__global__ void splitk_reduce_kernel(const float* __restrict__ ws, int splits, int M, int N,
-                                     float* __restrict__ C, int64_t ldc,
-                                     const float* __restrict__ bias, float* __restrict__ Z,
-                                     int gelu_out, const float* __restrict__ gz) {
-  const int64_t n = (int64_t)M * N;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    const int m = (int)(i / N), c = (int)(i % N);
-    float v = ws[i];
+// Split-K epilogue: C = sum over s (in order) of ws[s][m][n], then the
+// GEMM's epilogue (bias, GELU keeping the pre-activation, or the GELU
+// derivative of the backward pass).
+// Grid (column blocks, row blocks), both strided.
+__global__ void splitk_reduce_kernel(GemmArgs g, int splits) {
+  const int64_t n = (int64_t)g.M * g.N;
+  for (int row = blockIdx.y; row < g.M; row += gridDim.y)
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.N; c += gridDim.x * blockDim.x) {
+      const int64_t i = (int64_t)row * g.N + c;
+      float v = g.ws[i];
 #pragma unroll 8
-    for (int s = 1; s < splits; ++s) v += ws[s * n + i];
-    if (bias) v += bias[c];
-    const int64_t o = (int64_t)m * ldc + c;
-    if (gelu_out) {
-      Z[o] = v;
-      v = gelu(v);
+      for (int s = 1; s < splits; ++s) v += g.ws[s * n + i];
+      finish_store(g, row, c, v, g.bias ? g.bias[c] : 0.0f);
     }
-    if (gz) v *= gelu_grad(gz[o]);
-    C[o] = v;
-  }
 }
 
+// d *= gelu'(z) elementwise (the backward pass through a GELU layer after an
+// unsplit GEMM: a wide launch, where the GEMM's epilogue would run this on
+// the few threads of a small output tiling).
 __global__ void gelu_backward_kernel(float* __restrict__ d, const float* __restrict__ z,
                                      int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -354,41 +407,71 @@ __global__ void gelu_backward_kernel(float* __restrict__ d, const float* __restr
     d[i] *= gelu_grad(z[i]);
 }
 
-// db[n] = sum over rows of d[m][n]: block (64 columns x 16 row slices); each
-// slice sums its rows in order, the 16 slice sums are added in order.
+dim3 reduce_grid(int M, int N) {
+  return dim3((unsigned)std::min(32, (N + 255) / 256), (unsigned)std::min(M, 2048));
+}
+
+// Column sums over M rows in a fixed order: the rows are cut into S slices
+// (S = col_slices(M)), one block per (64 columns, slice); a block's 16 row
+// sub-slices sum their rows in order and are added in order; with S > 1 the
+// slice sums go to part[s][n] and colsum_final_kernel adds them in slice
+// order.
 constexpr int kColsumSlices = 16;
+constexpr int kMaxColSlices = 32;
+int col_slices(int M) { return std::max(1, std::min(kMaxColSlices, M / 256)); }
+
+// Rows [r0, r1) of a slice.
+__device__ __forceinline__ void slice_rows(int M, int S, int s, int* r0, int* r1) {
+  *r0 = (int)((int64_t)M * s / S);
+  *r1 = (int)((int64_t)M * (s + 1) / S);
+}
+
 __global__ void __launch_bounds__(64 * kColsumSlices)
-colsum_kernel(const float* __restrict__ d, int M, int N, float* __restrict__ db) {
+colsum_kernel(const float* __restrict__ d, int M, int N, float* __restrict__ out) {
   __shared__ float part[kColsumSlices][65];
   const int c = blockIdx.x * 64 + threadIdx.x;
+  int r0, r1;
+  slice_rows(M, gridDim.y, blockIdx.y, &r0, &r1);
   float s = 0.0f;
   if (c < N) {
 #pragma unroll 4
-    for (int m = threadIdx.y; m < M; m += kColsumSlices) s += d[(int64_t)m * N + c];
+    for (int m = r0 + threadIdx.y; m < r1; m += kColsumSlices) s += d[(int64_t)m * N + c];
   }
   part[threadIdx.y][threadIdx.x] = s;
   __syncthreads();
   if (threadIdx.y == 0 && c < N) {
     float t = 0.0f;
     for (int i = 0; i < kColsumSlices; ++i) t += part[i][threadIdx.x];
+    out[(size_t)blockIdx.y * N + c] = t;
+  }
+}
+
+// db[n] = sum over slices s (in order) of part[s][n].
+__global__ void colsum_final_kernel(const float* __restrict__ part, int S, int N,
+                                    float* __restrict__ db) {
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x) {
+    float t = part[c];
+    for (int s = 1; s < S; ++s) t += part[(size_t)s * N + c];
     db[c] = t;
   }
 }
 
 // The loss head of the backward pass, laid out like colsum_kernel:
-// dY = 2 (Y - T) / (M N), db[n] = the column sums of dY, and the block's sum
-// of squared errors in partial[blockIdx.x] (each in a fixed order).
+// dY = 2 (Y - T) / (M N), the (slice) column sums of dY, and the block's sum
+// of squared errors in partial[slice][blockIdx.x] (each in a fixed order).
 __global__ void __launch_bounds__(64 * kColsumSlices)
 mse_colsum_kernel(const float* __restrict__ y, const float* __restrict__ t,
-                  float* __restrict__ dy, int M, int N, float scale, float* __restrict__ db,
+                  float* __restrict__ dy, int M, int N, float scale, float* __restrict__ out,
                   float* __restrict__ partial) {
   __shared__ float part[kColsumSlices][65];
   __shared__ float sq[kColsumSlices][65];
   const int c = blockIdx.x * 64 + threadIdx.x;
+  int r0, r1;
+  slice_rows(M, gridDim.y, blockIdx.y, &r0, &r1);
   float s = 0.0f, e2 = 0.0f;
   if (c < N) {
 #pragma unroll 4
-    for (int m = threadIdx.y; m < M; m += kColsumSlices) {
+    for (int m = r0 + threadIdx.y; m < r1; m += kColsumSlices) {
       const int64_t i = (int64_t)m * N + c;
       const float e = y[i] - t[i];
       const float g = 2.0f * e * scale;
@@ -406,33 +489,41 @@ mse_colsum_kernel(const float* __restrict__ y, const float* __restrict__ t,
       a += part[i][threadIdx.x];
       q += sq[i][threadIdx.x];
     }
-    if (c < N) db[c] = a;
+    if (c < N) out[(size_t)blockIdx.y * N + c] = a;
     part[0][threadIdx.x] = q;
   }
   __syncthreads();
   if (threadIdx.y == 0 && threadIdx.x == 0) {
     float q = 0.0f;
     for (int i = 0; i < 64; ++i) q += part[0][i];
-    partial[blockIdx.x] = q;
+    partial[(size_t)blockIdx.y * gridDim.x + blockIdx.x] = q;
+  }
+}
+
+// One row of n floats (16-byte vectors when n is a multiple of 4: the rows
+// then start 16-byte aligned), by the threads of the blocks that share the
+// row (gridDim.y of them, block blockIdx.y of those).
+__device__ __forceinline__ void copy_row(const float* __restrict__ src, float* __restrict__ dst,
+                                         int n) {
+  const int t0 = blockIdx.y * blockDim.x + threadIdx.x, ts = gridDim.y * blockDim.x;
+  if ((n & 3) == 0) {
+    const float4* s4 = reinterpret_cast<const float4*>(src);
+    float4* d4 = reinterpret_cast<float4*>(dst);
+    for (int i = t0; i < n / 4; i += ts) d4[i] = s4[i];
+  } else {
+    for (int i = t0; i < n; i += ts) dst[i] = src[i];
   }
 }
 
 // Rows idx[0 .. rows) of the resident inputs and targets into the minibatch
-// buffers, in one launch.
+// buffers, in one launch: blocks (row, part of the row), rows strided.
 __global__ void gather_rows_kernel(const float* __restrict__ sx, const float* __restrict__ st,
                                    const int32_t* __restrict__ idx, int rows, int cx, int ct,
                                    float* __restrict__ dx, float* __restrict__ dt) {
-  const int64_t nx = (int64_t)rows * cx, n = nx + (int64_t)rows * ct;
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
-    if (i < nx) {
-      const int r = (int)(i / cx), c = (int)(i % cx);
-      dx[i] = sx[(int64_t)idx[r] * cx + c];
-    } else {
-      const int64_t j = i - nx;
-      const int r = (int)(j / ct), c = (int)(j % ct);
-      dt[j] = st[(int64_t)idx[r] * ct + c];
-    }
+  for (int r = blockIdx.x; r < rows; r += gridDim.x) {
+    const int64_t s = idx[r];
+    copy_row(sx + s * cx, dx + (int64_t)r * cx, cx);
+    copy_row(st + s * ct, dt + (int64_t)r * ct, ct);
   }
 }
 
@@ -452,21 +543,65 @@ __device__ __forceinline__ int64_t bf16_index(const Bf16Map& q, int64_t i) {
 }
 
 // Adam (Kingma & Ba 2015) on the FP32 master copy; refreshes the BF16 copy.
+// Four parameters per thread with 16-byte loads and stores (the arrays are
+// 16-byte aligned); their BF16 copies as one 8-byte store when they are
+// consecutive and aligned there, else one by one.
+__device__ __forceinline__ float adam_one(float p, float gi, float& m1, float& m2, float lr,
+                                          float b1, float b2, float eps, float c1, float c2) {
+  const float a = b1 * m1 + (1.0f - b1) * gi;
+  const float b = b2 * m2 + (1.0f - b2) * gi * gi;
+  m1 = a;
+  m2 = b;
+  return p - lr * (a / c1) / (sqrtf(b / c2) + eps);
+}
+
 __global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g,
                             float* __restrict__ m1, float* __restrict__ m2,
                             __bf16* __restrict__ pb, Bf16Map q, int64_t n, float lr, float b1,
                             float b2, float eps, float c1, float c2) {
-  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+  typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+  const int64_t n4 = n / 4;
+  for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n4;
+       v += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = 4 * v;
+    const float4 gv = *reinterpret_cast<const float4*>(g + i);
+    float4 a = *reinterpret_cast<const float4*>(m1 + i);
+    float4 b = *reinterpret_cast<const float4*>(m2 + i);
+    float4 w = *reinterpret_cast<const float4*>(p + i);
+    w.x = adam_one(w.x, gv.x, a.x, b.x, lr, b1, b2, eps, c1, c2);
+    w.y = adam_one(w.y, gv.y, a.y, b.y, lr, b1, b2, eps, c1, c2);
+    w.z = adam_one(w.z, gv.z, a.z, b.z, lr, b1, b2, eps, c1, c2);
+    w.w = adam_one(w.w, gv.w, a.w, b.w, lr, b1, b2, eps, c1, c2);
+    *reinterpret_cast<float4*>(m1 + i) = a;
+    *reinterpret_cast<float4*>(m2 + i) = b;
+    *reinterpret_cast<float4*>(p + i) = w;
+    const int64_t j0 = bf16_index(q, i), j3 = bf16_index(q, i + 3);
+    if (j0 >= 0 && j3 == j0 + 3 && (j0 & 3) == 0) {
+      bf16x4 h;
+      h[0] = (__bf16)w.x;
+      h[1] = (__bf16)w.y;
+      h[2] = (__bf16)w.z;
+      h[3] = (__bf16)w.w;
+      *reinterpret_cast<bf16x4*>(pb + j0) = h;
+    } else {
+      const float e[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+      for (int k = 0; k < 4; ++k) {
+        const int64_t j = bf16_index(q, i + k);
+        if (j >= 0) pb[j] = (__bf16)e[k];
+      }
+    }
+  }
+  // the n % 4 last parameters
+  for (int64_t i = 4 * n4 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const float gi = g[i];
-    const float a = b1 * m1[i] + (1.0f - b1) * gi;
-    const float b = b2 * m2[i] + (1.0f - b2) * gi * gi;
+    float a = m1[i], b = m2[i];
+    const float w = adam_one(p[i], g[i], a, b, lr, b1, b2, eps, c1, c2);
     m1[i] = a;
     m2[i] = b;
-    const float v = p[i] - lr * (a / c1) / (sqrtf(b / c2) + eps);
-    p[i] = v;
+    p[i] = w;
     const int64_t j = bf16_index(q, i);
-    if (j >= 0) pb[j] = (__bf16)v;
+    if (j >= 0) pb[j] = (__bf16)w;
   }
 }
 
@@ -581,7 +716,7 @@ struct gcm_mlp {
   int cap = 0;
   float *x = nullptr, *z1 = nullptr, *h1 = nullptr, *z2 = nullptr, *h2 = nullptr,
         *y = nullptr, *t = nullptr, *dy = nullptr, *d2 = nullptr, *d1 = nullptr,
-        *partial = nullptr;
+        *partial = nullptr, *colpart = nullptr;
   // split-K partial sums
   float* ws = nullptr;
   size_t ws_cap = 0;
@@ -651,48 +786,89 @@ bool vec_ok(const void* p, int64_t ld, int kc, int bf16, int R) {
   return a % (2 * es) == 0 && ld % 2 == 0 && R % 2 == 0;
 }
 
-hipError_t launch_gemm(int sel, dim3 grid, hipStream_t stream, const GemmArgs& g) {
+// The kernel instance for a tile side (64 or 128) and operand layout
+// selector sel = A K-contiguous | A BF16 | B K-contiguous | B BF16 (bits 3..0).
+template <int T>
+const void* gemm_fn(int sel) {
   switch (sel) {
-#define GCM_GEMM_CASE(i)                                                                   \
-  case i:                                                                                  \
-    gemm_bf16_kernel<((i)&8) != 0, ((i)&4) != 0, ((i)&2) != 0, ((i)&1) != 0><<<grid, 256, 0, \
-                                                                             stream>>>(g); \
-    break;
+#define GCM_GEMM_CASE(i) \
+  case i:                \
+    return (const void*)gemm_bf16_kernel<T, T, ((i)&8) != 0, ((i)&4) != 0, ((i)&2) != 0, ((i)&1) != 0>;
     GCM_GEMM_CASE(0) GCM_GEMM_CASE(1) GCM_GEMM_CASE(2) GCM_GEMM_CASE(3)
     GCM_GEMM_CASE(4) GCM_GEMM_CASE(5) GCM_GEMM_CASE(6) GCM_GEMM_CASE(7)
     GCM_GEMM_CASE(8) GCM_GEMM_CASE(9) GCM_GEMM_CASE(10) GCM_GEMM_CASE(11)
     GCM_GEMM_CASE(12) GCM_GEMM_CASE(13) GCM_GEMM_CASE(14) GCM_GEMM_CASE(15)
 #undef GCM_GEMM_CASE
   }
-  return hipGetLastError();
+  return nullptr;
+}
+
+// The large tile's LDS is past the default 64 KB: allow it (once per process).
+hipError_t gemm_set_lds_attrs() {
+  for (int sel = 0; sel < 16; ++sel) {
+    const hipError_t e = hipFuncSetAttribute(gemm_fn<kBig>(sel),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             gemm_lds_bytes(kBig, kBig));
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+hipError_t launch_gemm(bool big, int sel, dim3 grid, hipStream_t stream, const GemmArgs& g) {
+  const void* fn = big ? gemm_fn<kBig>(sel) : gemm_fn<TM>(sel);
+  GemmArgs a = g;
+  void* args[] = {&a};
+  const hipError_t e = hipLaunchKernel(fn, grid, dim3(256), args,
+                                       big ? gemm_lds_bytes(kBig, kBig) : gemm_lds_bytes(TM, TN),
+                                       stream);
+  return e != hipSuccess ? e : hipGetLastError();
 }
 
 // C[M x N] = A . B (+ bias, GELU with the pre-activation in Z), operand
 // layouts as in GemmArgs.  With gz, C is dense (ldc == N) and is multiplied by
-// gelu'(gz) elementwise (the backward pass through a GELU layer).  Output
-// tilings of fewer than 128 tiles with at least 16 K steps are split along K,
-// towards 1024 blocks, into up to 16 chunks of at least two K steps (a
-// block's K steps are bound by load latency, so more blocks in flight finish
-// sooner; shorter reductions gain less than the extra reduction launch
-// costs); the partials go to a workspace and are summed in split order, so
-// the result does not depend on scheduling.
-// Split count of an M x N x K product and its K steps per split.
-void gemm_splits(int M, int N, int K, int* splits_out, int* chunk_out) {
-  const int tiles = ((M + TM - 1) / TM) * ((N + TN - 1) / TN);
+// gelu'(gz) elementwise (the backward pass through a GELU layer).
+//
+// Tile and split-K plan of an M x N x K product (the same on every path, so
+// that every entry point computes the same sums in the same order):
+//  - the 128 x 128 tile when M, N >= 128, K has at least four steps and the
+//    product has at least 4096 tile-K-steps of work; else 64 x 64;
+//  - split along K (a block's K steps are bound by load latency, so more
+//    blocks in flight finish sooner; shorter reductions gain less than the
+//    extra reduction launch costs):
+//      64 x 64: outputs of fewer than 128 tiles with at least 16 K steps,
+//        towards 1024 blocks, into up to 16 chunks of at least two K steps;
+//      128 x 128: outputs of fewer than 384 tiles with at least 16 K steps,
+//        towards 512 blocks, into up to 8 chunks of at least eight K steps.
+//    The partials go to a workspace and a reduction launch sums them in
+//    split order, so the result does not depend on scheduling.
+struct GemmPlan {
+  bool big;
+  int T, splits, chunk_steps;
+};
+
+GemmPlan gemm_plan(int M, int N, int K) {
   const int steps = std::max(1, (K + TK - 1) / TK);
+  const int64_t big_tiles = (int64_t)((M + kBig - 1) / kBig) * ((N + kBig - 1) / kBig);
+  GemmPlan p{};
+  p.big = M >= kBig && N >= kBig && steps >= 4 && big_tiles * steps >= 4096;
+  p.T = p.big ? kBig : TM;
+  const int tiles = ((M + p.T - 1) / p.T) * ((N + p.T - 1) / p.T);
   int splits = 1;
-  if (tiles < 128 && steps >= kMinSplitSteps)
+  if (p.big) {
+    if (tiles < 384 && steps >= kMinSplitSteps)
+      splits = std::max(1, std::min({steps / 8, (512 + tiles - 1) / tiles, 8}));
+  } else if (tiles < 128 && steps >= kMinSplitSteps) {
     splits = std::max(1, std::min({steps / 2, (1024 + tiles - 1) / tiles, kMaxSplit}));
-  const int chunk_steps = (steps + splits - 1) / splits;
-  *splits_out = (steps + chunk_steps - 1) / chunk_steps;
-  *chunk_out = chunk_steps;
+  }
+  p.chunk_steps = (steps + splits - 1) / splits;
+  p.splits = (steps + p.chunk_steps - 1) / p.chunk_steps;
+  return p;
 }
 
 // Split-K partial floats an M x N x K product needs (0 when not split).
 size_t gemm_ws_floats(int M, int N, int K) {
-  int splits, chunk;
-  gemm_splits(M, N, K, &splits, &chunk);
-  return splits > 1 ? (size_t)splits * M * N : 0;
+  const GemmPlan p = gemm_plan(M, N, K);
+  return p.splits > 1 ? (size_t)p.splits * M * N : 0;
 }
 
 // Where a GEMM runs: a stream and a fixed split-K workspace that is never
@@ -710,15 +886,15 @@ int gemm(gcm_mlp* m, int M, int N, int K, const void* A, int64_t lda, int a_kc, 
   if (M <= 0 || N <= 0) return GCM_OK;
   if (gz && ldc != N) return fail(GCM_ERR_ARG, "gemm: GELU backward needs a dense output");
   const hipStream_t stream = ex ? ex->stream : m->stream;
-  int splits, chunk_steps;
-  gemm_splits(M, N, K, &splits, &chunk_steps);
+  const GemmPlan p = gemm_plan(M, N, K);
   GemmArgs g{M, N, K, A, lda, B, ldb, vec_ok(A, lda, a_kc, a_bf16, M) && vec_ok(B, ldb, b_kc, b_bf16, N),
              (int64_t)N * K * (b_bf16 ? 2 : 4) > (int64_t)M * K * (a_bf16 ? 2 : 4),
-             C, ldc, 0, chunk_steps * TK, bias, Z, Z != nullptr};
+             C, ldc, nullptr, 0, p.chunk_steps * TK, bias, Z, Z != nullptr, gz};
   const int sel = (a_kc ? 8 : 0) | (a_bf16 ? 4 : 0) | (b_kc ? 2 : 0) | (b_bf16 ? 1 : 0);
-  dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM, splits);
-  if (splits == 1) {
-    MLP_HIP(launch_gemm(sel, grid, stream, g));
+  const dim3 grid((N + p.T - 1) / p.T, (M + p.T - 1) / p.T, p.splits);
+  if (p.splits == 1) {
+    g.gz = nullptr;
+    MLP_HIP(launch_gemm(p.big, sel, grid, stream, g));
     if (gz) {
       gelu_backward_kernel<<<grid_for((int64_t)M * N), 256, 0, stream>>>(C, gz, (int64_t)M * N);
       MLP_HIP(hipGetLastError());
@@ -726,24 +902,17 @@ int gemm(gcm_mlp* m, int M, int N, int K, const void* A, int64_t lda, int a_kc, 
     return GCM_OK;
   }
   const int64_t mn = (int64_t)M * N;
-  float* ws = nullptr;
-  if (ex) {
-    if ((size_t)splits * mn > ex->ws_cap) return fail(GCM_ERR_ARG, "gemm: workspace too small");
-    ws = ex->ws;
-  } else {
-    int rc = m->ensure_ws((size_t)splits * mn);
-    if (rc) return rc;
-    ws = m->ws;
-  }
-  g.C = ws;
-  g.ldc = N;
   g.split_stride = mn;
-  g.bias = nullptr;
-  g.Z = nullptr;
-  g.gelu = 0;
-  MLP_HIP(launch_gemm(sel, grid, stream, g));
-  splitk_reduce_kernel<<<grid_for(mn), 256, 0, stream>>>(ws, splits, M, N, C, ldc, bias, Z,
-                                                         Z != nullptr, gz);
+  if (ex) {
+    if ((size_t)p.splits * mn > ex->ws_cap) return fail(GCM_ERR_ARG, "gemm: workspace too small");
+    g.ws = ex->ws;
+  } else {
+    int rc = m->ensure_ws((size_t)p.splits * mn);
+    if (rc) return rc;
+    g.ws = m->ws;
+  }
+  MLP_HIP(launch_gemm(p.big, sel, grid, stream, g));
+  splitk_reduce_kernel<<<reduce_grid(M, N), 256, 0, stream>>>(g, p.splits);
   MLP_HIP(hipGetLastError());
   return GCM_OK;
 }
@@ -769,14 +938,27 @@ int forward(gcm_mlp* m, int rows) {
 }
 
 // Loss and gradients into m->g for the rows in m->x / m->t.
+// Loss and gradients into m->g for the rows in m->x / m->t.
 int backward(gcm_mlp* m, int rows, double* loss) {
   const int64_t ny = (int64_t)rows * m->n_out;
-  const int nb = (m->n_out + 63) / 64;  // mse_colsum blocks, one partial each
+  const int nb = (m->n_out + 63) / 64;  // mse_colsum column blocks
+  const int S = col_slices(rows);        // row slices: one partial per (slice, block)
   const __bf16* w = m->pb;
   float* g = m->g;
   const dim3 cs_block(64, kColsumSlices);
-  mse_colsum_kernel<<<nb, cs_block, 0, m->stream>>>(m->y, m->t, m->dy, rows, m->n_out,
-                                                    1.0f / (float)ny, g + m->off_b3, m->partial);
+  // column sums of an M x N matrix into db: one pass, or slices and a final pass
+  auto colsum = [&](const float* d, int N, float* db) {
+    colsum_kernel<<<dim3((N + 63) / 64, S), cs_block, 0, m->stream>>>(d, rows, N,
+                                                                       S > 1 ? m->colpart : db);
+    if (S > 1)
+      colsum_final_kernel<<<(N + 255) / 256, 256, 0, m->stream>>>(m->colpart, S, N, db);
+  };
+  mse_colsum_kernel<<<dim3(nb, S), cs_block, 0, m->stream>>>(
+      m->y, m->t, m->dy, rows, m->n_out, 1.0f / (float)ny, S > 1 ? m->colpart : g + m->off_b3,
+      m->partial);
+  if (S > 1)
+    colsum_final_kernel<<<(m->n_out + 255) / 256, 256, 0, m->stream>>>(m->colpart, S, m->n_out,
+                                                                       g + m->off_b3);
   MLP_HIP(hipGetLastError());
   // dW3 = dY^T h2 : (n_out x rows) . (rows x H)
   int rc = gemm(m, m->n_out, m->H, rows, m->dy, m->n_out, 0, 0, m->h2, m->H, 0, 0,
@@ -790,7 +972,7 @@ int backward(gcm_mlp* m, int rows, double* loss) {
   rc = gemm(m, m->H, m->H, rows, m->d2, m->H, 0, 0, m->h1, m->H, 0, 0, g + m->off_w2, m->H,
             nullptr, nullptr);
   if (rc) return rc;
-  colsum_kernel<<<(m->H + 63) / 64, cs_block, 0, m->stream>>>(m->d2, rows, m->H, g + m->off_b2);
+  colsum(m->d2, m->H, g + m->off_b2);
   // dH1 = (dH2 W2) * gelu'(z1)
   rc = gemm(m, rows, m->H, m->H, m->d2, m->H, 1, 0, w + m->bmap.bw2, m->H, 0, 1, m->d1, m->H,
             nullptr, nullptr, m->z1);
@@ -799,11 +981,12 @@ int backward(gcm_mlp* m, int rows, double* loss) {
   rc = gemm(m, m->H, m->n_in, rows, m->d1, m->H, 0, 0, m->x, m->n_in, 0, 0, g + m->off_w1,
             m->n_in, nullptr, nullptr);
   if (rc) return rc;
-  colsum_kernel<<<(m->H + 63) / 64, cs_block, 0, m->stream>>>(m->d1, rows, m->H, g + m->off_b1);
+  colsum(m->d1, m->H, g + m->off_b1);
   MLP_HIP(hipGetLastError());
   if (loss) {
-    std::vector<float> part(nb);
-    MLP_HIP(hipMemcpyAsync(part.data(), m->partial, nb * sizeof(float), hipMemcpyDeviceToHost, m->stream));
+    std::vector<float> part((size_t)nb * S);
+    MLP_HIP(hipMemcpyAsync(part.data(), m->partial, part.size() * sizeof(float),
+                           hipMemcpyDeviceToHost, m->stream));
     MLP_HIP(hipStreamSynchronize(m->stream));
     double s = 0;
     for (float v : part) s += v;
@@ -847,10 +1030,13 @@ int gcm_mlp_create(int device, int n_in, int n_hidden, int n_out, uint64_t seed,
   step(m->alloc(&m->m1, m->n_params));
   step(m->alloc(&m->m2, m->n_params));
   step(m->alloc(&m->pb, m->n_bf16));
-  step(m->alloc(&m->partial, (size_t)(n_out + 63) / 64));  // one per mse_colsum block
+  step(m->alloc(&m->partial, (size_t)(n_out + 63) / 64 * kMaxColSlices));  // mse_colsum blocks
+  step(m->alloc(&m->colpart, (size_t)kMaxColSlices * std::max(n_out, n_hidden)));
   if (rc == GCM_OK && hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess)
     rc = fail(GCM_ERR_HIP, "stream creation failed");
   step(noalloc_scratch(device, &m->scratch));
+  if (rc == GCM_OK && gemm_set_lds_attrs() != hipSuccess)
+    rc = fail(GCM_ERR_HIP, "GEMM LDS attribute");
   if (rc != GCM_OK) {
     for (void* q : m->allocs) (void)hipFree(q);
     delete m;
@@ -1085,7 +1271,7 @@ int gcm_mlp_train(gcm_mlp* m, int epochs, int batch, float lr, uint64_t seed,
     int nbatch = 0, nsampled = 0;
     for (int64_t r0 = 0; r0 + batch <= m->n_data; r0 += batch, ++nbatch) {
       const int32_t* idx = m->order + r0;
-      gather_rows_kernel<<<grid_for((int64_t)batch * (m->n_in + m->n_out)), 256, 0, m->stream>>>(
+      gather_rows_kernel<<<dim3(std::min(batch, 4096), 4), 256, 0, m->stream>>>(
           m->dx, m->dt, idx, batch, m->n_in, m->n_out, m->x, m->t);
       if ((rc = forward(m, batch))) return rc;
       // the loss is fetched (a synchronisation) on every 16th minibatch only
@@ -1099,7 +1285,7 @@ int gcm_mlp_train(gcm_mlp* m, int epochs, int batch, float lr, uint64_t seed,
       m->adam_t++;
       const float c1 = 1.0f - std::pow(b1, (float)m->adam_t);
       const float c2 = 1.0f - std::pow(b2, (float)m->adam_t);
-      adam_kernel<<<grid_for(m->n_params), 256, 0, m->stream>>>(m->p, m->g, m->m1, m->m2, m->pb, m->bmap,
+      adam_kernel<<<grid_for((m->n_params + 3) / 4), 256, 0, m->stream>>>(m->p, m->g, m->m1, m->m2, m->pb, m->bmap,
                                                                 m->n_params, lr, b1, b2, eps, c1, c2);
       MLP_HIP(hipGetLastError());
     }

@@ -25,10 +25,15 @@ def _rel(a, b):
 # every GEMM of the step is split 2-5 ways along K; at the second the
 # 3000-long reduction of the output layer's input gradient is split sixteen
 # ways.  (300, 512, 6000) is the 5M-event model's network (300 parameters,
-# 6000 bins) at the training minibatch of 256 rows.
+# 6000 bins) at the training minibatch of 256 rows and at 4096 rows, where
+# the output layer's forward product takes the 128 x 128 tile.
+# (37, 700, 1500) at 3000 rows puts that tile on edges that are not tile
+# multiples in both dimensions, with a K tail (700 = 10 x 64 + 60), next to
+# an unaligned input layer.
 @pytest.mark.parametrize("shape,rows", [((20, 64, 40), 100), ((37, 96, 130), 257),
                                         ((300, 512, 600), 256), ((24, 64, 3000), 256),
-                                        ((300, 512, 6000), 256)])
+                                        ((300, 512, 6000), 256), ((300, 512, 6000), 4096),
+                                        ((37, 700, 1500), 3000)])
 def test_mlp_forward_and_gradients_match_fp64(shape, rows):
     print("mlp grad check", shape, rows, flush=True)
     n_in, h, n_out = shape
@@ -151,3 +156,78 @@ def test_surrogate_delayed_acceptance_gain():
           "; squared jump per exact evaluation, 4-step sub-chains vs exact", jump_gain)
     assert gain > 1.5
     assert jump_gain > 2.0
+
+
+def test_surrogate_delayed_acceptance_c2_model():
+    """Delayed acceptance on the 5M-event model at its stated network
+    (300 -> 512 -> 512 -> 6000, device table refresh): the surrogate, trained
+    on 16k exact pairs, agrees with the exact chi2 up to a smooth offset; the
+    device likelihood head equals the host Barlow-Beeston restatement; the
+    delayed-acceptance chains (32, from the exact burn-in states) stay on the
+    exact chains' posterior at these lengths and spend fewer exact
+    evaluations per accepted move."""
+    print("surrogate DA-MCMC workload=2 (5M events)", flush=True)
+    from gundam_amd import _native as N
+    wl = Workload(2, seed=3)
+    t = wl.tables
+    wl.bin_events_on_device(0)
+    C_ = 32
+    cm = CacheManager(0, block_threads=384, tile_words=3840, blocks_per_cu=2, ring_buffers=2)
+    cm.build(t, max_points=64)
+    cm.add_table_update(N.TABLE_UPDATE_COS, N.TABLE_UPDATE_COS_KERNEL, wl.table_jobs())
+    mode = wl.nominal()
+    cm.propagate_parameters(mode)
+    data, err = (a.copy() for a in cm.hist(0))
+    ratio = var_ratio(data, err)
+    cm.set_likelihood("barlow", data, wl.prior_mean, wl.prior_sigma)
+    exact = ExactTarget(cm)
+    sc = laplace_scales(exact, mode)
+    d = mode.size
+    assert d == 300 and t.n_bins == 6000
+    rng = np.random.default_rng(11)
+    exact.record = True
+    burn = run_batched_exact(exact, mode + sc * rng.standard_normal((C_, d)),
+                             (2.38 ** 2 / d) * np.diag(sc ** 2), 120, rng)
+    exact(mode + sc * rng.standard_normal((16000 - exact.points, d)))
+    exact.record = False
+    X, Y = exact.recorded()
+    print(f"{X.shape[0]} training pairs", flush=True)
+    cov = adapt_cov(burn.samples[60:], sc)
+    sur = Surrogate(d, t.n_bins, hidden=512, seed=2)
+    losses = sur.fit(X, Y, epochs=120, batch=256, lr=1e-3)
+    assert losses[-1] < 0.5 * losses[0], (losses[0], losses[-1])
+    surr = SurrogateTarget(sur, data, ratio, wl.prior_mean, wl.prior_sigma)
+    test = mode + sc * rng.standard_normal((256, d))
+    pred = sur.predict_sums(test)
+    np.testing.assert_allclose(sur.chi2(test), barlow_llh(data, pred, pred * ratio), rtol=1e-9)
+    # surrogate minus exact chi2 = an offset plus a spread well below the
+    # spread of the exact statistic term over the same points (what a
+    # surrogate that ignores theta would leave; the prior term is exact in
+    # both)
+    dchi2 = 2.0 * (surr(test) - exact(test))
+    cen = np.abs(dchi2 - np.median(dchi2))
+    stat = []
+    for i in range(0, test.shape[0], 64):
+        cm.set_parameters(test[i:i + 64])
+        cm.fill()
+        stat.append(cm.llh()[1].copy())
+    stat = np.concatenate(stat)
+    mad = np.median(np.abs(stat - np.median(stat)))
+    print("surrogate - exact chi2: offset", np.median(dchi2), "|centred| p50", np.median(cen),
+          "exact stat term spread (MAD)", mad, flush=True)
+    assert np.median(cen) < 0.8 * mad
+    start = burn.samples[-1]
+    da = run_delayed(exact, surr, start.copy(), cov, 300, rng)
+    print("delayed chains done", flush=True)
+    ex = run_batched_exact(exact, start.copy(), cov, 300, rng)
+    a = da.samples[100:].reshape(-1, d)
+    b = ex.samples[100:].reshape(-1, d)
+    diff = np.abs(a.mean(0) - b.mean(0)) / sc
+    print("posterior mean difference / sigma: median", np.median(diff), "max", diff.max(),
+          flush=True)
+    assert np.median(diff) < 0.15 and diff.max() < 0.6
+    gain = (ex.exact_evals / ex.accepted) / (da.exact_evals / da.accepted)
+    print("exact evaluations per accepted move: exact", ex.exact_evals / ex.accepted,
+          "delayed", da.exact_evals / da.accepted, "gain", gain, flush=True)
+    assert gain > 1.5
+    cm.close()
