@@ -145,15 +145,37 @@ __device__ __forceinline__ void lds_barrier() {
 
 // Producer wave: copy `bytes` (a multiple of 16) from global memory straight
 // into LDS with 16-byte-per-lane DMA loads; lane i of a chunk lands at
-// dst + chunk + 16 i, and lanes past the end are masked off.
+// dst + chunk + 16 i.  src and dst are wave-uniform: each load takes its
+// address as a scalar base plus the lane's fixed offset, so whole chunks cost
+// no vector instruction; only the last, partial chunk masks lanes off.
 __device__ __forceinline__ void dma_to_lds(const unsigned char* src,
                                            unsigned char* dst, int bytes,
                                            int lane) {
-  for (int j = 0; j < bytes; j += 1024) {
-    if (j + lane * 16 < bytes)
-      __builtin_amdgcn_global_load_lds((global_void_ptr)(src + j + lane * 16),
-                                       (lds_void_ptr)(dst + j), 16, 0, kStreamPolicy);
-  }
+  const uint32_t off = (uint32_t)lane * 16u;
+  int j = 0;
+  for (; j + 1024 <= bytes; j += 1024)
+    __builtin_amdgcn_global_load_lds((global_void_ptr)(src + j + off), (lds_void_ptr)(dst + j),
+                                     16, 0, kStreamPolicy);
+  if (j < bytes && j + (int)off < bytes)
+    __builtin_amdgcn_global_load_lds((global_void_ptr)(src + j + off), (lds_void_ptr)(dst + j),
+                                     16, 0, kStreamPolicy);
+}
+
+// A tile header with every field in scalar registers (the producer's DMA
+// addresses are then scalar).
+__device__ __forceinline__ TileHdr uniform_header(const TileHdr& h) {
+  TileHdr u;
+  u.first_event = __builtin_amdgcn_readfirstlane(h.first_event);
+  u.first_dial = __builtin_amdgcn_readfirstlane(h.first_dial);
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)h.rec_begin);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(h.rec_begin >> 32));
+  u.rec_begin = ((uint64_t)hi << 32) | lo;
+  u.rec_words = __builtin_amdgcn_readfirstlane(h.rec_words);
+  u.n_events = (uint16_t)__builtin_amdgcn_readfirstlane(h.n_events);
+  u.n_dials = (uint16_t)__builtin_amdgcn_readfirstlane(h.n_dials);
+  u.first_partial = __builtin_amdgcn_readfirstlane(h.first_partial);
+  u.grid_begin = __builtin_amdgcn_readfirstlane(h.grid_begin);
+  return u;
 }
 
 template <typename R>

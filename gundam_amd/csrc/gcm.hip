@@ -217,7 +217,209 @@ __global__ void penalty_sets_kernel(const double* __restrict__ theta, int n_par,
 //   waves C+1 ..   NB-1 producers: producer p streams the tiles whose ordinal
 //                  is p mod (NB-1) HBM -> LDS with DMA loads, NB-1 tiles
 //                  ahead, into a ring of NB buffers.
-// A step is one (tile, point); every wave meets one barrier per step.
+// A step is one (tile, point); every wave meets one barrier per step.  Each
+// role runs its own loop over the block's steps (the barriers pair up by
+// count, not by place), so that a role keeps only its own state live across
+// steps: with one shared loop the union of the three roles' state spilled
+// scalar registers, reloaded by every wave at every step.
+
+// Producer p: at the first step of tile i, producer i mod P loads tile i + P
+// into the buffer of tile i - 1; at the last step of tile i, producer
+// (i + 1) mod P waits for tile i + 1.  The next tile's header is held in
+// registers one DMA period ahead, so that no DMA waits on a header load.
+template <typename R, int NB>
+__device__ __forceinline__ void fill_producer(const FillArgs& a, unsigned char* ring,
+                                              size_t buf_bytes, size_t hdr_off, int me,
+                                              int lane, int64_t t0, int64_t G, TileHdr hq) {
+  constexpr int P = NB - 1;
+  const unsigned char* rec = reinterpret_cast<const unsigned char*>(a.rec);
+  const unsigned char* desc = reinterpret_cast<const unsigned char*>(a.desc);
+  int64_t i = 0;
+  for (int64_t t = t0; t < a.n_tiles; t += G, ++i) {
+    for (int k = 0; k < a.k_points; ++k) {
+      if (k == 0 && me == (int)(i % P)) {
+        const int64_t tn = t + P * G;
+        if (tn < a.n_tiles) {
+          unsigned char* nbuf = ring + (size_t)((i + P) % NB) * buf_bytes;
+          dma_tile<R>(hq, rec, desc, a.ev, nbuf, a.rec_bytes, a.max_dials, lane);
+          dma_header(a.tiles + tn, nbuf, hdr_off, lane);
+          if (tn + P * G < a.n_tiles) hq = uniform_header(a.tiles[tn + P * G]);
+        }
+      }
+      if (k == a.k_points - 1 && me == (int)((i + 1) % P))
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      lds_barrier();
+    }
+  }
+  // no LDS-DMA load of this block may land after its LDS is handed on
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
+// Event wave: the event weights and bin partials of step s, formed after the
+// barrier that ends step s (while the evaluators run step s + 1).
+template <bool kComp, int NB>
+__device__ __forceinline__ void fill_events(const FillArgs& a, unsigned char* ring,
+                                            size_t buf_bytes, size_t hdr_off,
+                                            double* resp_base, int lane, int64_t t0,
+                                            int64_t G) {
+  int step = 0;
+  int64_t i = 0;
+  for (int64_t t = t0; t < a.n_tiles; t += G, ++i) {
+    const unsigned char* cur = ring + (size_t)(i % NB) * buf_bytes;
+    const TileHdr h = *reinterpret_cast<const TileHdr*>(cur + hdr_off);
+    const int n_ev = __builtin_amdgcn_readfirstlane(h.n_events);
+    const uint32_t first_partial = __builtin_amdgcn_readfirstlane(h.first_partial);
+    const uint32_t first_event = __builtin_amdgcn_readfirstlane(h.first_event);
+    for (int k = 0; k < a.k_points; ++k, ++step) {
+      const double* resp = resp_base + (size_t)(step & 1) * a.max_dials;
+      // The event record is read before the barrier: after it, a producer
+      // may refill this buffer.
+      EventRec er{0.0, INT_MAX, 0u};
+      if (lane < n_ev)
+        er = reinterpret_cast<const EventRec*>(cur + a.rec_bytes + (size_t)a.max_dials * 8)[lane];
+      lds_barrier();
+      double w = 0.0, w2 = 0.0, c = 0.0, c2 = 0.0;
+      int bin = INT_MAX;
+      if (lane < n_ev) {
+        const int b0 = (int)(er.dials & 0xffff), cnt = (int)(er.dials >> 16);
+        // EventDialCache.cpp:159-170: the product in dial-list order; the
+        // loads are batched four at a time, the multiplies stay in order.
+        double temp = 1;
+        for (int j = 0; j < cnt; j += 4) {
+          const double r0 = resp[b0 + j];
+          const double r1 = (j + 1 < cnt) ? resp[b0 + j + 1] : 1.0;
+          const double r2 = (j + 2 < cnt) ? resp[b0 + j + 2] : 1.0;
+          const double r3 = (j + 3 < cnt) ? resp[b0 + j + 3] : 1.0;
+          temp *= r0;
+          if (j + 1 < cnt) temp *= r1;
+          if (j + 2 < cnt) temp *= r2;
+          if (j + 3 < cnt) temp *= r3;
+        }
+        if (temp > a.cap) temp = a.cap;
+        w = er.base;
+        w *= temp;
+        bin = er.bin;
+        if (k == 0 && a.weights_out)
+          a.weights_out[a.ev_result[(int64_t)first_event + lane]] = w;
+        if (isnan(w)) atomicOr(a.flags, 1);
+        w2 = w * w;
+      }
+      // Inclusive segmented scan over the wave; bins are sorted, so equal
+      // bins are one contiguous segment.
+#pragma unroll
+      for (int dlt = 1; dlt < kWave; dlt <<= 1) {
+        const double ws = __shfl_up(w, dlt);
+        const double w2s = __shfl_up(w2, dlt);
+        const int bs = __shfl_up(bin, dlt);
+        if (kComp) {
+          const double cs = __shfl_up(c, dlt);
+          const double c2s = __shfl_up(c2, dlt);
+          if (lane >= dlt && bs == bin) {
+            double sum, e;
+            two_sum(ws, w, sum, e);
+            w = sum;
+            c = c + cs + e;
+            two_sum(w2s, w2, sum, e);
+            w2 = sum;
+            c2 = c2 + c2s + e;
+          }
+        } else {
+          if (lane >= dlt && bs == bin) {
+            w = ws + w;
+            w2 = w2s + w2;
+          }
+        }
+      }
+      const int prev = __shfl_up(bin, 1);
+      const int next = __shfl_down(bin, 1);
+      const bool valid = lane < n_ev;
+      const bool head = valid && (lane == 0 || prev != bin);
+      const bool tail = valid && (lane == n_ev - 1 || next != bin);
+      const unsigned long long heads = __ballot(head);
+      const int seg = __popcll(heads & ((2ull << lane) - 1ull)) - 1;
+      if (tail) {
+        const size_t slot = (size_t)k * a.n_partials + first_partial + seg;
+        a.psum[slot] = w;
+        a.psum2[slot] = w2;
+        if (kComp) {
+          a.pcomp[slot] = c;
+          a.pcomp2[slot] = c2;
+        }
+      }
+    }
+  }
+}
+
+// Evaluator waves: every dial of the tile for point k into the step's
+// response buffer, one dial per lane.
+template <typename R, bool kInputsLds, int NB, bool kGrid>
+__device__ __forceinline__ void fill_evaluate(const FillArgs& a, unsigned char* ring,
+                                              size_t buf_bytes, size_t hdr_off,
+                                              double* resp_base, const double* s_x,
+                                              const ClassRec* s_cls, int eval_threads,
+                                              int64_t t0, int64_t G) {
+  int step = 0;
+  int64_t i = 0;
+  for (int64_t t = t0; t < a.n_tiles; t += G, ++i) {
+    const unsigned char* cur = ring + (size_t)(i % NB) * buf_bytes;
+    const TileHdr h = *reinterpret_cast<const TileHdr*>(cur + hdr_off);
+    const int n_dials = __builtin_amdgcn_readfirstlane(h.n_dials);
+    // 1-D dials; the 2-D dials (sorted last) follow in their own loop, so
+    // that their register-hungry grid code stays out of this one
+    const int grid_begin = kGrid ? __builtin_amdgcn_readfirstlane(h.grid_begin) : n_dials;
+    const R* s_rec = reinterpret_cast<const R*>(cur);
+    const uint64_t* s_desc = reinterpret_cast<const uint64_t*>(cur + a.rec_bytes);
+    for (int k = 0; k < a.k_points; ++k, ++step) {
+      double* resp = resp_base + (size_t)(step & 1) * a.max_dials;
+      const InputRec* inp = a.xin + (size_t)k * a.n_inputs;
+      const double* xk = kInputsLds ? s_x + (size_t)k * a.n_inputs : nullptr;
+      const double* tk = a.tab + (size_t)k * a.table_len;
+      for (int d0 = threadIdx.x; d0 < grid_begin; d0 += eval_threads) {
+        const uint64_t d = s_desc[d0];
+        const int in = (int)(d & 0xffff);
+        const int type = (int)((d >> 16) & 0xf);
+        const int n = (int)((d >> 20) & 0xff);
+        const int dest = (int)((d >> 32) & 0xffff);
+        const int off = (int)(d >> 48);
+        const int cl = (int)((d >> 28) & 0xf);
+        double x, lo, hi;
+        int flags;
+        if (kInputsLds && cl != kEscapeClass) {
+          x = xk[in];
+          lo = s_cls[cl].lo;
+          hi = s_cls[cl].hi;
+          flags = s_cls[cl].flags;
+        } else {
+          const InputRec ir = inp[in];
+          x = ir.x;
+          lo = ir.lo;
+          hi = ir.hi;
+          flags = ir.flags;
+        }
+        double r = eval_dial<R>(type, n, s_rec + off, x, flags, tk);
+        // DialResponseSupervisor::process (DialResponseSupervisor.cpp:10-18)
+        if (r < lo) r = lo;
+        else if (r > hi) r = hi;
+        resp[dest] = r;
+      }
+      for (int d0 = grid_begin + (int)threadIdx.x; kGrid && d0 < n_dials; d0 += eval_threads) {
+        const uint64_t d = s_desc[d0];
+        const int type = (int)((d >> 16) & 0xf);
+        const int dest = (int)((d >> 32) & 0xffff);
+        const int off = (int)(d >> 48);
+        const InputRec ir = inp[(int)(d & 0xffff)];
+        // record word 0: the second input (y axis); the payload follows
+        const InputRec ir2 = inp[(int)s_rec[off]];
+        double r = eval_dial2d<R>(type, s_rec + off + 1, ir.x, ir2.x, ir.flags);
+        if (r < ir.lo) r = ir.lo;
+        else if (r > ir.hi) r = ir.hi;
+        resp[dest] = r;
+      }
+      lds_barrier();
+    }
+  }
+}
+
 template <typename R, bool kComp, bool kInputsLds, int NB, bool kGrid>
 __global__ void __launch_bounds__(1024, 6) fill_kernel(FillArgs a) {
   constexpr int P = NB - 1;  // producer waves
@@ -241,8 +443,6 @@ __global__ void __launch_bounds__(1024, 6) fill_kernel(FillArgs a) {
   const size_t buf_bytes = hdr_off + kHeaderBytes;
   unsigned char* const ring = lds + in_pad;  // NB tile buffers
   double* const resp_base = reinterpret_cast<double*>(ring + NB * buf_bytes);
-  const unsigned char* rec = reinterpret_cast<const unsigned char*>(a.rec);
-  const unsigned char* desc = reinterpret_cast<const unsigned char*>(a.desc);
   const int64_t G = gridDim.x;
   const int64_t t0 = blockIdx.x;
 
@@ -252,184 +452,29 @@ __global__ void __launch_bounds__(1024, 6) fill_kernel(FillArgs a) {
   }
   if (threadIdx.x < kInputClasses) s_cls[threadIdx.x] = a.cls[threadIdx.x];
   // producer p loads tile ordinal p into buffer p; tile 0 is waited for now,
-  // tile p > 0 by producer p at the end of tile p - 1.  Each producer keeps
-  // the header of its next tile in registers, fetched one DMA period ahead,
-  // so that no DMA waits on a header load.
+  // tile p > 0 by producer p at the end of tile p - 1
   TileHdr hq{};
   if (wave >= prod0) {
     const int p = wave - prod0;
     const int64_t tp = t0 + p * G;
     if (tp < a.n_tiles) {
-      dma_tile<R>(a.tiles[tp], rec, desc, a.ev, ring + (size_t)p * buf_bytes,
-                  a.rec_bytes, a.max_dials, lane);
+      dma_tile<R>(uniform_header(a.tiles[tp]), reinterpret_cast<const unsigned char*>(a.rec),
+                  reinterpret_cast<const unsigned char*>(a.desc), a.ev,
+                  ring + (size_t)p * buf_bytes, a.rec_bytes, a.max_dials, lane);
       dma_header(a.tiles + tp, ring + (size_t)p * buf_bytes, hdr_off, lane);
     }
-    if (tp + P * G < a.n_tiles) hq = a.tiles[tp + P * G];
+    if (tp + P * G < a.n_tiles) hq = uniform_header(a.tiles[tp + P * G]);
     if (p == 0) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   }
   lds_barrier();
 
-  int step = 0;
-  int64_t i = 0;  // ordinal of the tile within this block
-  for (int64_t t = t0; t < a.n_tiles; t += G, ++i) {
-    unsigned char* cur = ring + (size_t)(i % NB) * buf_bytes;
-    TileHdr h;
-    if (wave <= ev_wave) h = *reinterpret_cast<const TileHdr*>(cur + hdr_off);
-    for (int k = 0; k < a.k_points; ++k, ++step) {
-      double* resp = resp_base + (size_t)(step & 1) * a.max_dials;
-      if (wave >= prod0) {
-        // at the first step of tile i, producer i mod P loads tile i + P into
-        // the buffer of tile i - 1; at its last step, producer (i + 1) mod P
-        // waits for tile i + 1
-        const int me = wave - prod0;
-        if (k == 0 && me == (int)(i % P)) {
-          const int64_t tn = t + P * G;
-          if (tn < a.n_tiles) {
-            unsigned char* nbuf = ring + (size_t)((i + P) % NB) * buf_bytes;
-            dma_tile<R>(hq, rec, desc, a.ev, nbuf, a.rec_bytes, a.max_dials, lane);
-            dma_header(a.tiles + tn, nbuf, hdr_off, lane);
-            if (tn + P * G < a.n_tiles) hq = a.tiles[tn + P * G];
-          }
-        }
-        if (k == a.k_points - 1 && me == (int)((i + 1) % P))
-          asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
-        lds_barrier();
-        continue;
-      }
-      if (wave == ev_wave) {
-        // The event record is read before the barrier: after it, a producer
-        // may refill this buffer.
-        EventRec er{0.0, INT_MAX, 0u};
-        if (lane < h.n_events)
-          er = reinterpret_cast<const EventRec*>(cur + a.rec_bytes +
-                                                 (size_t)a.max_dials * 8)[lane];
-        lds_barrier();
-        double w = 0.0, w2 = 0.0, c = 0.0, c2 = 0.0;
-        int bin = INT_MAX;
-        if (lane < h.n_events) {
-          const int b0 = (int)(er.dials & 0xffff), cnt = (int)(er.dials >> 16);
-          // EventDialCache.cpp:159-170: the product in dial-list order; the
-          // loads are batched four at a time, the multiplies stay in order.
-          double temp = 1;
-          for (int j = 0; j < cnt; j += 4) {
-            const double r0 = resp[b0 + j];
-            const double r1 = (j + 1 < cnt) ? resp[b0 + j + 1] : 1.0;
-            const double r2 = (j + 2 < cnt) ? resp[b0 + j + 2] : 1.0;
-            const double r3 = (j + 3 < cnt) ? resp[b0 + j + 3] : 1.0;
-            temp *= r0;
-            if (j + 1 < cnt) temp *= r1;
-            if (j + 2 < cnt) temp *= r2;
-            if (j + 3 < cnt) temp *= r3;
-          }
-          if (temp > a.cap) temp = a.cap;
-          w = er.base;
-          w *= temp;
-          bin = er.bin;
-          if (k == 0 && a.weights_out)
-            a.weights_out[a.ev_result[(int64_t)h.first_event + lane]] = w;
-          if (isnan(w)) atomicOr(a.flags, 1);
-          w2 = w * w;
-        }
-        // Inclusive segmented scan over the wave; bins are sorted, so equal
-        // bins are one contiguous segment.
-#pragma unroll
-        for (int dlt = 1; dlt < kWave; dlt <<= 1) {
-          const double ws = __shfl_up(w, dlt);
-          const double w2s = __shfl_up(w2, dlt);
-          const int bs = __shfl_up(bin, dlt);
-          if (kComp) {
-            const double cs = __shfl_up(c, dlt);
-            const double c2s = __shfl_up(c2, dlt);
-            if (lane >= dlt && bs == bin) {
-              double sum, e;
-              two_sum(ws, w, sum, e);
-              w = sum;
-              c = c + cs + e;
-              two_sum(w2s, w2, sum, e);
-              w2 = sum;
-              c2 = c2 + c2s + e;
-            }
-          } else {
-            if (lane >= dlt && bs == bin) {
-              w = ws + w;
-              w2 = w2s + w2;
-            }
-          }
-        }
-        const int prev = __shfl_up(bin, 1);
-        const int next = __shfl_down(bin, 1);
-        const bool valid = lane < h.n_events;
-        const bool head = valid && (lane == 0 || prev != bin);
-        const bool tail = valid && (lane == h.n_events - 1 || next != bin);
-        const unsigned long long heads = __ballot(head);
-        const int seg = __popcll(heads & ((2ull << lane) - 1ull)) - 1;
-        if (tail) {
-          const size_t slot = (size_t)k * a.n_partials + h.first_partial + seg;
-          a.psum[slot] = w;
-          a.psum2[slot] = w2;
-          if (kComp) {
-            a.pcomp[slot] = c;
-            a.pcomp2[slot] = c2;
-          }
-        }
-        continue;
-      }
-      // ---- evaluators: every dial of the tile for point k ----
-      {
-        const R* s_rec = reinterpret_cast<const R*>(cur);
-        const uint64_t* s_desc = reinterpret_cast<const uint64_t*>(cur + a.rec_bytes);
-        const InputRec* inp = a.xin + (size_t)k * a.n_inputs;
-        const double* xk = kInputsLds ? s_x + (size_t)k * a.n_inputs : nullptr;
-        const double* tk = a.tab + (size_t)k * a.table_len;
-        // 1-D dials; the 2-D dials (sorted last) follow in their own loop, so
-        // that their register-hungry grid code stays out of this one
-        const int grid_begin = kGrid ? (int)h.grid_begin : (int)h.n_dials;
-        for (int d0 = threadIdx.x; d0 < grid_begin; d0 += eval_threads) {
-          const uint64_t d = s_desc[d0];
-          const int in = (int)(d & 0xffff);
-          const int type = (int)((d >> 16) & 0xf);
-          const int n = (int)((d >> 20) & 0xff);
-          const int dest = (int)((d >> 32) & 0xffff);
-          const int off = (int)(d >> 48);
-          const int cl = (int)((d >> 28) & 0xf);
-          double x, lo, hi;
-          int flags;
-          if (kInputsLds && cl != kEscapeClass) {
-            x = xk[in];
-            lo = s_cls[cl].lo;
-            hi = s_cls[cl].hi;
-            flags = s_cls[cl].flags;
-          } else {
-            const InputRec ir = inp[in];
-            x = ir.x;
-            lo = ir.lo;
-            hi = ir.hi;
-            flags = ir.flags;
-          }
-          double r = eval_dial<R>(type, n, s_rec + off, x, flags, tk);
-          // DialResponseSupervisor::process (DialResponseSupervisor.cpp:10-18)
-          if (r < lo) r = lo;
-          else if (r > hi) r = hi;
-          resp[dest] = r;
-        }
-        for (int d0 = grid_begin + (int)threadIdx.x; kGrid && d0 < h.n_dials;
-             d0 += eval_threads) {
-          const uint64_t d = s_desc[d0];
-          const int type = (int)((d >> 16) & 0xf);
-          const int dest = (int)((d >> 32) & 0xffff);
-          const int off = (int)(d >> 48);
-          const InputRec ir = inp[(int)(d & 0xffff)];
-          // record word 0: the second input (y axis); the payload follows
-          const InputRec ir2 = inp[(int)s_rec[off]];
-          double r = eval_dial2d<R>(type, s_rec + off + 1, ir.x, ir2.x, ir.flags);
-          if (r < ir.lo) r = ir.lo;
-          else if (r > ir.hi) r = ir.hi;
-          resp[dest] = r;
-        }
-      }
-      lds_barrier();
-    }
-  }
+  if (wave >= prod0)
+    fill_producer<R, NB>(a, ring, buf_bytes, hdr_off, wave - prod0, lane, t0, G, hq);
+  else if (wave == ev_wave)
+    fill_events<kComp, NB>(a, ring, buf_bytes, hdr_off, resp_base, lane, t0, G);
+  else
+    fill_evaluate<R, kInputsLds, NB, kGrid>(a, ring, buf_bytes, hdr_off, resp_base, s_x, s_cls,
+                                            eval_threads, t0, G);
 }
 
 // The fill kernel instance for a precision, input staging, ring depth and
