@@ -80,7 +80,7 @@ def candidates(n_inputs: int, precision: str = "fp64", k_points: int = 1,
     out = []
     for nb, tile, block, bpc in itertools.product((2, 3), (1024, 1536, 2048, 2560, 3072, 3584,
                                                          3840, 4096),
-                                                  (256, 384, 512, 768),
+                                                  (256, 320, 384, 448, 512, 768),
                                                   (1, 2, 3, 4, 5, 6)):
         lds = lds_per_block(tile, n_inputs, k_points, ring_buffers=nb, blocks_per_cu=bpc,
                             mean_bytes=mean_bytes, most=most)

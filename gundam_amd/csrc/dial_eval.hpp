@@ -313,6 +313,90 @@ struct TableEntry<float> {
   }
 };
 
+// Tabulated and Kriged dials in two halves, so that a dial's table loads are
+// in flight while its thread evaluates another dial.  table_issue starts the
+// loads of the first kTabBatch table values the dial reads (Tabulated: the
+// entries at slot and slot + 1; Kriged: its first kTabBatch entries);
+// table_finish forms the response from them in the reference's order and
+// with its types (Tabulated.h:27-28: the float fraction weights the lower
+// entry; Kriged.cpp:14-21: the float weights summed in entry order), loading
+// any further Kriged entries itself.  The result equals eval_dial's.
+constexpr int kTabBatch = 8;
+
+template <typename R>
+__device__ __forceinline__ void table_issue(int type, int n, const R* d,
+                                            const double* __restrict__ tab,
+                                            double (&tv)[kTabBatch]) {
+  uint32_t s[kTabBatch];
+  int m;
+  if (type == GCM_DIAL_TABULATED) {
+    float f;
+    TableEntry<R>::get(d, 0, &s[0], &f);
+    s[1] = s[0] + 1;
+#pragma unroll
+    for (int q = 2; q < kTabBatch; ++q) s[q] = s[0];
+    m = 2;
+  } else {
+    m = n < kTabBatch ? n : kTabBatch;
+#pragma unroll
+    for (int q = 0; q < kTabBatch; ++q) {
+      float w;
+      s[q] = 0;
+      if (q < m) TableEntry<R>::get(d, q, &s[q], &w);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < kTabBatch; ++q) tv[q] = (q < m) ? tab[s[q]] : 0.0;
+}
+
+template <typename R>
+__device__ __forceinline__ double table_finish(int type, int n, const R* d,
+                                               const double* __restrict__ tab,
+                                               const double (&tv)[kTabBatch]) {
+  if (type == GCM_DIAL_TABULATED) {
+    uint32_t slot;
+    float f;
+    TableEntry<R>::get(d, 0, &slot, &f);
+    if (sizeof(R) == 8) {
+      const double fd = (double)f;
+      return fd * tv[0] + (1.0 - fd) * tv[1];
+    }
+    return (double)(f * (float)tv[0] + (1.0f - f) * (float)tv[1]);
+  }
+  double v = 0.0;
+  float vf = 0.0f;
+#pragma unroll
+  for (int q = 0; q < kTabBatch; ++q) {
+    if (q < n) {
+      uint32_t slot;
+      float w;
+      TableEntry<R>::get(d, q, &slot, &w);
+      if (sizeof(R) == 8) v += (double)w * tv[q];
+      else vf += w * (float)tv[q];
+    }
+  }
+  for (int j0 = kTabBatch; j0 < n; j0 += kTabBatch) {
+    double tw[kTabBatch];
+    float wv[kTabBatch];
+#pragma unroll
+    for (int q = 0; q < kTabBatch; ++q) {
+      uint32_t slot = 0;
+      float w = 0.0f;
+      if (j0 + q < n) TableEntry<R>::get(d, j0 + q, &slot, &w);
+      wv[q] = w;
+      tw[q] = (j0 + q < n) ? tab[slot] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < kTabBatch; ++q) {
+      if (j0 + q < n) {
+        if (sizeof(R) == 8) v += (double)wv[q] * tw[q];
+        else vf += wv[q] * (float)tw[q];
+      }
+    }
+  }
+  return sizeof(R) == 8 ? v : (double)vf;
+}
+
 // Evaluate one dial.  x is the mirrored input value; flags are the input's
 // GCM_INPUT_* flags; the point's table value at slot s is tab[s] (the
 // point's own table vector), or with kInterleaved tab[s * k_stride] (the

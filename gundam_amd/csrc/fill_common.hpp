@@ -52,15 +52,20 @@ struct EventRec {
 };
 static_assert(sizeof(EventRec) == 16, "event record is 16 bytes");
 
+// A tile's dials are type-sorted: the 1-D dials other than Tabulated and
+// Kriged in [0, tab_begin), the table dials in [tab_begin, grid_begin), the
+// 2-D dials in [grid_begin, n_dials).
 struct TileHdr {
   uint32_t first_event;
   uint32_t first_dial;
   uint64_t rec_begin;  // in record words of the precision's type
-  uint32_t rec_words;
+  uint16_t rec_words;  // <= tile_words x 8 / sizeof(R) <= 16384
+  uint16_t tab_begin;  // first Tabulated / Kriged dial
   uint16_t n_events;
   uint16_t n_dials;
   uint32_t first_partial;
-  uint32_t grid_begin;  // first 2-D dial in the tile's type-sorted order
+  uint16_t grid_begin;  // first 2-D dial
+  uint16_t pad;
 };
 static_assert(sizeof(TileHdr) == 32, "tile header is 32 bytes");
 
@@ -170,11 +175,13 @@ __device__ __forceinline__ TileHdr uniform_header(const TileHdr& h) {
   const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)h.rec_begin);
   const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(h.rec_begin >> 32));
   u.rec_begin = ((uint64_t)hi << 32) | lo;
-  u.rec_words = __builtin_amdgcn_readfirstlane(h.rec_words);
+  u.rec_words = (uint16_t)__builtin_amdgcn_readfirstlane(h.rec_words);
+  u.tab_begin = (uint16_t)__builtin_amdgcn_readfirstlane(h.tab_begin);
   u.n_events = (uint16_t)__builtin_amdgcn_readfirstlane(h.n_events);
   u.n_dials = (uint16_t)__builtin_amdgcn_readfirstlane(h.n_dials);
   u.first_partial = __builtin_amdgcn_readfirstlane(h.first_partial);
-  u.grid_begin = __builtin_amdgcn_readfirstlane(h.grid_begin);
+  u.grid_begin = (uint16_t)__builtin_amdgcn_readfirstlane(h.grid_begin);
+  u.pad = 0;
   return u;
 }
 

@@ -351,22 +351,32 @@ __device__ __forceinline__ void fill_events(const FillArgs& a, unsigned char* ri
 }
 
 // Evaluator waves: every dial of the tile for point k into the step's
-// response buffer, one dial per lane.
+// response buffer.  Per round of eval_threads dials, the other 1-D dials go
+// to the threads in order and the table dials (Tabulated, Kriged) to the
+// threads in reverse order, so that while a tile's dials fit in one round
+// the table dials, whose loads go to global memory, sit in the last waves by
+// themselves (fewest waves and load instructions waiting on global memory;
+// measured faster than spreading them over the waves).  A thread holding
+// both kinds in a round starts the table loads, evaluates its other dial,
+// then finishes the table dial.  The 2-D dials follow in their own loop, so
+// that their register-hungry grid code stays out of this one.
 template <typename R, bool kInputsLds, int NB, bool kGrid>
 __device__ __forceinline__ void fill_evaluate(const FillArgs& a, unsigned char* ring,
                                               size_t buf_bytes, size_t hdr_off,
                                               double* resp_base, const double* s_x,
                                               const ClassRec* s_cls, int eval_threads,
                                               int64_t t0, int64_t G) {
+  const int rev = eval_threads - 1 - (int)threadIdx.x;
   int step = 0;
   int64_t i = 0;
   for (int64_t t = t0; t < a.n_tiles; t += G, ++i) {
     const unsigned char* cur = ring + (size_t)(i % NB) * buf_bytes;
     const TileHdr h = *reinterpret_cast<const TileHdr*>(cur + hdr_off);
     const int n_dials = __builtin_amdgcn_readfirstlane(h.n_dials);
-    // 1-D dials; the 2-D dials (sorted last) follow in their own loop, so
-    // that their register-hungry grid code stays out of this one
+    const int tab_begin = __builtin_amdgcn_readfirstlane(h.tab_begin);
     const int grid_begin = kGrid ? __builtin_amdgcn_readfirstlane(h.grid_begin) : n_dials;
+    const int n_tab = grid_begin - tab_begin;
+    const int span = tab_begin > n_tab ? tab_begin : n_tab;
     const R* s_rec = reinterpret_cast<const R*>(cur);
     const uint64_t* s_desc = reinterpret_cast<const uint64_t*>(cur + a.rec_bytes);
     for (int k = 0; k < a.k_points; ++k, ++step) {
@@ -374,33 +384,65 @@ __device__ __forceinline__ void fill_evaluate(const FillArgs& a, unsigned char* 
       const InputRec* inp = a.xin + (size_t)k * a.n_inputs;
       const double* xk = kInputsLds ? s_x + (size_t)k * a.n_inputs : nullptr;
       const double* tk = a.tab + (size_t)k * a.table_len;
-      for (int d0 = threadIdx.x; d0 < grid_begin; d0 += eval_threads) {
-        const uint64_t d = s_desc[d0];
-        const int in = (int)(d & 0xffff);
-        const int type = (int)((d >> 16) & 0xf);
-        const int n = (int)((d >> 20) & 0xff);
-        const int dest = (int)((d >> 32) & 0xffff);
-        const int off = (int)(d >> 48);
-        const int cl = (int)((d >> 28) & 0xf);
-        double x, lo, hi;
-        int flags;
-        if (kInputsLds && cl != kEscapeClass) {
-          x = xk[in];
-          lo = s_cls[cl].lo;
-          hi = s_cls[cl].hi;
-          flags = s_cls[cl].flags;
-        } else {
-          const InputRec ir = inp[in];
-          x = ir.x;
-          lo = ir.lo;
-          hi = ir.hi;
-          flags = ir.flags;
+      for (int r0 = 0; r0 < span; r0 += eval_threads) {
+        // the table dial: start its loads and its supervisor bounds
+        const int qt = r0 + rev;
+        const bool has_t = qt < n_tab;
+        uint64_t dt = 0;
+        double tv[kTabBatch];
+        double tlo = 0.0, thi = 0.0;
+        if (has_t) {
+          dt = s_desc[tab_begin + qt];
+          const int cl = (int)((dt >> 28) & 0xf);
+          table_issue<R>((int)((dt >> 16) & 0xf), (int)((dt >> 20) & 0xff),
+                         s_rec + (int)(dt >> 48), tk, tv);
+          if (kInputsLds && cl != kEscapeClass) {
+            tlo = s_cls[cl].lo;
+            thi = s_cls[cl].hi;
+          } else {
+            const InputRec& ir = inp[(int)(dt & 0xffff)];
+            tlo = ir.lo;
+            thi = ir.hi;
+          }
         }
-        double r = eval_dial<R>(type, n, s_rec + off, x, flags, tk);
-        // DialResponseSupervisor::process (DialResponseSupervisor.cpp:10-18)
-        if (r < lo) r = lo;
-        else if (r > hi) r = hi;
-        resp[dest] = r;
+        // another 1-D dial meanwhile
+        const int qn = r0 + (int)threadIdx.x;
+        if (qn < tab_begin) {
+          const uint64_t d = s_desc[qn];
+          const int in = (int)(d & 0xffff);
+          const int type = (int)((d >> 16) & 0xf);
+          const int n = (int)((d >> 20) & 0xff);
+          const int dest = (int)((d >> 32) & 0xffff);
+          const int off = (int)(d >> 48);
+          const int cl = (int)((d >> 28) & 0xf);
+          double x, lo, hi;
+          int flags;
+          if (kInputsLds && cl != kEscapeClass) {
+            x = xk[in];
+            lo = s_cls[cl].lo;
+            hi = s_cls[cl].hi;
+            flags = s_cls[cl].flags;
+          } else {
+            const InputRec ir = inp[in];
+            x = ir.x;
+            lo = ir.lo;
+            hi = ir.hi;
+            flags = ir.flags;
+          }
+          double r = eval_dial<R>(type, n, s_rec + off, x, flags, tk);
+          // DialResponseSupervisor::process (DialResponseSupervisor.cpp:10-18)
+          if (r < lo) r = lo;
+          else if (r > hi) r = hi;
+          resp[dest] = r;
+        }
+        // the table dial's response
+        if (has_t) {
+          double r = table_finish<R>((int)((dt >> 16) & 0xf), (int)((dt >> 20) & 0xff),
+                                     s_rec + (int)(dt >> 48), tk, tv);
+          if (r < tlo) r = tlo;
+          else if (r > thi) r = thi;
+          resp[(int)((dt >> 32) & 0xffff)] = r;
+        }
       }
       for (int d0 = grid_begin + (int)threadIdx.x; kGrid && d0 < n_dials; d0 += eval_threads) {
         const uint64_t d = s_desc[d0];
@@ -1009,7 +1051,7 @@ int build_layout(gcm_context* c, const gcm_tables* t) {
     int64_t rec_pos = 0, dial_pos = 0;
     auto close = [&](int64_t ev_end) {
       (void)ev_end;
-      cur.rec_words = (uint32_t)cur_words;
+      cur.rec_words = (uint16_t)cur_words;
       cur.n_dials = (uint16_t)cur_dials;
       cur.n_events = (uint16_t)cur_events;
       tiles.push_back(cur);
@@ -1144,27 +1186,36 @@ int build_layout(gcm_context* c, const gcm_tables* t) {
             slot.push_back(local++);
           }
         }
-        // type-major order (table dials also by table), stable otherwise
+        // type-major order, stable otherwise: the 1-D types other than
+        // Tabulated and Kriged, then those two (also by table), then the
+        // 2-D types
         std::vector<int> idx(dial_ids.size());
         for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
         auto sort_key = [&](int q) -> int64_t {
           const int64_t d = dial_ids[q];
           const int ty = t->dial_type[d];
           int64_t tb = 0;
-          if (ty == GCM_DIAL_TABULATED || ty == GCM_DIAL_KRIGED)
+          int rank = ty;
+          if (ty == GCM_DIAL_TABULATED || ty == GCM_DIAL_KRIGED) {
             tb = (int64_t)t->dial_data[t->data_begin[d]];
-          return (int64_t)ty << 40 | tb;
+            rank = 16 + ty;
+          } else if (ty == GCM_DIAL_BILINEAR || ty == GCM_DIAL_BICUBIC) {
+            rank = 32 + ty;
+          }
+          return (int64_t)rank << 40 | tb;
         };
         std::stable_sort(idx.begin(), idx.end(),
                          [&](int x, int y) { return sort_key(x) < sort_key(y); });
         int64_t off = 0;
-        uint32_t grid0 = (uint32_t)idx.size();
+        uint32_t grid0 = (uint32_t)idx.size(), tab0 = (uint32_t)idx.size();
         for (size_t q = 0; q < idx.size(); ++q) {
           const int64_t d = dial_ids[idx[q]];
           const int type = t->dial_type[d];
           const int n = knots[d];
           if ((type == GCM_DIAL_BILINEAR || type == GCM_DIAL_BICUBIC) && grid0 > q)
             grid0 = (uint32_t)q;
+          if ((type == GCM_DIAL_TABULATED || type == GCM_DIAL_KRIGED) && tab0 > q)
+            tab0 = (uint32_t)q;
           const double* p = t->dial_data + t->data_begin[d];
           R* out = rec.data() + h.rec_begin + off;
           switch (type) {
@@ -1209,7 +1260,8 @@ int build_layout(gcm_context* c, const gcm_tables* t) {
                                    ((uint64_t)slot[idx[q]] << 32) | ((uint64_t)off << 48);
           off += record_words(type, n, fp32, t->data_begin[d + 1] - t->data_begin[d]);
         }
-        h.grid_begin = grid0;  // the 2-D types sort last
+        h.grid_begin = (uint16_t)grid0;  // the 2-D types sort last
+        h.tab_begin = (uint16_t)std::min(tab0, grid0);
       }
     });
   }
