@@ -52,7 +52,7 @@ def lds_per_block(tile_words: int, n_inputs: int, k_points: int = 1,
     blocks_per_cu blocks share the CU; the 512 B of input-class records are
     always staged)."""
     max_dials = max_dials_for(tile_words, mean_bytes, most)
-    ring = ring_buffers * (tile_words * 8 + max_dials * 8 + 64 * 16 + 32) + 2 * max_dials * 8
+    ring = ring_buffers * (tile_words * 8 + max_dials * 8 + 64 * 16 + 32) + 2 * max_dials * 8 + 16
     in_bytes = k_points * n_inputs * 8
     classes = 16 * 32
     staged = (in_bytes + classes + 1023) // 1024 * 1024

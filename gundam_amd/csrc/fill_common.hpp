@@ -42,6 +42,7 @@ constexpr size_t kInputsLdsMax = 16384;  // stage input records in LDS up to thi
 constexpr size_t kLdsPerCU = 160 * 1024;   // gfx950 LDS per compute unit
 constexpr size_t kHeaderBytes = 32;      // tile header copy in each LDS buffer
 constexpr int kMaxDynamicLds = 160 * 1024;  // gfx950 LDS per CU
+constexpr size_t kReadyBytes = 16;  // the fill kernel's per-buffer ready flags
 
 // One binned event, in bin-sorted order: base weight, global bin, and its
 // dials as (first event-major slot in the tile, count).
@@ -52,20 +53,20 @@ struct EventRec {
 };
 static_assert(sizeof(EventRec) == 16, "event record is 16 bytes");
 
-// A tile's dials are type-sorted: the 1-D dials other than Tabulated and
-// Kriged in [0, tab_begin), the table dials in [tab_begin, grid_begin), the
-// 2-D dials in [grid_begin, n_dials).
+// A tile's dials are type-sorted: the table dials (Tabulated, Kriged) in
+// [0, n_tab), whose records are the first tab_words record words, the other
+// 1-D dials in [n_tab, grid_begin), the 2-D dials in [grid_begin, n_dials).
 struct TileHdr {
   uint32_t first_event;
   uint32_t first_dial;
   uint64_t rec_begin;  // in record words of the precision's type
   uint16_t rec_words;  // <= tile_words x 8 / sizeof(R) <= 16384
-  uint16_t tab_begin;  // first Tabulated / Kriged dial
+  uint16_t n_tab;      // table dials
   uint16_t n_events;
   uint16_t n_dials;
   uint32_t first_partial;
   uint16_t grid_begin;  // first 2-D dial
-  uint16_t pad;
+  uint16_t tab_words;   // record words of the table dials
 };
 static_assert(sizeof(TileHdr) == 32, "tile header is 32 bytes");
 
@@ -130,6 +131,7 @@ struct FillArgs {
   int rec_bytes;   // LDS record capacity per buffer (multiple of 16 bytes)
   int max_dials;   // LDS descriptor/response capacity (multiple of 128)
   int inputs_lds;  // input records staged in LDS (K * n_inputs * 32 bytes)
+  int tab_waves;   // evaluator waves that take only the table dials (0: none)
 };
 
 // Cache policy of the tile stream: non-temporal, so the streamed records do
@@ -176,14 +178,38 @@ __device__ __forceinline__ TileHdr uniform_header(const TileHdr& h) {
   const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(h.rec_begin >> 32));
   u.rec_begin = ((uint64_t)hi << 32) | lo;
   u.rec_words = (uint16_t)__builtin_amdgcn_readfirstlane(h.rec_words);
-  u.tab_begin = (uint16_t)__builtin_amdgcn_readfirstlane(h.tab_begin);
+  u.n_tab = (uint16_t)__builtin_amdgcn_readfirstlane(h.n_tab);
   u.n_events = (uint16_t)__builtin_amdgcn_readfirstlane(h.n_events);
   u.n_dials = (uint16_t)__builtin_amdgcn_readfirstlane(h.n_dials);
   u.first_partial = __builtin_amdgcn_readfirstlane(h.first_partial);
   u.grid_begin = (uint16_t)__builtin_amdgcn_readfirstlane(h.grid_begin);
-  u.pad = 0;
+  u.tab_words = (uint16_t)__builtin_amdgcn_readfirstlane(h.tab_words);
   return u;
 }
+
+// s_waitcnt vmcnt(m) for the largest m <= min(n, 47): waits until at most n
+// of this wave's vector-memory loads are outstanding (or fewer).
+__device__ __forceinline__ void wait_vm_at_most(int n) {
+  switch (n < 0 ? 0 : n > 47 ? 47 : n) {
+#define GCM_VM_CASE(m) \
+    case m: asm volatile("s_waitcnt vmcnt(" #m ")" ::: "memory"); break;
+    GCM_VM_CASE(0) GCM_VM_CASE(1) GCM_VM_CASE(2) GCM_VM_CASE(3) GCM_VM_CASE(4)
+    GCM_VM_CASE(5) GCM_VM_CASE(6) GCM_VM_CASE(7) GCM_VM_CASE(8) GCM_VM_CASE(9)
+    GCM_VM_CASE(10) GCM_VM_CASE(11) GCM_VM_CASE(12) GCM_VM_CASE(13) GCM_VM_CASE(14)
+    GCM_VM_CASE(15) GCM_VM_CASE(16) GCM_VM_CASE(17) GCM_VM_CASE(18) GCM_VM_CASE(19)
+    GCM_VM_CASE(20) GCM_VM_CASE(21) GCM_VM_CASE(22) GCM_VM_CASE(23) GCM_VM_CASE(24)
+    GCM_VM_CASE(25) GCM_VM_CASE(26) GCM_VM_CASE(27) GCM_VM_CASE(28) GCM_VM_CASE(29)
+    GCM_VM_CASE(30) GCM_VM_CASE(31) GCM_VM_CASE(32) GCM_VM_CASE(33) GCM_VM_CASE(34)
+    GCM_VM_CASE(35) GCM_VM_CASE(36) GCM_VM_CASE(37) GCM_VM_CASE(38) GCM_VM_CASE(39)
+    GCM_VM_CASE(40) GCM_VM_CASE(41) GCM_VM_CASE(42) GCM_VM_CASE(43) GCM_VM_CASE(44)
+    GCM_VM_CASE(45) GCM_VM_CASE(46) GCM_VM_CASE(47)
+#undef GCM_VM_CASE
+    default: break;
+  }
+}
+
+// DMA instructions dma_to_lds issues for `bytes`.
+__device__ __forceinline__ int dma_instrs(int bytes) { return (bytes + 1023) / 1024; }
 
 template <typename R>
 __device__ __forceinline__ void dma_tile(const TileHdr& h, const unsigned char* rec,
@@ -196,6 +222,35 @@ __device__ __forceinline__ void dma_tile(const TileHdr& h, const unsigned char* 
              (h.n_dials + 1) / 2 * 16, lane);
   dma_to_lds(reinterpret_cast<const unsigned char*>(ev + h.first_event),
              buf + rec_bytes + (size_t)max_dials * 8, h.n_events * 16, lane);
+}
+
+// The tile in two parts: first the header and the table dials' records and
+// descriptors (what the table waves need a step early), then the rest.  The
+// parts are cut on 16-byte boundaries; the up to 8 bytes on either side of a
+// cut are copied by both parts (the same bytes).  Returns the number of DMA
+// instructions of the second part.
+template <typename R>
+__device__ __forceinline__ int dma_tile_split(const TileHdr& h, const TileHdr* hdr_src,
+                                              const unsigned char* rec,
+                                              const unsigned char* desc, const EventRec* ev,
+                                              unsigned char* buf, int rec_bytes, int max_dials,
+                                              size_t hdr_off, int lane) {
+  const unsigned char* r0 = rec + h.rec_begin * sizeof(R);
+  const unsigned char* d0 = desc + (size_t)h.first_dial * 8;
+  const int tab_bytes = (int)(h.tab_words * sizeof(R));
+  const int rec_all = (int)((h.rec_words * sizeof(R) + 15) / 16 * 16);
+  const int desc_all = (h.n_dials + 1) / 2 * 16;
+  const int tr = (tab_bytes + 15) / 16 * 16, tr0 = tab_bytes / 16 * 16;
+  const int td = (h.n_tab * 8 + 15) / 16 * 16, td0 = h.n_tab * 8 / 16 * 16;
+  dma_to_lds(reinterpret_cast<const unsigned char*>(hdr_src), buf + hdr_off,
+             (int)sizeof(TileHdr), lane);
+  dma_to_lds(r0, buf, tr, lane);
+  dma_to_lds(d0, buf + rec_bytes, td, lane);
+  dma_to_lds(r0 + tr0, buf + tr0, rec_all - tr0, lane);
+  dma_to_lds(d0 + td0, buf + rec_bytes + td0, desc_all - td0, lane);
+  dma_to_lds(reinterpret_cast<const unsigned char*>(ev + h.first_event),
+             buf + rec_bytes + (size_t)max_dials * 8, h.n_events * 16, lane);
+  return dma_instrs(rec_all - tr0) + dma_instrs(desc_all - td0) + dma_instrs(h.n_events * 16);
 }
 
 // The tile header travels with the tile (evaluators and the event wave read

@@ -227,10 +227,14 @@ __global__ void penalty_sets_kernel(const double* __restrict__ theta, int n_par,
 // into the buffer of tile i - 1; at the last step of tile i, producer
 // (i + 1) mod P waits for tile i + 1.  The next tile's header is held in
 // registers one DMA period ahead, so that no DMA waits on a header load.
+// With table waves the tile is copied in two parts and, as soon as the first
+// (header, table dials' records and descriptors) has landed, the producer
+// writes the tile's ordinal into ready[buffer] for the table waves.
 template <typename R, int NB>
 __device__ __forceinline__ void fill_producer(const FillArgs& a, unsigned char* ring,
-                                              size_t buf_bytes, size_t hdr_off, int me,
-                                              int lane, int64_t t0, int64_t G, TileHdr hq) {
+                                              size_t buf_bytes, size_t hdr_off, int* ready,
+                                              int me, int lane, int64_t t0, int64_t G,
+                                              TileHdr hq) {
   constexpr int P = NB - 1;
   const unsigned char* rec = reinterpret_cast<const unsigned char*>(a.rec);
   const unsigned char* desc = reinterpret_cast<const unsigned char*>(a.desc);
@@ -241,8 +245,15 @@ __device__ __forceinline__ void fill_producer(const FillArgs& a, unsigned char* 
         const int64_t tn = t + P * G;
         if (tn < a.n_tiles) {
           unsigned char* nbuf = ring + (size_t)((i + P) % NB) * buf_bytes;
-          dma_tile<R>(hq, rec, desc, a.ev, nbuf, a.rec_bytes, a.max_dials, lane);
-          dma_header(a.tiles + tn, nbuf, hdr_off, lane);
+          if (a.tab_waves > 0) {
+            const int n2 = dma_tile_split<R>(hq, a.tiles + tn, rec, desc, a.ev, nbuf,
+                                             a.rec_bytes, a.max_dials, hdr_off, lane);
+            wait_vm_at_most(n2);
+            if (lane == 0) ready[(i + P) % NB] = (int)(i + P);
+          } else {
+            dma_tile<R>(hq, rec, desc, a.ev, nbuf, a.rec_bytes, a.max_dials, lane);
+            dma_header(a.tiles + tn, nbuf, hdr_off, lane);
+          }
           if (tn + P * G < a.n_tiles) hq = uniform_header(a.tiles[tn + P * G]);
         }
       }
@@ -253,6 +264,109 @@ __device__ __forceinline__ void fill_producer(const FillArgs& a, unsigned char* 
   }
   // no LDS-DMA load of this block may land after its LDS is handed on
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+}
+
+// Table waves: the table dials (Tabulated, Kriged) of each step, with their
+// global-memory loads issued one step ahead.  In step s a table wave finishes
+// the dials whose table values it loaded in step s - 1, then reads the
+// records of step s + 1's tile (for a new tile, once the producer has flagged
+// its first part as landed) and starts their loads, which complete across
+// the barrier.  Lane l of table wave w takes table dial 64 w + l; dials past
+// the waves' lanes are done within the step.
+template <typename R, bool kInputsLds, int NB>
+__device__ __forceinline__ void fill_tables(const FillArgs& a, unsigned char* ring,
+                                            size_t buf_bytes, size_t hdr_off,
+                                            double* resp_base, const int* ready,
+                                            const ClassRec* s_cls, int tw, int lane,
+                                            int64_t t0, int64_t G) {
+  const int q0 = tw * kWave + lane;
+  const int stride = a.tab_waves * kWave;
+  double tv[kTabBatch];
+  uint64_t dt = 0;
+  double tlo = 0.0, thi = 0.0;
+  bool pend = false;
+  // start the loads of this lane's round-0 table dial of (tile ordinal j, point k)
+  auto issue = [&](int64_t j, int k) {
+    const unsigned char* buf = ring + (size_t)(j % NB) * buf_bytes;
+    const TileHdr* hp = reinterpret_cast<const TileHdr*>(buf + hdr_off);
+    const int n_tab = __builtin_amdgcn_readfirstlane(hp->n_tab);
+    pend = q0 < n_tab;
+    if (pend) {
+      dt = reinterpret_cast<const uint64_t*>(buf + a.rec_bytes)[q0];
+      table_issue<R>((int)((dt >> 16) & 0xf), (int)((dt >> 20) & 0xff),
+                     reinterpret_cast<const R*>(buf) + (int)(dt >> 48),
+                     a.tab + (size_t)k * a.table_len, tv);
+      const int cl = (int)((dt >> 28) & 0xf);
+      if (kInputsLds && cl != kEscapeClass) {
+        tlo = s_cls[cl].lo;
+        thi = s_cls[cl].hi;
+      } else {
+        const InputRec& ir = a.xin[(size_t)k * a.n_inputs + (int)(dt & 0xffff)];
+        tlo = ir.lo;
+        thi = ir.hi;
+      }
+    }
+  };
+  auto await_tile = [&](int64_t j) {
+    const volatile int* rd = ready;
+    while (rd[j % NB] != (int)j) __builtin_amdgcn_s_sleep(1);
+    asm volatile("" ::: "memory");
+  };
+  if (t0 < a.n_tiles) {
+    await_tile(0);
+    issue(0, 0);
+  }
+  int step = 0;
+  int64_t i = 0;
+  for (int64_t t = t0; t < a.n_tiles; t += G, ++i) {
+    const unsigned char* cur = ring + (size_t)(i % NB) * buf_bytes;
+    const TileHdr h = *reinterpret_cast<const TileHdr*>(cur + hdr_off);
+    const int n_tab = __builtin_amdgcn_readfirstlane(h.n_tab);
+    const R* s_rec = reinterpret_cast<const R*>(cur);
+    const uint64_t* s_desc = reinterpret_cast<const uint64_t*>(cur + a.rec_bytes);
+    for (int k = 0; k < a.k_points; ++k, ++step) {
+      double* resp = resp_base + (size_t)(step & 1) * a.max_dials;
+      const double* tk = a.tab + (size_t)k * a.table_len;
+      if (pend) {
+        double r = table_finish<R>((int)((dt >> 16) & 0xf), (int)((dt >> 20) & 0xff),
+                                   s_rec + (int)(dt >> 48), tk, tv);
+        // DialResponseSupervisor::process (DialResponseSupervisor.cpp:10-18)
+        if (r < tlo) r = tlo;
+        else if (r > thi) r = thi;
+        resp[(int)((dt >> 32) & 0xffff)] = r;
+      }
+      for (int q = q0 + stride; q < n_tab; q += stride) {
+        const uint64_t d = s_desc[q];
+        const int type = (int)((d >> 16) & 0xf), n = (int)((d >> 20) & 0xff);
+        double tw2[kTabBatch];
+        table_issue<R>(type, n, s_rec + (int)(d >> 48), tk, tw2);
+        double r = table_finish<R>(type, n, s_rec + (int)(d >> 48), tk, tw2);
+        const int cl = (int)((d >> 28) & 0xf);
+        double lo, hi;
+        if (kInputsLds && cl != kEscapeClass) {
+          lo = s_cls[cl].lo;
+          hi = s_cls[cl].hi;
+        } else {
+          const InputRec& ir = a.xin[(size_t)k * a.n_inputs + (int)(d & 0xffff)];
+          lo = ir.lo;
+          hi = ir.hi;
+        }
+        if (r < lo) r = lo;
+        else if (r > hi) r = hi;
+        resp[(int)((d >> 32) & 0xffff)] = r;
+      }
+      // the next step's table loads
+      if (k + 1 < a.k_points) {
+        issue(i, k + 1);
+      } else if (t + G < a.n_tiles) {
+        await_tile(i + 1);
+        issue(i + 1, 0);
+      } else {
+        pend = false;
+      }
+      lds_barrier();
+    }
+  }
 }
 
 // Event wave: the event weights and bin partials of step s, formed after the
@@ -365,7 +479,7 @@ __device__ __forceinline__ void fill_evaluate(const FillArgs& a, unsigned char* 
                                               size_t buf_bytes, size_t hdr_off,
                                               double* resp_base, const double* s_x,
                                               const ClassRec* s_cls, int eval_threads,
-                                              int64_t t0, int64_t G) {
+                                              bool with_tables, int64_t t0, int64_t G) {
   const int rev = eval_threads - 1 - (int)threadIdx.x;
   int step = 0;
   int64_t i = 0;
@@ -373,10 +487,11 @@ __device__ __forceinline__ void fill_evaluate(const FillArgs& a, unsigned char* 
     const unsigned char* cur = ring + (size_t)(i % NB) * buf_bytes;
     const TileHdr h = *reinterpret_cast<const TileHdr*>(cur + hdr_off);
     const int n_dials = __builtin_amdgcn_readfirstlane(h.n_dials);
-    const int tab_begin = __builtin_amdgcn_readfirstlane(h.tab_begin);
+    const int n_tab = with_tables ? __builtin_amdgcn_readfirstlane(h.n_tab) : 0;
+    const int tab_begin = __builtin_amdgcn_readfirstlane(h.n_tab);  // first other 1-D dial
     const int grid_begin = kGrid ? __builtin_amdgcn_readfirstlane(h.grid_begin) : n_dials;
-    const int n_tab = grid_begin - tab_begin;
-    const int span = tab_begin > n_tab ? tab_begin : n_tab;
+    const int n_other = grid_begin - tab_begin;
+    const int span = n_other > n_tab ? n_other : n_tab;
     const R* s_rec = reinterpret_cast<const R*>(cur);
     const uint64_t* s_desc = reinterpret_cast<const uint64_t*>(cur + a.rec_bytes);
     for (int k = 0; k < a.k_points; ++k, ++step) {
@@ -392,7 +507,7 @@ __device__ __forceinline__ void fill_evaluate(const FillArgs& a, unsigned char* 
         double tv[kTabBatch];
         double tlo = 0.0, thi = 0.0;
         if (has_t) {
-          dt = s_desc[tab_begin + qt];
+          dt = s_desc[qt];
           const int cl = (int)((dt >> 28) & 0xf);
           table_issue<R>((int)((dt >> 16) & 0xf), (int)((dt >> 20) & 0xff),
                          s_rec + (int)(dt >> 48), tk, tv);
@@ -407,8 +522,8 @@ __device__ __forceinline__ void fill_evaluate(const FillArgs& a, unsigned char* 
         }
         // another 1-D dial meanwhile
         const int qn = r0 + (int)threadIdx.x;
-        if (qn < tab_begin) {
-          const uint64_t d = s_desc[qn];
+        if (qn < n_other) {
+          const uint64_t d = s_desc[tab_begin + qn];
           const int in = (int)(d & 0xffff);
           const int type = (int)((d >> 16) & 0xf);
           const int n = (int)((d >> 20) & 0xff);
@@ -485,8 +600,10 @@ __global__ void __launch_bounds__(1024, 6) fill_kernel(FillArgs a) {
   const size_t buf_bytes = hdr_off + kHeaderBytes;
   unsigned char* const ring = lds + in_pad;  // NB tile buffers
   double* const resp_base = reinterpret_cast<double*>(ring + NB * buf_bytes);
+  int* const ready = reinterpret_cast<int*>(resp_base + 2 * (size_t)a.max_dials);
   const int64_t G = gridDim.x;
   const int64_t t0 = blockIdx.x;
+  const int tab0 = n_eval - a.tab_waves;  // first table wave
 
   if (kInputsLds) {
     const int n = a.k_points * a.n_inputs;
@@ -495,15 +612,33 @@ __global__ void __launch_bounds__(1024, 6) fill_kernel(FillArgs a) {
   if (threadIdx.x < kInputClasses) s_cls[threadIdx.x] = a.cls[threadIdx.x];
   // producer p loads tile ordinal p into buffer p; tile 0 is waited for now,
   // tile p > 0 by producer p at the end of tile p - 1
+  // With table waves each producer clears the ready flag of its first
+  // buffer (producer 0 also those of the buffers no producer fills now)
+  // before it copies a tile there, then flags the tile.
   TileHdr hq{};
   if (wave >= prod0) {
     const int p = wave - prod0;
     const int64_t tp = t0 + p * G;
+    if (a.tab_waves > 0 && lane == 0) {
+      ready[p] = -1;
+      if (p == 0)
+        for (int b = P; b < NB; ++b) ready[b] = -1;
+    }
     if (tp < a.n_tiles) {
-      dma_tile<R>(uniform_header(a.tiles[tp]), reinterpret_cast<const unsigned char*>(a.rec),
-                  reinterpret_cast<const unsigned char*>(a.desc), a.ev,
-                  ring + (size_t)p * buf_bytes, a.rec_bytes, a.max_dials, lane);
-      dma_header(a.tiles + tp, ring + (size_t)p * buf_bytes, hdr_off, lane);
+      const TileHdr h0 = uniform_header(a.tiles[tp]);
+      if (a.tab_waves > 0) {
+        const int n2 = dma_tile_split<R>(h0, a.tiles + tp, reinterpret_cast<const unsigned char*>(a.rec),
+                                         reinterpret_cast<const unsigned char*>(a.desc), a.ev,
+                                         ring + (size_t)p * buf_bytes, a.rec_bytes, a.max_dials,
+                                         hdr_off, lane);
+        wait_vm_at_most(n2);
+        if (lane == 0) ready[p] = p;
+      } else {
+        dma_tile<R>(h0, reinterpret_cast<const unsigned char*>(a.rec),
+                    reinterpret_cast<const unsigned char*>(a.desc), a.ev,
+                    ring + (size_t)p * buf_bytes, a.rec_bytes, a.max_dials, lane);
+        dma_header(a.tiles + tp, ring + (size_t)p * buf_bytes, hdr_off, lane);
+      }
     }
     if (tp + P * G < a.n_tiles) hq = uniform_header(a.tiles[tp + P * G]);
     if (p == 0) asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
@@ -511,12 +646,15 @@ __global__ void __launch_bounds__(1024, 6) fill_kernel(FillArgs a) {
   lds_barrier();
 
   if (wave >= prod0)
-    fill_producer<R, NB>(a, ring, buf_bytes, hdr_off, wave - prod0, lane, t0, G, hq);
+    fill_producer<R, NB>(a, ring, buf_bytes, hdr_off, ready, wave - prod0, lane, t0, G, hq);
   else if (wave == ev_wave)
     fill_events<kComp, NB>(a, ring, buf_bytes, hdr_off, resp_base, lane, t0, G);
+  else if (wave >= tab0)
+    fill_tables<R, kInputsLds, NB>(a, ring, buf_bytes, hdr_off, resp_base, ready, s_cls,
+                                   wave - tab0, lane, t0, G);
   else
     fill_evaluate<R, kInputsLds, NB, kGrid>(a, ring, buf_bytes, hdr_off, resp_base, s_x, s_cls,
-                                            eval_threads, t0, G);
+                                            tab0 * kWave, a.tab_waves == 0, t0, G);
 }
 
 // The fill kernel instance for a precision, input staging, ring depth and
@@ -735,6 +873,7 @@ struct gcm_context {
   int n_inputs = 0, n_par = 0, n_bins = 0, n_tables = 0;
   int64_t table_len = 0;
   int max_dials = 1024;
+  int tab_dials_hi = 0;  // table dials per tile, 99th percentile over tiles
   int fin_blocks = 1;
   double cap = __builtin_huge_val();
   LlhParams llh{GCM_LLH_POISSON, 0, 1e150, -1.0, 0, 0, nullptr, nullptr, nullptr};
@@ -1186,19 +1325,18 @@ int build_layout(gcm_context* c, const gcm_tables* t) {
             slot.push_back(local++);
           }
         }
-        // type-major order, stable otherwise: the 1-D types other than
-        // Tabulated and Kriged, then those two (also by table), then the
-        // 2-D types
+        // type-major order, stable otherwise: Tabulated and Kriged (also by
+        // table), then the other 1-D types, then the 2-D types
         std::vector<int> idx(dial_ids.size());
         for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
         auto sort_key = [&](int q) -> int64_t {
           const int64_t d = dial_ids[q];
           const int ty = t->dial_type[d];
           int64_t tb = 0;
-          int rank = ty;
+          int rank = 16 + ty;
           if (ty == GCM_DIAL_TABULATED || ty == GCM_DIAL_KRIGED) {
             tb = (int64_t)t->dial_data[t->data_begin[d]];
-            rank = 16 + ty;
+            rank = ty;
           } else if (ty == GCM_DIAL_BILINEAR || ty == GCM_DIAL_BICUBIC) {
             rank = 32 + ty;
           }
@@ -1206,16 +1344,15 @@ int build_layout(gcm_context* c, const gcm_tables* t) {
         };
         std::stable_sort(idx.begin(), idx.end(),
                          [&](int x, int y) { return sort_key(x) < sort_key(y); });
-        int64_t off = 0;
-        uint32_t grid0 = (uint32_t)idx.size(), tab0 = (uint32_t)idx.size();
+        int64_t off = 0, tab_words = 0;
+        uint32_t grid0 = (uint32_t)idx.size(), n_tab = 0;
         for (size_t q = 0; q < idx.size(); ++q) {
           const int64_t d = dial_ids[idx[q]];
           const int type = t->dial_type[d];
           const int n = knots[d];
           if ((type == GCM_DIAL_BILINEAR || type == GCM_DIAL_BICUBIC) && grid0 > q)
             grid0 = (uint32_t)q;
-          if ((type == GCM_DIAL_TABULATED || type == GCM_DIAL_KRIGED) && tab0 > q)
-            tab0 = (uint32_t)q;
+          const bool table_dial = type == GCM_DIAL_TABULATED || type == GCM_DIAL_KRIGED;
           const double* p = t->dial_data + t->data_begin[d];
           R* out = rec.data() + h.rec_begin + off;
           switch (type) {
@@ -1259,9 +1396,14 @@ int build_layout(gcm_context* c, const gcm_tables* t) {
                                    ((uint64_t)in_class[in] << 28) |
                                    ((uint64_t)slot[idx[q]] << 32) | ((uint64_t)off << 48);
           off += record_words(type, n, fp32, t->data_begin[d + 1] - t->data_begin[d]);
+          if (table_dial) {
+            ++n_tab;
+            tab_words = off;
+          }
         }
         h.grid_begin = (uint16_t)grid0;  // the 2-D types sort last
-        h.tab_begin = (uint16_t)std::min(tab0, grid0);
+        h.n_tab = (uint16_t)n_tab;
+        h.tab_words = (uint16_t)tab_words;
       }
     });
   }
@@ -1271,6 +1413,19 @@ int build_layout(gcm_context* c, const gcm_tables* t) {
   c->n_dials = t->n_dials;
   c->n_tiles = T;
   c->n_partials = np;
+  // table dials per tile that the table waves should cover in one round:
+  // the 99th percentile over the tiles
+  {
+    std::vector<int> nt(T);
+    for (int64_t ti = 0; ti < T; ++ti) nt[ti] = tiles[ti].n_tab;
+    int hi = 0;
+    if (T > 0) {
+      const int64_t at = std::min<int64_t>(T - 1, (T * 99) / 100);
+      std::nth_element(nt.begin(), nt.begin() + at, nt.end());
+      hi = nt[at];
+    }
+    c->tab_dials_hi = hi;
+  }
   // the producer wave copies whole KiB chunks: pad both streams by 1 KiB
   desc.resize(desc.size() + 1024 / sizeof(uint64_t), 0);
   rec.resize(rec.size() + 1024 / sizeof(R), R(0));
@@ -1888,8 +2043,15 @@ static int enqueue_fill(gcm_context* c, hipStream_t s, int K, bool copies,
   const size_t ring_lds =
       (size_t)c->launch.ring_buffers * ((size_t)a.rec_bytes + (size_t)a.max_dials * 8 +
                                         kMaxTileEvents * sizeof(EventRec) + kHeaderBytes) +
-      2 * (size_t)a.max_dials * 8;
+      2 * (size_t)a.max_dials * 8 + kReadyBytes;
   const size_t staged_pad = (in_bytes + kClassBytes + 1023) / 1024 * 1024;
+  // Table waves: enough to take the tiles' table dials in one round, while
+  // at least one evaluator wave is left for the other dials.
+  {
+    const int n_eval = c->launch.block_threads / kWave;
+    const int want = (c->tab_dials_hi + kWave - 1) / kWave;
+    a.tab_waves = std::max(0, std::min(want, n_eval - 1));
+  }
   a.inputs_lds = in_bytes <= kInputsLdsMax &&
                  (staged_pad + ring_lds) * (size_t)c->launch.blocks_per_cu <= kLdsPerCU;
   const size_t in_pad = a.inputs_lds ? staged_pad : (kClassBytes + 1023) / 1024 * 1024;
