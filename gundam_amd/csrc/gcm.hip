@@ -873,7 +873,7 @@ struct gcm_context {
   int n_inputs = 0, n_par = 0, n_bins = 0, n_tables = 0;
   int64_t table_len = 0;
   int max_dials = 1024;
-  int tab_dials_hi = 0;  // table dials per tile, 99th percentile over tiles
+  int tab_dials_mid = 0;  // table dials per tile, median over tiles
   int fin_blocks = 1;
   double cap = __builtin_huge_val();
   LlhParams llh{GCM_LLH_POISSON, 0, 1e150, -1.0, 0, 0, nullptr, nullptr, nullptr};
@@ -1413,18 +1413,19 @@ int build_layout(gcm_context* c, const gcm_tables* t) {
   c->n_dials = t->n_dials;
   c->n_tiles = T;
   c->n_partials = np;
-  // table dials per tile that the table waves should cover in one round:
-  // the 99th percentile over the tiles
+  // table dials per tile that the table waves cover in one round: the
+  // median over the tiles (a tile with more does the rest within its step;
+  // covering the 99th percentile instead took an evaluator wave from the
+  // other dials and was slower)
   {
     std::vector<int> nt(T);
     for (int64_t ti = 0; ti < T; ++ti) nt[ti] = tiles[ti].n_tab;
-    int hi = 0;
+    int mid = 0;
     if (T > 0) {
-      const int64_t at = std::min<int64_t>(T - 1, (T * 99) / 100);
-      std::nth_element(nt.begin(), nt.begin() + at, nt.end());
-      hi = nt[at];
+      std::nth_element(nt.begin(), nt.begin() + T / 2, nt.end());
+      mid = nt[T / 2];
     }
-    c->tab_dials_hi = hi;
+    c->tab_dials_mid = mid;
   }
   // the producer wave copies whole KiB chunks: pad both streams by 1 KiB
   desc.resize(desc.size() + 1024 / sizeof(uint64_t), 0);
@@ -2045,11 +2046,11 @@ static int enqueue_fill(gcm_context* c, hipStream_t s, int K, bool copies,
                                         kMaxTileEvents * sizeof(EventRec) + kHeaderBytes) +
       2 * (size_t)a.max_dials * 8 + kReadyBytes;
   const size_t staged_pad = (in_bytes + kClassBytes + 1023) / 1024 * 1024;
-  // Table waves: enough to take the tiles' table dials in one round, while
-  // at least one evaluator wave is left for the other dials.
+  // Table waves: enough to take the median tile's table dials in one round,
+  // while at least one evaluator wave is left for the other dials.
   {
     const int n_eval = c->launch.block_threads / kWave;
-    const int want = (c->tab_dials_hi + kWave - 1) / kWave;
+    const int want = (c->tab_dials_mid + kWave - 1) / kWave;
     a.tab_waves = std::max(0, std::min(want, n_eval - 1));
   }
   a.inputs_lds = in_bytes <= kInputsLdsMax &&
