@@ -96,6 +96,49 @@ def test_batched_points_match_single():
         assert cm.llh()[0][0] == llh_k[k]
 
 
+@pytest.mark.parametrize("shape,K,prec", [
+    # (tile words, evaluator threads, blocks per CU, LDS buffers)
+    ((2048, 256, 2, 3), 3, "fp64"),  # three buffers: two producers flag alternate tiles
+    ((3840, 128, 2, 2), 2, "fp64"),  # one table wave, ~90 table dials a tile: extra rounds
+    ((3840, 384, 2, 2), 1, "fp64"),  # the 5M-event shape: two table waves
+    ((3840, 64, 2, 2), 2, "fp64"),   # one evaluator wave: no table waves
+    ((3584, 384, 2, 2), 3, "fp32"),
+])
+def test_table_wave_shapes_match_oracle(shape, K, prec):
+    """The Tabulated/Kriged dials go to table waves that load their table
+    values one step ahead (the tile's table part flagged in LDS by the
+    producer that copied it); every launch shape and point count gives the
+    oracle's bin sums (FP64: 1e-10, weights bit-exact; FP32: 1e-4)."""
+    tw, bt, bpc, rb = shape
+    print(f"table-wave shape {shape} K={K} {prec}", flush=True)
+    wl, t = _setup(2, 30000)
+    orc = pyoracle.Oracle(t, n_threads=8)
+    cm = CacheManager(0, precision=prec, tile_words=tw, block_threads=bt,
+                      blocks_per_cu=bpc, ring_buffers=rb)
+    cm.build(t, max_points=K)
+    cm.set_event_weight_copy(True)
+    pts = wl.random_points(K, 0.5, seed=13)
+    tabs = wl.table_values(pts)
+    cm.set_parameters(pts)
+    cm.set_table_values(tabs)
+    cm.fill().get()
+    w = cm.copy_event_weights(np.full(t.n_events, np.nan))  # point 0
+    binned = t.bin >= 0
+    for k in range(K):
+        orc.set_table_values(tabs[k])
+        w_ref, s_ref, e_ref = orc.propagate(pts[k])
+        s, e = cm.copy_histogram_contents(k)
+        if prec == "fp64":
+            np.testing.assert_allclose(s, s_ref, rtol=RTOL_SUM, atol=0)
+            np.testing.assert_allclose(e, e_ref, rtol=RTOL_SUM, atol=0)
+            if k == 0:
+                np.testing.assert_array_equal(w[binned], w_ref[binned])
+        else:
+            nz = s_ref > 0
+            assert (np.abs(s[nz] - s_ref[nz]) / s_ref[nz]).max() < 1e-4
+            assert np.all(s[~nz] == 0)
+
+
 def test_fp32_mode_error_bounds():
     print("fp32 workload=2", flush=True)
     wl, t = _setup(2, 40000)
