@@ -198,7 +198,11 @@ def batched_workload3(args, log):
         stencil[2 * j, j] += 1e-3
         stencil[2 * j + 1, j] -= 1e-3
     pts = np.concatenate([chains, stencil])
-    tuned = autotune(t, pts, None, n_events=t.n_events, reps=3)
+    if args.no_autotune:
+        tuned = {"block_threads": args.block or 384, "tile_words": args.tile_words or 2048,
+                 "blocks_per_cu": args.blocks_per_cu or 2, "ring_buffers": args.ring or 3}
+    else:
+        tuned = autotune(t, pts, None, n_events=t.n_events, reps=3)
     cm = CacheManager(0, block_threads=tuned["block_threads"],
                       tile_words=tuned["tile_words"],
                       blocks_per_cu=tuned["blocks_per_cu"],
@@ -506,8 +510,10 @@ def mlp_kernels(args, log, n_in=300, hidden=512, n_out=6000, rows=16384, batch=2
             "large_batch": {"batch": big_batch, "steps": steps_big,
                             "us_per_step": el_big / steps_big * 1e6, "tflops": tflops_big,
                             "frac": tflops_big / MFMA_BF16_PEAK_TFLOPS},
-            "bound": ("launch latency and L2/MALL operand re-reads: 16 launches per "
-                      "5.3-GFLOP step, 64x64 tiles")}
+            "bound": ("256 rows: launch and load latency (16 launches of 3-19 us per "
+                      "5.3-GFLOP step); 4096 rows: the three large GEMMs (216-399 TFLOP/s) "
+                      "and the memory-bound loss head, row gather, split-K reductions "
+                      "and Adam")}
 
 
 def posterior_agreement(args, log):
@@ -676,6 +682,10 @@ def main():
                     help="batched exact burn-in iterations of the posterior study")
     ap.add_argument("--post-workload", type=int, default=1, choices=[1, 2],
                     help="posterior study on the 100k-event (1) or 5M-event (2) model")
+    ap.add_argument("--only-c3", action="store_true",
+                    help="run only the 64-point batched leg on the 100k-event set (with "
+                         "--no-autotune: the shape from --tile-words/--block/"
+                         "--blocks-per-cu/--ring, default 2048/384/2/3)")
     ap.add_argument("--only-mlp", action="store_true",
                     help="run only the surrogate network's training-kernel timing")
     args = ap.parse_args()
@@ -696,6 +706,9 @@ def main():
         return
     if args.only_mlp:
         print(json.dumps({"surrogate_mlp": mlp_kernels(args, log)}), flush=True)
+        return
+    if args.only_c3:
+        print(json.dumps({"batched_c3": batched_workload3(args, log)}), flush=True)
         return
 
     t0 = time.time()
