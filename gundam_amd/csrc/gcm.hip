@@ -587,7 +587,6 @@ __global__ void __launch_bounds__(1024, 6) fill_kernel(FillArgs a) {
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   const int lane = threadIdx.x % kWave;
   const int n_eval = nwaves - 1 - P;
-  const int eval_threads = n_eval * kWave;
   const int ev_wave = n_eval;
   const int prod0 = n_eval + 1;
   // LDS: [input x values if staged][class records][tile ring][responses]
