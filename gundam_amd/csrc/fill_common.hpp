@@ -29,11 +29,11 @@ constexpr int kWave = 64;
 constexpr int kMaxTileEvents = 64;
 // Launches of at least kPointsMin points evaluate with one lane per point
 // (fill_points_kernel); each tile's events are split into kRanges contiguous
-// ranges, one per evaluator wave, each with its own partial slots (14: with
-// the producers at most 16 waves a block; 64-point launches at 5M events
-// 22.5 ms against 27.3 ms with 8 ranges).
+// ranges, one per evaluator wave, each with its own partial slots.  10
+// ranges: with up to two producers, two blocks of 12 waves fit a CU at six
+// waves per SIMD (measured against 8-14 ranges at 5-8 waves per SIMD).
 constexpr int kPointsMin = 16;
-constexpr int kRanges = 14;
+constexpr int kRanges = 10;
 constexpr int kDefaultBlock = 256;
 constexpr int kDefaultTileWords = 2048;  // 16 KiB of records per tile
 constexpr int kDefaultBlocksPerCU = 3;

@@ -37,10 +37,10 @@ __device__ __forceinline__ double lane_f64(double v, int j) {
 // landed, writes the tile's event-major index (dial-list position -> the
 // type-sorted descriptor) into the buffer, so no evaluator searches.
 
-// Launch bounds: the block's kRanges + NB - 1 waves, and registers for
-// seven waves per SIMD, so that two blocks share a CU.
+// Launch bounds: the block's kRanges + NB - 1 waves, and registers for six
+// waves per SIMD, so that two blocks share a CU.
 template <typename R, bool kComp, int NB, bool kGrid>
-__global__ void __launch_bounds__((kRanges + NB - 1) * kWave, 7)
+__global__ void __launch_bounds__((kRanges + NB - 1) * kWave, 6)
     fill_points_kernel(FillArgs a, PointsArgs pa) {
   constexpr int P = NB - 1;  // producer waves
   constexpr int W = kRanges;  // evaluator waves
