@@ -71,7 +71,8 @@ struct GemmArgs {
   int64_t ldb;
   int vec;    // both operands meet the alignment of the unchecked vector loads
   int yfast;  // tile order: M tiles fastest (B the larger operand), else N tiles
-  float* C;   // C[m * ldc + n]
+  float* C;   // C[m * ldc + n] (FP32), or
+  __bf16* Cb;  // the same in BF16, when set (an operand of later GEMMs only)
   int64_t ldc;
   // split-K launches (gridDim.z > 1): split z writes its partial product to
   // ws + z * split_stride (dense, ld N); a reduction launch adds them in
@@ -234,7 +235,8 @@ __device__ __forceinline__ void finish_store(const GemmArgs& g, int row, int col
     v = gelu(v);
   }
   if (g.gz) v *= gelu_grad(g.gz[o]);
-  g.C[o] = v;
+  if (g.Cb) g.Cb[o] = (__bf16)v;
+  else g.C[o] = v;
 }
 
 // One K step (64) of the wave's (BM/2) x (BN/2) sub-tile from the LDS tiles.
@@ -457,11 +459,13 @@ __global__ void colsum_final_kernel(const float* __restrict__ part, int S, int N
 }
 
 // The loss head of the backward pass, laid out like colsum_kernel:
-// dY = 2 (Y - T) / (M N), the (slice) column sums of dY, and the block's sum
-// of squared errors in partial[slice][blockIdx.x] (each in a fixed order).
+// dY = 2 (Y - T) / (M N), stored in BF16 (the GEMMs that read it round their
+// operands to BF16 anyway), the (slice) column sums of dY in FP32, and the
+// block's sum of squared errors in partial[slice][blockIdx.x] (each in a
+// fixed order).
 __global__ void __launch_bounds__(64 * kColsumSlices)
 mse_colsum_kernel(const float* __restrict__ y, const float* __restrict__ t,
-                  float* __restrict__ dy, int M, int N, float scale, float* __restrict__ out,
+                  __bf16* __restrict__ dy, int M, int N, float scale, float* __restrict__ out,
                   float* __restrict__ partial) {
   __shared__ float part[kColsumSlices][65];
   __shared__ float sq[kColsumSlices][65];
@@ -475,7 +479,7 @@ mse_colsum_kernel(const float* __restrict__ y, const float* __restrict__ t,
       const int64_t i = (int64_t)m * N + c;
       const float e = y[i] - t[i];
       const float g = 2.0f * e * scale;
-      dy[i] = g;
+      dy[i] = (__bf16)g;
       s += g;
       e2 += e * e;
     }
@@ -714,9 +718,11 @@ struct gcm_mlp {
   int64_t adam_t = 0;
   // activations for a batch of up to cap rows
   int cap = 0;
-  float *x = nullptr, *z1 = nullptr, *h1 = nullptr, *z2 = nullptr, *h2 = nullptr,
-        *y = nullptr, *t = nullptr, *dy = nullptr, *d2 = nullptr, *d1 = nullptr,
-        *partial = nullptr, *colpart = nullptr;
+  // (the GELU outputs h1, h2 and the loss gradient dy are read only by
+  // GEMMs, which take BF16 operands: they are stored in BF16)
+  float *x = nullptr, *z1 = nullptr, *z2 = nullptr, *y = nullptr, *t = nullptr, *d2 = nullptr,
+        *d1 = nullptr, *partial = nullptr, *colpart = nullptr;
+  __bf16 *h1 = nullptr, *h2 = nullptr, *dy = nullptr;
   // split-K partial sums
   float* ws = nullptr;
   size_t ws_cap = 0;
@@ -880,16 +886,18 @@ struct Exec {
   size_t ws_cap;  // floats
 };
 
+// C is FP32 (C) or BF16 (Cb, with C null).
 int gemm(gcm_mlp* m, int M, int N, int K, const void* A, int64_t lda, int a_kc, int a_bf16,
          const void* B, int64_t ldb, int b_kc, int b_bf16, float* C, int64_t ldc,
-         const float* bias, float* Z, const float* gz = nullptr, const Exec* ex = nullptr) {
+         const float* bias, float* Z, const float* gz = nullptr, const Exec* ex = nullptr,
+         __bf16* Cb = nullptr) {
   if (M <= 0 || N <= 0) return GCM_OK;
-  if (gz && ldc != N) return fail(GCM_ERR_ARG, "gemm: GELU backward needs a dense output");
+  if (gz && (ldc != N || Cb)) return fail(GCM_ERR_ARG, "gemm: GELU backward needs a dense FP32 output");
   const hipStream_t stream = ex ? ex->stream : m->stream;
   const GemmPlan p = gemm_plan(M, N, K);
   GemmArgs g{M, N, K, A, lda, B, ldb, vec_ok(A, lda, a_kc, a_bf16, M) && vec_ok(B, ldb, b_kc, b_bf16, N),
              (int64_t)N * K * (b_bf16 ? 2 : 4) > (int64_t)M * K * (a_bf16 ? 2 : 4),
-             C, ldc, nullptr, 0, p.chunk_steps * TK, bias, Z, Z != nullptr, gz};
+             C, Cb, ldc, nullptr, 0, p.chunk_steps * TK, bias, Z, Z != nullptr, gz};
   const int sel = (a_kc ? 8 : 0) | (a_bf16 ? 4 : 0) | (b_kc ? 2 : 0) | (b_bf16 ? 1 : 0);
   const dim3 grid((N + p.T - 1) / p.T, (M + p.T - 1) / p.T, p.splits);
   if (p.splits == 1) {
@@ -918,17 +926,17 @@ int gemm(gcm_mlp* m, int M, int N, int K, const void* A, int64_t lda, int a_kc, 
 }
 
 // Forward: h1 = gelu(x W1^T + b1), h2 = gelu(h1 W2^T + b2), y = h2 W3^T +
-// b3, with the pre-activations in z1, z2.
-int forward(gcm_mlp* m, int rows, const float* x, float* z1, float* h1, float* z2, float* h2,
-            float* y, const Exec* ex) {
+// b3, with the pre-activations in z1, z2 (FP32) and h1, h2 in BF16.
+int forward(gcm_mlp* m, int rows, const float* x, float* z1, __bf16* h1, float* z2,
+            __bf16* h2, float* y, const Exec* ex) {
   const __bf16* w = m->pb;
-  int rc = gemm(m, rows, m->H, m->n_in, x, m->n_in, 1, 0, w, m->bmap.n_in_pad, 1, 1, h1,
-                m->H, m->p + m->off_b1, z1, nullptr, ex);
+  int rc = gemm(m, rows, m->H, m->n_in, x, m->n_in, 1, 0, w, m->bmap.n_in_pad, 1, 1, nullptr,
+                m->H, m->p + m->off_b1, z1, nullptr, ex, h1);
   if (rc) return rc;
-  rc = gemm(m, rows, m->H, m->H, h1, m->H, 1, 0, w + m->bmap.bw2, m->H, 1, 1, h2, m->H,
-            m->p + m->off_b2, z2, nullptr, ex);
+  rc = gemm(m, rows, m->H, m->H, h1, m->H, 1, 1, w + m->bmap.bw2, m->H, 1, 1, nullptr, m->H,
+            m->p + m->off_b2, z2, nullptr, ex, h2);
   if (rc) return rc;
-  return gemm(m, rows, m->n_out, m->H, h2, m->H, 1, 0, w + m->bmap.bw3, m->H, 1, 1, y,
+  return gemm(m, rows, m->n_out, m->H, h2, m->H, 1, 1, w + m->bmap.bw3, m->H, 1, 1, y,
               m->n_out, m->p + m->off_b3, nullptr, nullptr, ex);
 }
 
@@ -961,15 +969,15 @@ int backward(gcm_mlp* m, int rows, double* loss) {
                                                                        g + m->off_b3);
   MLP_HIP(hipGetLastError());
   // dW3 = dY^T h2 : (n_out x rows) . (rows x H)
-  int rc = gemm(m, m->n_out, m->H, rows, m->dy, m->n_out, 0, 0, m->h2, m->H, 0, 0,
+  int rc = gemm(m, m->n_out, m->H, rows, m->dy, m->n_out, 0, 1, m->h2, m->H, 0, 1,
                 g + m->off_w3, m->H, nullptr, nullptr);
   if (rc) return rc;
   // dH2 = (dY W3) * gelu'(z2) : (rows x n_out) . (n_out x H)
-  rc = gemm(m, rows, m->H, m->n_out, m->dy, m->n_out, 1, 0, w + m->bmap.bw3, m->H, 0, 1, m->d2,
+  rc = gemm(m, rows, m->H, m->n_out, m->dy, m->n_out, 1, 1, w + m->bmap.bw3, m->H, 0, 1, m->d2,
             m->H, nullptr, nullptr, m->z2);
   if (rc) return rc;
   // dW2 = dH2^T h1
-  rc = gemm(m, m->H, m->H, rows, m->d2, m->H, 0, 0, m->h1, m->H, 0, 0, g + m->off_w2, m->H,
+  rc = gemm(m, m->H, m->H, rows, m->d2, m->H, 0, 0, m->h1, m->H, 0, 1, g + m->off_w2, m->H,
             nullptr, nullptr);
   if (rc) return rc;
   colsum(m->d2, m->H, g + m->off_b2);
@@ -1169,7 +1177,8 @@ int gcm_mlp_predict_llh(gcm_mlp* m, const float* x, int64_t rows, double* chi2) 
 // the scratch, each part 256-byte aligned: z1, h1, z2, h2 [rows][H], y
 // [rows][n_out], then the split-K partials of the largest split GEMM.  The
 // same kernels with the same split decisions as gcm_mlp_predict_llh, so the
-// results are the same for the same rows.  Nothing in the scratch needs to
+// results are the same for the same rows (h1, h2 in BF16 in their FP32-sized
+// slots).  Nothing in the scratch needs to
 // start at zero (every kernel writes what it later reads).
 int gcm_mlp_predict_llh_noalloc(gcm_mlp* m, const float* x_device, int64_t rows,
                                 double* chi2_device, void* stream_ptr) {
@@ -1191,9 +1200,9 @@ int gcm_mlp_predict_llh_noalloc(gcm_mlp* m, const float* x_device, int64_t rows,
   MLP_HIP(hipSetDevice(m->device));
   unsigned char* base = m->scratch;
   float* z1 = reinterpret_cast<float*>(base);
-  float* h1 = reinterpret_cast<float*>(base + act);
+  __bf16* h1 = reinterpret_cast<__bf16*>(base + act);
   float* z2 = reinterpret_cast<float*>(base + 2 * act);
-  float* h2 = reinterpret_cast<float*>(base + 3 * act);
+  __bf16* h2 = reinterpret_cast<__bf16*>(base + 3 * act);
   float* y = reinterpret_cast<float*>(base + 4 * act);
   const Exec ex{stream_ptr ? (hipStream_t)stream_ptr : m->stream,
                 reinterpret_cast<float*>(base + 4 * act + out), ws};
