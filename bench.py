@@ -511,7 +511,7 @@ def mlp_kernels(args, log, n_in=300, hidden=512, n_out=6000, rows=16384, batch=2
                             "us_per_step": el_big / steps_big * 1e6, "tflops": tflops_big,
                             "frac": tflops_big / MFMA_BF16_PEAK_TFLOPS},
             "bound": ("256 rows: launch and load latency (16 launches of 3-19 us per "
-                      "5.3-GFLOP step); 4096 rows: the three large GEMMs (216-399 TFLOP/s) "
+                      "5.3-GFLOP step); 4096 rows: the three large GEMMs (273-436 TFLOP/s) "
                       "and the memory-bound loss head, row gather, split-K reductions "
                       "and Adam")}
 
