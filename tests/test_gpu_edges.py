@@ -1,8 +1,9 @@
 """Edge cases of the tables against the oracle: no events, every event
 unbinned, events without dials, bins without events, tiles closed by the
 64-event limit (norm-only events carry no record words), ragged dial counts
-and spline sizes including the largest knot counts, the reweight cap, one
-event, and an event too large for a tile (a clean build error).
+and spline sizes including the largest knot counts, uniform-knot splines
+with power-of-two and other knot steps, the reweight cap, one event, and an
+event too large for a tile (a clean build error).
 
 Bars as in the parity tests: weights bit-exact, sums within 1e-10 relative,
 LLH within 1e-10 relative.  Every case runs one point per launch, a 4-point
@@ -178,6 +179,42 @@ def test_ragged_dials_largest_knots_and_cap():
     th = [np.zeros(n_par), rng.normal(0, 1, n_par), rng.normal(0, 2.5, n_par),
           np.full(n_par, 1.7)]
     _check(t, th)
+
+
+def test_uniform_knot_steps():
+    """Uniform-knot splines (monotonic, compact, uniform) with knot steps
+    2^-10 .. 2^8, -1/2 and steps that are not powers of two (the knot index
+    is the correctly rounded quotient (x - x0) / step in every kernel), x
+    inside and far outside the knots.  Each rotation of the points puts
+    another point first, so that the K-point launch's event weights are
+    compared bit-exact for every point."""
+    print("edges: uniform knot steps", flush=True)
+    rng = np.random.default_rng(5)
+    n_par = 5
+    steps = [2.0 ** k for k in (-10, -3, -1, 0, 1, 3, 8)] + [-0.5, 0.75, 1.0 / 3.0, 3.0, 0.1]
+    ev = []
+    for i in range(1500):
+        dials = []
+        for j in range(int(rng.integers(1, 6))):
+            st = float(rng.choice(steps))
+            n = int(rng.integers(3, 12))
+            low = float(rng.uniform(-3.0, 0.0))
+            kind = int(rng.integers(0, 3))
+            y = 1 + 0.1 * rng.standard_normal(n)
+            if kind == 2:
+                pairs = np.empty(2 * n)
+                pairs[0::2], pairs[1::2] = y, 0.1 * rng.standard_normal(n)
+                dials.append((N.DIAL_UNIFORM, int(rng.integers(0, n_par)),
+                              np.concatenate([[low, st], pairs])))
+            else:
+                dials.append((N.DIAL_MONOTONIC if kind == 0 else N.DIAL_COMPACT,
+                              int(rng.integers(0, n_par)), np.concatenate([[low, st], y])))
+        ev.append((rng.uniform(0.1, 1.0), int(rng.integers(0, 10)), dials))
+    t = _tables(ev, n_par, 10)
+    th = [np.zeros(n_par), rng.normal(0, 1, n_par), rng.normal(0, 300, n_par),
+          np.full(n_par, -0.02)]
+    for r in range(len(th)):
+        _check(t, th[r:] + th[:r])
 
 
 def test_single_event_and_oversized_event():
