@@ -16,12 +16,6 @@ __device__ __forceinline__ uint32_t lane_u32(uint32_t v, int j) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, j);
 }
 
-__device__ __forceinline__ double lane_f64(double v, int j) {
-  const uint64_t b = (uint64_t)__double_as_longlong(v);
-  const uint64_t lo = lane_u32((uint32_t)b, j), hi = lane_u32((uint32_t)(b >> 32), j);
-  return __longlong_as_double((long long)(lo | hi << 32));
-}
-
 // K points per launch with one lane per point (K >= kPointsMin).  The tiles
 // stream into the same LDS ring as in fill_kernel; each tile's events are cut
 // into kRanges contiguous ranges, one per evaluator wave.  A wave walks its
@@ -173,10 +167,9 @@ __global__ void __launch_bounds__((kRanges + NB - 1) * kWave, 6)
       };
       if (e0 < e1) {
         // the range's dials are the event-major slots [S0, S1), taken 64 at
-        // a time: lane j reads the descriptor of slot sb + j and its input's
-        // supervisor bounds, and the walk over the batch broadcasts them
-        // with readlane; the points' x of the next dial is loaded while the
-        // current one is evaluated
+        // a time: lane j reads the descriptor of slot sb + j, and the walk
+        // over the batch broadcasts it with readlane; the points' x of the
+        // next dial is loaded while the current one is evaluated
         const int S0 = (int)(s_ev[e0].dials & 0xffff);
         const EventRec last = s_ev[e1 - 1];
         const int S1 = (int)(last.dials & 0xffff) + (int)(last.dials >> 16);
@@ -188,17 +181,6 @@ __global__ void __launch_bounds__((kRanges + NB - 1) * kWave, 6)
           const int nbat = S1 - sb < kWave ? S1 - sb : kWave;
           const uint64_t dv = s_desc[s_inv[sb + (lane < nbat ? lane : 0)]];
           const uint32_t vlo = (uint32_t)dv, vhi = (uint32_t)(dv >> 32);
-          const int vcl = (int)((vlo >> 28) & 0xf);
-          // the supervisor does not depend on the point: the class record,
-          // or for the escape class the input's own record
-          double blo = s_cls[vcl].lo, bhi = s_cls[vcl].hi;
-          int bfl = s_cls[vcl].flags;
-          if (vcl == kEscapeClass) {
-            const InputRec* ir = a.xin + (vlo & 0xffff);
-            blo = ir->lo;
-            bhi = ir->hi;
-            bfl = ir->flags;
-          }
           double xn = pa.xT[(int64_t)(lane_u32(vlo, 0) & 0xffff) * K + kk];
           for (int j = 0; j < nbat; ++j) {
             const int sl = sb + j;
@@ -214,7 +196,22 @@ __global__ void __launch_bounds__((kRanges + NB - 1) * kWave, 6)
             const int type = (int)((dlo >> 16) & 0xf);
             const int nk = (int)((dlo >> 20) & 0xff);
             const int off = (int)(dhi >> 16);
-            const int flags = (int)lane_u32((uint32_t)bfl, j);
+            // the supervisor does not depend on the point: the class record
+            // (a wave-uniform LDS read), or for the escape class the input's
+            // own record
+            const int cl = (int)((dlo >> 28) & 0xf);
+            double lo, hi;
+            int flags;
+            if (cl != kEscapeClass) {
+              lo = s_cls[cl].lo;
+              hi = s_cls[cl].hi;
+              flags = __builtin_amdgcn_readfirstlane(s_cls[cl].flags);
+            } else {
+              const InputRec* ir = a.xin + (dlo & 0xffff);
+              lo = ir->lo;
+              hi = ir->hi;
+              flags = __builtin_amdgcn_readfirstlane(ir->flags);
+            }
             double r;
             if (kGrid && (type == GCM_DIAL_BILINEAR || type == GCM_DIAL_BICUBIC)) {
               const int in2 = __builtin_amdgcn_readfirstlane((int)s_rec[off]);
@@ -224,7 +221,6 @@ __global__ void __launch_bounds__((kRanges + NB - 1) * kWave, 6)
               r = eval_dial<R, true>(type, nk, s_rec + off, x, flags, pa.tabT + kk, K);
             }
             // DialResponseSupervisor::process (DialResponseSupervisor.cpp:10-18)
-            const double lo = lane_f64(blo, j), hi = lane_f64(bhi, j);
             if (r < lo) r = lo;
             else if (r > hi) r = hi;
             temp *= r;
