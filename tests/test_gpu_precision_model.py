@@ -6,9 +6,10 @@ precision study validates its reference, and the FP32 mode against the FP64
 path on the same points.
 
 Bars: FP64 bin sums and sqrt(sum w^2) within 1e-10 relative of the oracle,
-point 0's event weights bit-exact; FP32 per-bin relative error below 5e-3
-(the precision study measures <= 1.5e-3 over 1000 points) and the
-Barlow-Beeston LLH within 1e-5 relative."""
+point 0's event weights bit-exact; FP32 per-bin relative error below 2e-4
+(measured at these seeded points: at most 1.4e-5; the precision study's
+20M-event run 5.3e-5 over 128 points) and the Barlow-Beeston LLH within 1e-5
+relative."""
 import numpy as np
 import pytest
 
@@ -49,7 +50,10 @@ def test_precision_model_fp64_matches_oracle_fp32_bounded():
         np.testing.assert_allclose(np.sqrt(s2), e_ref, rtol=1e-10, atol=0)
         f, _ = cm32.hist(k)
         nz = s != 0
-        assert np.max(np.abs(f[nz] - s[nz]) / np.abs(s[nz])) < 5e-3
+        rel = np.max(np.abs(f[nz] - s[nz]) / np.abs(s[nz]))
+        print(f"point {k}: FP32 max per-bin rel err {rel:.3g}, "
+              f"|dLLH| {abs(llh32[k] - llh64[k]):.3g}", flush=True)
+        assert rel < 2e-4
         assert np.all(f[~nz] == 0)
         assert abs(llh32[k] - llh64[k]) <= 1e-5 * abs(llh64[k])
         if k == 0:
