@@ -54,6 +54,7 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 // per step.  A 128 x 128 tile reads each operand byte once per 128 outputs
 // instead of 64, halving the operand traffic from L2 per FLOP.
 constexpr int TM = 64, TN = 64, TK = 64;
+constexpr int kWaveMlp = 64;
 constexpr int kBig = 128;  // the large tile's side
 constexpr int LDK = TK + 8;
 constexpr int kMaxSplit = 16;
@@ -86,6 +87,19 @@ struct GemmArgs {
   float* Z;           // pre-activation copy (GELU only) or null
   int gelu;
   const float* gz;
+  // The loss head fused into the output layer (unsplit launches, head set):
+  // for y = C + bias, dY = 2 (y - T) scale is stored in BF16 to dyb (ld N)
+  // instead of C, with T's row tidx[row] (or row) at tgt + row * ldt; each
+  // block writes its column sums of dY to hcol[tile row][n] and its sum of
+  // squared errors to hsq[tile row][tile column], each in a fixed order.
+  int head;
+  const float* tgt;
+  const int32_t* tidx;
+  int64_t ldt;
+  __bf16* dyb;
+  float scale;
+  float* hcol;
+  float* hsq;
 };
 
 // One ROWS x 64 (k) operand tile (ROWS = 64 or 128) held in registers
@@ -365,6 +379,67 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
     __syncthreads();
   }
   const int splits = gridDim.z;
+  if (g.head) {
+    // the loss head: per lane, its rows of each of its columns in a fixed
+    // order; lane l then adds lane l + 32 (the other rows of the same
+    // columns), the two waves of a column half add in wave order through LDS
+    float* hs = reinterpret_cast<float*>(gemm_lds);  // [2][BN] column sums, [4] squared errors
+    float csum[NJ];
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) csum[j] = 0.0f;
+    float sq = 0.0f;
+#pragma unroll
+    for (int i = 0; i < MI; ++i) {
+      // the target rows of this lane's 16 rows, then each column's 16 target
+      // values, all loaded unconditionally (indices clamped) so that they
+      // are in flight together
+      int64_t src[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int row = min(m0 + wm * (BM / 2) + i * 32 + (r / 4) * 8 + (lane / 32) * 4 + r % 4,
+                            g.M - 1);
+        src[r] = (g.tidx ? (int64_t)g.tidx[row] : (int64_t)row) * g.ldt;
+      }
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) {
+        const int col = n0 + wn * (BN / 2) + j * 32 + lane % 32;
+        const int cc = min(col, g.N - 1);
+        const float bcol = g.bias[cc];
+        float tv[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) tv[r] = g.tgt[src[r] + cc];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const int row = m0 + wm * (BM / 2) + i * 32 + (r / 4) * 8 + (lane / 32) * 4 + r % 4;
+          if (row < g.M && col < g.N) {
+            const float e = (acc[i][j][r] + bcol) - tv[r];
+            const float gr = 2.0f * e * g.scale;
+            g.dyb[(int64_t)row * g.N + col] = (__bf16)gr;
+            csum[j] += gr;
+            sq += e * e;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int j = 0; j < NJ; ++j) csum[j] += __shfl_down(csum[j], 32);
+#pragma unroll
+    for (int off = kWaveMlp / 2; off > 0; off >>= 1) {
+      const float o = __shfl_xor(sq, off);
+      sq = (lane & off) == 0 ? sq + o : o + sq;
+    }
+    if (lane < 32) {
+#pragma unroll
+      for (int j = 0; j < NJ; ++j) hs[wm * BN + wn * (BN / 2) + j * 32 + lane] = csum[j];
+    }
+    if (lane == 0) hs[2 * BN + wave] = sq;
+    __syncthreads();
+    for (int c = tid; c < BN; c += blockDim.x)
+      if (n0 + c < g.N) g.hcol[(int64_t)by * g.N + n0 + c] = hs[c] + hs[BN + c];
+    if (tid == 0)
+      g.hsq[(int64_t)by * gx + bx] = ((hs[2 * BN] + hs[2 * BN + 1]) + hs[2 * BN + 2]) + hs[2 * BN + 3];
+    return;
+  }
   float* P = g.ws + (int64_t)bz * g.split_stride;
 #pragma unroll
   for (int i = 0; i < MI; ++i)
@@ -722,6 +797,8 @@ struct gcm_mlp {
   // GEMMs, which take BF16 operands: they are stored in BF16)
   float *x = nullptr, *z1 = nullptr, *z2 = nullptr, *y = nullptr, *t = nullptr, *d2 = nullptr,
         *d1 = nullptr, *partial = nullptr, *colpart = nullptr;
+  // the fused loss head's per-tile column sums and squared errors
+  float *hcol = nullptr, *hsq = nullptr;
   __bf16 *h1 = nullptr, *h2 = nullptr, *dy = nullptr;
   // split-K partial sums
   float* ws = nullptr;
@@ -747,7 +824,7 @@ struct gcm_mlp {
   int ensure_batch(int rows) {
     if (rows <= cap) return GCM_OK;
     for (void* q : {(void*)x, (void*)z1, (void*)h1, (void*)z2, (void*)h2, (void*)y,
-                    (void*)t, (void*)dy, (void*)d2, (void*)d1}) {
+                    (void*)t, (void*)dy, (void*)d2, (void*)d1, (void*)hcol, (void*)hsq}) {
       if (!q) continue;
       (void)hipFree(q);
       allocs.erase(std::remove(allocs.begin(), allocs.end(), q), allocs.end());
@@ -765,6 +842,9 @@ struct gcm_mlp {
     step(alloc(&dy, (size_t)rows * n_out));
     step(alloc(&d2, (size_t)rows * H));
     step(alloc(&d1, (size_t)rows * H));
+    const size_t row_tiles = ((size_t)rows + 63) / 64;  // at most, whatever the tile
+    step(alloc(&hcol, row_tiles * n_out));
+    step(alloc(&hsq, row_tiles * (((size_t)n_out + 63) / 64)));
     return rc;
   }
   int ensure_ws(size_t floats) {
@@ -890,7 +970,7 @@ struct Exec {
 int gemm(gcm_mlp* m, int M, int N, int K, const void* A, int64_t lda, int a_kc, int a_bf16,
          const void* B, int64_t ldb, int b_kc, int b_bf16, float* C, int64_t ldc,
          const float* bias, float* Z, const float* gz = nullptr, const Exec* ex = nullptr,
-         __bf16* Cb = nullptr) {
+         __bf16* Cb = nullptr, const GemmArgs* head = nullptr) {
   if (M <= 0 || N <= 0) return GCM_OK;
   if (gz && (ldc != N || Cb)) return fail(GCM_ERR_ARG, "gemm: GELU backward needs a dense FP32 output");
   const hipStream_t stream = ex ? ex->stream : m->stream;
@@ -900,6 +980,17 @@ int gemm(gcm_mlp* m, int M, int N, int K, const void* A, int64_t lda, int a_kc, 
              C, Cb, ldc, nullptr, 0, p.chunk_steps * TK, bias, Z, Z != nullptr, gz};
   const int sel = (a_kc ? 8 : 0) | (a_bf16 ? 4 : 0) | (b_kc ? 2 : 0) | (b_bf16 ? 1 : 0);
   const dim3 grid((N + p.T - 1) / p.T, (M + p.T - 1) / p.T, p.splits);
+  if (head) {  // the fused loss head (the caller checked that the launch is unsplit)
+    if (p.splits != 1 || !bias) return fail(GCM_ERR_ARG, "gemm: loss head needs an unsplit launch with a bias");
+    g.head = 1;
+    g.tgt = head->tgt;
+    g.tidx = head->tidx;
+    g.ldt = head->ldt;
+    g.dyb = head->dyb;
+    g.scale = head->scale;
+    g.hcol = head->hcol;
+    g.hsq = head->hsq;
+  }
   if (p.splits == 1) {
     g.gz = nullptr;
     MLP_HIP(launch_gemm(p.big, sel, grid, stream, g));
@@ -945,9 +1036,42 @@ int forward(gcm_mlp* m, int rows) {
   return forward(m, rows, m->x, m->z1, m->h1, m->z2, m->h2, m->y, nullptr);
 }
 
-// Loss and gradients into m->g for the rows in m->x / m->t.
-// Loss and gradients into m->g for the rows in m->x / m->t.
+// Whether a training step of `rows` fuses the loss head into the output
+// layer: when that launch is not split along K.
+bool head_fused(const gcm_mlp* m, int rows) {
+  return gemm_plan(rows, m->n_out, m->H).splits == 1;
+}
+
+// The training forward pass on the rows in m->x.  With the loss head fused
+// (head_fused), the output layer writes dY (BF16) to m->dy from the targets'
+// rows (row r at tgt + (tidx ? tidx[r] : r) * n_out) and its per-tile column
+// sums and squared errors, and no output y; otherwise y goes to m->y.
+int forward_train(gcm_mlp* m, int rows, const float* tgt, const int32_t* tidx) {
+  if (!head_fused(m, rows)) return forward(m, rows);
+  const __bf16* w = m->pb;
+  int rc = gemm(m, rows, m->H, m->n_in, m->x, m->n_in, 1, 0, w, m->bmap.n_in_pad, 1, 1,
+                nullptr, m->H, m->p + m->off_b1, m->z1, nullptr, nullptr, m->h1);
+  if (rc) return rc;
+  rc = gemm(m, rows, m->H, m->H, m->h1, m->H, 1, 1, w + m->bmap.bw2, m->H, 1, 1, nullptr, m->H,
+            m->p + m->off_b2, m->z2, nullptr, nullptr, m->h2);
+  if (rc) return rc;
+  GemmArgs hd{};
+  hd.tgt = tgt;
+  hd.tidx = tidx;
+  hd.ldt = m->n_out;
+  hd.dyb = m->dy;
+  hd.scale = 1.0f / (float)((int64_t)rows * m->n_out);
+  hd.hcol = m->hcol;
+  hd.hsq = m->hsq;
+  return gemm(m, rows, m->n_out, m->H, m->h2, m->H, 1, 1, w + m->bmap.bw3, m->H, 1, 1, nullptr,
+              m->n_out, m->p + m->off_b3, nullptr, nullptr, nullptr, nullptr, &hd);
+}
+
+// Loss and gradients into m->g for the rows in m->x / m->t, after
+// forward_train (with the loss head fused, dY and its partial sums are
+// already there).
 int backward(gcm_mlp* m, int rows, double* loss) {
+  const bool fused = head_fused(m, rows);
   const int64_t ny = (int64_t)rows * m->n_out;
   const int nb = (m->n_out + 63) / 64;  // mse_colsum column blocks
   const int S = col_slices(rows);        // row slices: one partial per (slice, block)
@@ -961,12 +1085,23 @@ int backward(gcm_mlp* m, int rows, double* loss) {
     if (S > 1)
       colsum_final_kernel<<<(N + 255) / 256, 256, 0, m->stream>>>(m->colpart, S, N, db);
   };
-  mse_colsum_kernel<<<dim3(nb, S), cs_block, 0, m->stream>>>(
-      m->y, m->t, m->dy, rows, m->n_out, 1.0f / (float)ny, S > 1 ? m->colpart : g + m->off_b3,
-      m->partial);
-  if (S > 1)
-    colsum_final_kernel<<<(m->n_out + 255) / 256, 256, 0, m->stream>>>(m->colpart, S, m->n_out,
-                                                                       g + m->off_b3);
+  // the loss partials: per (row tile, column tile) of the output layer when
+  // fused, else per (row slice, column block) of mse_colsum_kernel
+  const int head_tile = gemm_plan(rows, m->n_out, m->H).T;
+  const int head_rows = (rows + head_tile - 1) / head_tile;
+  const size_t n_loss = fused ? (size_t)head_rows * ((m->n_out + head_tile - 1) / head_tile)
+                              : (size_t)nb * S;
+  if (fused) {
+    colsum_final_kernel<<<(m->n_out + 255) / 256, 256, 0, m->stream>>>(m->hcol, head_rows,
+                                                                       m->n_out, g + m->off_b3);
+  } else {
+    mse_colsum_kernel<<<dim3(nb, S), cs_block, 0, m->stream>>>(
+        m->y, m->t, m->dy, rows, m->n_out, 1.0f / (float)ny, S > 1 ? m->colpart : g + m->off_b3,
+        m->partial);
+    if (S > 1)
+      colsum_final_kernel<<<(m->n_out + 255) / 256, 256, 0, m->stream>>>(m->colpart, S, m->n_out,
+                                                                         g + m->off_b3);
+  }
   MLP_HIP(hipGetLastError());
   // dW3 = dY^T h2 : (n_out x rows) . (rows x H)
   int rc = gemm(m, m->n_out, m->H, rows, m->dy, m->n_out, 0, 1, m->h2, m->H, 0, 1,
@@ -992,8 +1127,8 @@ int backward(gcm_mlp* m, int rows, double* loss) {
   colsum(m->d1, m->H, g + m->off_b1);
   MLP_HIP(hipGetLastError());
   if (loss) {
-    std::vector<float> part((size_t)nb * S);
-    MLP_HIP(hipMemcpyAsync(part.data(), m->partial, part.size() * sizeof(float),
+    std::vector<float> part(n_loss);
+    MLP_HIP(hipMemcpyAsync(part.data(), fused ? m->hsq : m->partial, part.size() * sizeof(float),
                            hipMemcpyDeviceToHost, m->stream));
     MLP_HIP(hipStreamSynchronize(m->stream));
     double s = 0;
@@ -1225,7 +1360,7 @@ int gcm_mlp_gradients(gcm_mlp* m, const float* x, const float* t, int rows, floa
   if (rc) return rc;
   MLP_HIP(hipMemcpyAsync(m->x, x, (size_t)rows * m->n_in * 4, hipMemcpyHostToDevice, m->stream));
   MLP_HIP(hipMemcpyAsync(m->t, t, (size_t)rows * m->n_out * 4, hipMemcpyHostToDevice, m->stream));
-  if ((rc = forward(m, rows))) return rc;
+  if ((rc = forward_train(m, rows, m->t, nullptr))) return rc;
   if ((rc = backward(m, rows, loss))) return rc;
   if (grad) {
     MLP_HIP(hipMemcpyAsync(grad, m->g, m->n_params * 4, hipMemcpyDeviceToHost, m->stream));
@@ -1280,9 +1415,12 @@ int gcm_mlp_train(gcm_mlp* m, int epochs, int batch, float lr, uint64_t seed,
     int nbatch = 0, nsampled = 0;
     for (int64_t r0 = 0; r0 + batch <= m->n_data; r0 += batch, ++nbatch) {
       const int32_t* idx = m->order + r0;
+      // the minibatch's input rows (and target rows, unless the loss head
+      // reads them from the training set itself)
+      const bool fused = head_fused(m, batch);
       gather_rows_kernel<<<dim3(std::min(batch, 4096), 4), 256, 0, m->stream>>>(
-          m->dx, m->dt, idx, batch, m->n_in, m->n_out, m->x, m->t);
-      if ((rc = forward(m, batch))) return rc;
+          m->dx, m->dt, idx, batch, m->n_in, fused ? 0 : m->n_out, m->x, m->t);
+      if ((rc = forward_train(m, batch, m->dt, idx))) return rc;
       // the loss is fetched (a synchronisation) on every 16th minibatch only
       const bool want = loss_out != nullptr && (nbatch % 16) == 0;
       double l = 0;
