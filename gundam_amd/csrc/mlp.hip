@@ -461,9 +461,34 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
 // Split-K epilogue: C = sum over s (in order) of ws[s][m][n], then the
 // GEMM's epilogue (bias, GELU keeping the pre-activation, or the GELU
 // derivative of the backward pass).
-// Grid (column blocks, row blocks), both strided.
+// Grid (column blocks, row blocks), both strided; with N a multiple of 4
+// each thread takes four adjacent columns with 16-byte loads (the
+// workspace is dense and 16-byte aligned), the columns counted in fours.
 __global__ void splitk_reduce_kernel(GemmArgs g, int splits) {
   const int64_t n = (int64_t)g.M * g.N;
+  if ((g.N & 3) == 0) {
+    const int n4 = g.N / 4;
+    for (int row = blockIdx.y; row < g.M; row += gridDim.y)
+      for (int c4 = blockIdx.x * blockDim.x + threadIdx.x; c4 < n4;
+           c4 += gridDim.x * blockDim.x) {
+        const int64_t i = (int64_t)row * g.N + 4 * c4;
+        float4 v = *reinterpret_cast<const float4*>(g.ws + i);
+#pragma unroll 8
+        for (int s = 1; s < splits; ++s) {
+          const float4 w = *reinterpret_cast<const float4*>(g.ws + s * n + i);
+          v.x += w.x;
+          v.y += w.y;
+          v.z += w.z;
+          v.w += w.w;
+        }
+        const int c = 4 * c4;
+        const float e[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+          finish_store(g, row, c + q, e[q], g.bias ? g.bias[c + q] : 0.0f);
+      }
+    return;
+  }
   for (int row = blockIdx.y; row < g.M; row += gridDim.y)
     for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.N; c += gridDim.x * blockDim.x) {
       const int64_t i = (int64_t)row * g.N + c;
@@ -475,8 +500,8 @@ __global__ void splitk_reduce_kernel(GemmArgs g, int splits) {
 }
 
 // d *= gelu'(z) elementwise (the backward pass through a GELU layer after an
-// unsplit GEMM: a wide launch, where the GEMM's epilogue would run this on
-// the few threads of a small output tiling).
+// unsplit GEMM with a small output tiling, whose epilogue would run this on
+// few threads).
 __global__ void gelu_backward_kernel(float* __restrict__ d, const float* __restrict__ z,
                                      int64_t n) {
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
@@ -992,9 +1017,12 @@ int gemm(gcm_mlp* m, int M, int N, int K, const void* A, int64_t lda, int a_kc, 
     g.hsq = head->hsq;
   }
   if (p.splits == 1) {
-    g.gz = nullptr;
+    // the GELU derivative in the epilogue when the output tiling is wide
+    // (>= 256 blocks); a small tiling leaves it to one wide elementwise pass
+    const bool gz_in_epilogue = gz && (int64_t)grid.x * grid.y >= 256;
+    if (!gz_in_epilogue) g.gz = nullptr;
     MLP_HIP(launch_gemm(p.big, sel, grid, stream, g));
-    if (gz) {
+    if (gz && !gz_in_epilogue) {
       gelu_backward_kernel<<<grid_for((int64_t)M * N), 256, 0, stream>>>(C, gz, (int64_t)M * N);
       MLP_HIP(hipGetLastError());
     }
@@ -1011,7 +1039,7 @@ int gemm(gcm_mlp* m, int M, int N, int K, const void* A, int64_t lda, int a_kc, 
     g.ws = m->ws;
   }
   MLP_HIP(launch_gemm(p.big, sel, grid, stream, g));
-  splitk_reduce_kernel<<<reduce_grid(M, N), 256, 0, stream>>>(g, p.splits);
+  splitk_reduce_kernel<<<reduce_grid(M, N % 4 == 0 ? N / 4 : N), 256, 0, stream>>>(g, p.splits);
   MLP_HIP(hipGetLastError());
   return GCM_OK;
 }
