@@ -240,15 +240,16 @@ __device__ __forceinline__ float gelu_grad(float z) {
 // The GEMM epilogue of element (row, col) of C: bias (bcol = bias[col],
 // loaded by the caller), GELU (the pre-activation kept in Z) or the GELU
 // derivative, then the store.
+// With gz, gzv is gz's element (row, col), loaded by the caller.
 __device__ __forceinline__ void finish_store(const GemmArgs& g, int row, int col, float v,
-                                             float bcol) {
+                                             float bcol, float gzv) {
   const int64_t o = (int64_t)row * g.ldc + col;
   if (g.bias) v += bcol;
   if (g.gelu) {
     g.Z[o] = v;
     v = gelu(v);
   }
-  if (g.gz) v *= gelu_grad(g.gz[o]);
+  if (g.gz) v *= gelu_grad(gzv);
   if (g.Cb) g.Cb[o] = (__bf16)v;
   else g.C[o] = v;
 }
@@ -447,11 +448,20 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
     for (int j = 0; j < NJ; ++j) {
       const int col = n0 + wn * (BN / 2) + j * 32 + lane % 32;
       const float bcol = (g.bias && col < g.N) ? g.bias[col] : 0.0f;  // one load per lane
+      // the GELU derivative's pre-activations, loaded unconditionally
+      // (indices clamped) so that all 16 are in flight together
+      float gzv[16];
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int row = min(m0 + wm * (BM / 2) + i * 32 + (r / 4) * 8 + (lane / 32) * 4 + r % 4,
+                            g.M - 1);
+        gzv[r] = g.gz ? g.gz[(int64_t)row * g.ldc + min(col, g.N - 1)] : 0.0f;
+      }
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         const int row = m0 + wm * (BM / 2) + i * 32 + (r / 4) * 8 + (lane / 32) * 4 + r % 4;
         if (row < g.M && col < g.N) {
-          if (splits == 1) finish_store(g, row, col, acc[i][j][r], bcol);
+          if (splits == 1) finish_store(g, row, col, acc[i][j][r], bcol, gzv[r]);
           else P[(int64_t)row * g.N + col] = acc[i][j][r];
         }
       }
@@ -485,7 +495,8 @@ __global__ void splitk_reduce_kernel(GemmArgs g, int splits) {
         const float e[4] = {v.x, v.y, v.z, v.w};
 #pragma unroll
         for (int q = 0; q < 4; ++q)
-          finish_store(g, row, c + q, e[q], g.bias ? g.bias[c + q] : 0.0f);
+          finish_store(g, row, c + q, e[q], g.bias ? g.bias[c + q] : 0.0f,
+                       g.gz ? g.gz[(int64_t)row * g.ldc + c + q] : 0.0f);
       }
     return;
   }
@@ -495,7 +506,8 @@ __global__ void splitk_reduce_kernel(GemmArgs g, int splits) {
       float v = g.ws[i];
 #pragma unroll 8
       for (int s = 1; s < splits; ++s) v += g.ws[s * n + i];
-      finish_store(g, row, c, v, g.bias ? g.bias[c] : 0.0f);
+      finish_store(g, row, c, v, g.bias ? g.bias[c] : 0.0f,
+                   g.gz ? g.gz[(int64_t)row * g.ldc + c] : 0.0f);
     }
 }
 
