@@ -46,8 +46,9 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
 // Block tile BM x BN (64 x 64, or 128 x 128 for products with plenty of
-// work) with a K step of 64; the four waves form a 2 x 2 grid, each owning a
-// (BM/2) x (BN/2) sub-tile of 32 x 32 accumulators.  Operand tiles are
+// work) with a K step of 64; the waves form a 2 x WN grid (WN = 2 for the
+// 64 x 64 tile, 4 for the 128 x 128 one), each owning a (BM/2) x (BN/WN)
+// sub-tile of 32 x 32 accumulators.  Operand tiles are
 // double-buffered in LDS as [row][k] BF16 with a padded row of 72 elements
 // (144 B, 16-B aligned for the 128-bit operand reads); the K step after next
 // is loaded into registers while the current one is multiplied, one barrier
@@ -113,12 +114,23 @@ struct GemmArgs {
 //      j < ROWS/64), k = 8 (t/32) .. + 7, read as 2-element vectors along
 //      the rows (one per k), stored transposed as one 16-byte LDS write per
 //      row.
-template <bool KC, bool BF, int ROWS>
+//  With NT = 512 threads the rows are spread over twice the threads: KC rows
+//  t/8 + 64 i, row-contiguous rows 2 (t%32) + e + 64 (2 j + t/256) with
+//  k = 8 ((t/32) % 8).
+template <bool KC, bool BF, int ROWS, int NT = 256>
 struct Stage {
   using E = typename std::conditional<BF, __bf16, float>::type;
-  static constexpr int NI = ROWS / 32;          // KC: rows per thread
-  static constexpr int NJ = ROWS / 64;          // row-contiguous: row pairs per thread
+  static constexpr int RPP = NT / 8;            // KC: rows per pass of the block
+  static constexpr int RB = NT / 256;           // row-contiguous: 64-row blocks per pass
+  static constexpr int NI = ROWS / RPP;         // KC: rows per thread
+  static constexpr int NJ = ROWS / (64 * RB);   // row-contiguous: row pairs per thread
   static constexpr int NS = KC ? NI : 2 * NJ;   // rows per thread
+  static_assert(NI >= 1 && NJ >= 1, "tile too small for the block");
+  __device__ __forceinline__ static int kc_row(int tid, int i) { return tid / 8 + RPP * i; }
+  __device__ __forceinline__ static int rc_row(int tid, int j) {
+    return 2 * (tid % 32) + 64 * (j * RB + tid / 256);
+  }
+  __device__ __forceinline__ static int rc_k(int tid) { return 8 * ((tid / 32) % 8); }
   // raw loaded values: converted to BF16 (FP32 operands) only when stored,
   // so that the loads stay in flight across the multiply of the previous step
   float f[NS][8];      // FP32: [row slot][k]
@@ -132,7 +144,7 @@ struct Stage {
     if constexpr (KC) {
 #pragma unroll
       for (int i = 0; i < NI; ++i) {
-        const int r = min(r0 + tid / 8 + 32 * i, R - 1);
+        const int r = min(r0 + kc_row(tid, i), R - 1);
         const E* p = base + (int64_t)r * ld + k0 + (tid % 8) * 8;
         if constexpr (BF) {
           hk[i] = *reinterpret_cast<const bf16x8*>(p);
@@ -146,8 +158,8 @@ struct Stage {
     } else {
 #pragma unroll
       for (int j = 0; j < NJ; ++j) {
-        const int r = min(r0 + 2 * (tid % 32) + 64 * j, R - 2);
-        const E* p = base + (int64_t)(k0 + 8 * (tid / 32)) * ld + r;
+        const int r = min(r0 + rc_row(tid, j), R - 2);
+        const E* p = base + (int64_t)(k0 + rc_k(tid)) * ld + r;
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
           if constexpr (BF) {
@@ -171,7 +183,7 @@ struct Stage {
     if constexpr (KC) {
 #pragma unroll
       for (int i = 0; i < NI; ++i) {
-        const int r = r0 + tid / 8 + 32 * i, k = k0 + (tid % 8) * 8;
+        const int r = r0 + kc_row(tid, i), k = k0 + (tid % 8) * 8;
         E v[8];
 #pragma unroll
         for (int j = 0; j < 8; ++j) {
@@ -188,7 +200,7 @@ struct Stage {
     } else {
 #pragma unroll
       for (int jj = 0; jj < NJ; ++jj) {
-        const int r = r0 + 2 * (tid % 32) + 64 * jj, k = k0 + 8 * (tid / 32);
+        const int r = r0 + rc_row(tid, jj), k = k0 + rc_k(tid);
         E v[8][2];
 #pragma unroll
         for (int q = 0; q < 8; ++q)
@@ -222,8 +234,8 @@ struct Stage {
           else v[j] = (__bf16)f[i][j];
         }
       }
-      const int r = KC ? tid / 8 + 32 * i : 2 * (tid % 32) + (i % 2) + 64 * (i / 2);
-      const int k = KC ? (tid % 8) * 8 : 8 * (tid / 32);
+      const int r = KC ? kc_row(tid, i) : rc_row(tid, i / 2) + (i % 2);
+      const int k = KC ? (tid % 8) * 8 : rc_k(tid);
       *reinterpret_cast<bf16x8*>(S + r * LDK + k) = v;
     }
   }
@@ -254,14 +266,15 @@ __device__ __forceinline__ void finish_store(const GemmArgs& g, int row, int col
   else g.C[o] = v;
 }
 
-// One K step (64) of the wave's (BM/2) x (BN/2) sub-tile from the LDS tiles.
-template <int BM, int BN>
+// One K step (64) of the wave's (BM/2) x (BN/WN) sub-tile from the LDS tiles
+// (the waves form a 2 x WN grid).
+template <int BM, int BN, int WN>
 __device__ __forceinline__ void mfma_step(const __bf16* As, const __bf16* Bs,
-                                          f32x16 (&acc)[BM / 64][BN / 64], int wm, int wn,
+                                          f32x16 (&acc)[BM / 64][BN / WN / 32], int wm, int wn,
                                           int lane) {
-  constexpr int MI = BM / 64, NJ = BN / 64;
+  constexpr int MI = BM / 64, NJ = BN / WN / 32;
   const __bf16* as = As + (wm * (BM / 2) + lane % 32) * LDK + (lane / 32) * 8;
-  const __bf16* bs = Bs + (wn * (BN / 2) + lane % 32) * LDK + (lane / 32) * 8;
+  const __bf16* bs = Bs + (wn * (BN / WN) + lane % 32) * LDK + (lane / 32) * 8;
 #pragma unroll
   for (int kk = 0; kk < TK; kk += 16) {
     bf16x8 a[MI], b[NJ];
@@ -284,21 +297,22 @@ constexpr int gemm_lds_bytes(int BM, int BN) { return 2 * (BM + BN) * LDK * 2; }
 // k in [z k_chunk, min(K, (z + 1) k_chunk)).  Aligned operands run the full
 // K steps pipelined with unchecked loads (edge tiles by clamping the rows);
 // the K tail and unaligned operands take checked, unpipelined steps.
-template <int BM, int BN, bool AKC, bool ABF, bool BKC, bool BBF>
-__global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
+template <int BM, int BN, bool AKC, bool ABF, bool BKC, bool BBF, int WN = 2>
+__global__ void __launch_bounds__(128 * WN) gemm_bf16_kernel(GemmArgs g) {
+  constexpr int NT = 128 * WN;  // threads: 2 x WN waves
   extern __shared__ __align__(16) unsigned char gemm_lds[];
   __bf16* const As[2] = {reinterpret_cast<__bf16*>(gemm_lds),
                          reinterpret_cast<__bf16*>(gemm_lds) + BM * LDK};
   __bf16* const Bs[2] = {reinterpret_cast<__bf16*>(gemm_lds) + 2 * BM * LDK,
                          reinterpret_cast<__bf16*>(gemm_lds) + 2 * BM * LDK + BN * LDK};
-  constexpr int MI = BM / 64, NJ = BN / 64;
-  using SA = Stage<AKC, ABF, BM>;
-  using SB = Stage<BKC, BBF, BN>;
+  constexpr int MI = BM / 64, NJ = BN / WN / 32;
+  using SA = Stage<AKC, ABF, BM, NT>;
+  using SB = Stage<BKC, BBF, BN, NT>;
   const auto* A = reinterpret_cast<const typename SA::E*>(g.A);
   const auto* B = reinterpret_cast<const typename SB::E*>(g.B);
   const int tid = threadIdx.x;
   const int lane = tid % 64, wave = tid / 64;
-  const int wm = wave / 2, wn = wave % 2;
+  const int wm = wave / WN, wn = wave % WN;
   // Workgroups are dispatched to the 8 XCDs round-robin in linear order; each
   // XCD gets a contiguous range of tiles, ordered so that the tiles sharing
   // the larger operand's tiles share that XCD's L2.
@@ -346,7 +360,7 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
       sa.load(A, g.lda, g.M, m0, kstep(st + 2), tid);
       sb.load(B, g.ldb, g.N, n0, kstep(st + 2), tid);
       __builtin_amdgcn_sched_barrier(0);  // issue the loads before the multiply
-      mfma_step<BM, BN>(As[0], Bs[0], acc, wm, wn, lane);
+      mfma_step<BM, BN, WN>(As[0], Bs[0], acc, wm, wn, lane);
       sa2.store(As[1], tid);
       sb2.store(Bs[1], tid);
       __syncthreads();
@@ -354,18 +368,18 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
       sa2.load(A, g.lda, g.M, m0, kstep(st + 3), tid);
       sb2.load(B, g.ldb, g.N, n0, kstep(st + 3), tid);
       __builtin_amdgcn_sched_barrier(0);
-      mfma_step<BM, BN>(As[1], Bs[1], acc, wm, wn, lane);
+      mfma_step<BM, BN, WN>(As[1], Bs[1], acc, wm, wn, lane);
       sa.store(As[0], tid);
       sb.store(Bs[0], tid);
       __syncthreads();
     }
     // one or two full steps left: LDS 0 holds step st, set 2 step st + 1
-    mfma_step<BM, BN>(As[0], Bs[0], acc, wm, wn, lane);
+    mfma_step<BM, BN, WN>(As[0], Bs[0], acc, wm, wn, lane);
     if (st + 1 < nfull) {
       sa2.store(As[1], tid);
       sb2.store(Bs[1], tid);
       __syncthreads();
-      mfma_step<BM, BN>(As[1], Bs[1], acc, wm, wn, lane);
+      mfma_step<BM, BN, WN>(As[1], Bs[1], acc, wm, wn, lane);
     }
     __syncthreads();
   }
@@ -376,7 +390,7 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
     sa.store(As[0], tid);
     sb.store(Bs[0], tid);
     __syncthreads();
-    mfma_step<BM, BN>(As[0], Bs[0], acc, wm, wn, lane);
+    mfma_step<BM, BN, WN>(As[0], Bs[0], acc, wm, wn, lane);
     __syncthreads();
   }
   const int splits = gridDim.z;
@@ -384,7 +398,7 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
     // the loss head: per lane, its rows of each of its columns in a fixed
     // order; lane l then adds lane l + 32 (the other rows of the same
     // columns), the two waves of a column half add in wave order through LDS
-    float* hs = reinterpret_cast<float*>(gemm_lds);  // [2][BN] column sums, [4] squared errors
+    float* hs = reinterpret_cast<float*>(gemm_lds);  // [2][BN] column sums, [2 WN] squared errors
     float csum[NJ];
 #pragma unroll
     for (int j = 0; j < NJ; ++j) csum[j] = 0.0f;
@@ -403,7 +417,7 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
       }
 #pragma unroll
       for (int j = 0; j < NJ; ++j) {
-        const int col = n0 + wn * (BN / 2) + j * 32 + lane % 32;
+        const int col = n0 + wn * (BN / WN) + j * 32 + lane % 32;
         const int cc = min(col, g.N - 1);
         const float bcol = g.bias[cc];
         float tv[16];
@@ -431,14 +445,17 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
     }
     if (lane < 32) {
 #pragma unroll
-      for (int j = 0; j < NJ; ++j) hs[wm * BN + wn * (BN / 2) + j * 32 + lane] = csum[j];
+      for (int j = 0; j < NJ; ++j) hs[wm * BN + wn * (BN / WN) + j * 32 + lane] = csum[j];
     }
     if (lane == 0) hs[2 * BN + wave] = sq;
     __syncthreads();
     for (int c = tid; c < BN; c += blockDim.x)
       if (n0 + c < g.N) g.hcol[(int64_t)by * g.N + n0 + c] = hs[c] + hs[BN + c];
-    if (tid == 0)
-      g.hsq[(int64_t)by * gx + bx] = ((hs[2 * BN] + hs[2 * BN + 1]) + hs[2 * BN + 2]) + hs[2 * BN + 3];
+    if (tid == 0) {
+      float q = hs[2 * BN];
+      for (int w = 1; w < 2 * WN; ++w) q += hs[2 * BN + w];
+      g.hsq[(int64_t)by * gx + bx] = q;
+    }
     return;
   }
   float* P = g.ws + (int64_t)bz * g.split_stride;
@@ -446,7 +463,7 @@ __global__ void __launch_bounds__(256) gemm_bf16_kernel(GemmArgs g) {
   for (int i = 0; i < MI; ++i)
 #pragma unroll
     for (int j = 0; j < NJ; ++j) {
-      const int col = n0 + wn * (BN / 2) + j * 32 + lane % 32;
+      const int col = n0 + wn * (BN / WN) + j * 32 + lane % 32;
       const float bcol = (g.bias && col < g.N) ? g.bias[col] : 0.0f;  // one load per lane
       // the GELU derivative's pre-activations, loaded unconditionally
       // (indices clamped) so that all 16 are in flight together
@@ -911,12 +928,12 @@ bool vec_ok(const void* p, int64_t ld, int kc, int bf16, int R) {
 
 // The kernel instance for a tile side (64 or 128) and operand layout
 // selector sel = A K-contiguous | A BF16 | B K-contiguous | B BF16 (bits 3..0).
-template <int T>
+template <int T, int WN = 2>
 const void* gemm_fn(int sel) {
   switch (sel) {
 #define GCM_GEMM_CASE(i) \
   case i:                \
-    return (const void*)gemm_bf16_kernel<T, T, ((i)&8) != 0, ((i)&4) != 0, ((i)&2) != 0, ((i)&1) != 0>;
+    return (const void*)gemm_bf16_kernel<T, T, ((i)&8) != 0, ((i)&4) != 0, ((i)&2) != 0, ((i)&1) != 0, WN>;
     GCM_GEMM_CASE(0) GCM_GEMM_CASE(1) GCM_GEMM_CASE(2) GCM_GEMM_CASE(3)
     GCM_GEMM_CASE(4) GCM_GEMM_CASE(5) GCM_GEMM_CASE(6) GCM_GEMM_CASE(7)
     GCM_GEMM_CASE(8) GCM_GEMM_CASE(9) GCM_GEMM_CASE(10) GCM_GEMM_CASE(11)
@@ -928,20 +945,21 @@ const void* gemm_fn(int sel) {
 
 // The large tile's LDS is past the default 64 KB: allow it (once per process).
 hipError_t gemm_set_lds_attrs() {
-  for (int sel = 0; sel < 16; ++sel) {
-    const hipError_t e = hipFuncSetAttribute(gemm_fn<kBig>(sel),
-                                             hipFuncAttributeMaxDynamicSharedMemorySize,
-                                             gemm_lds_bytes(kBig, kBig));
-    if (e != hipSuccess) return e;
-  }
+  for (int sel = 0; sel < 16; ++sel)
+    for (const void* fn : {gemm_fn<kBig>(sel), gemm_fn<kBig, 4>(sel)}) {
+      const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               gemm_lds_bytes(kBig, kBig));
+      if (e != hipSuccess) return e;
+    }
   return hipSuccess;
 }
 
-hipError_t launch_gemm(bool big, int sel, dim3 grid, hipStream_t stream, const GemmArgs& g) {
-  const void* fn = big ? gemm_fn<kBig>(sel) : gemm_fn<TM>(sel);
+hipError_t launch_gemm(bool big, int wn, int sel, dim3 grid, hipStream_t stream,
+                       const GemmArgs& g) {
+  const void* fn = big ? (wn == 4 ? gemm_fn<kBig, 4>(sel) : gemm_fn<kBig>(sel)) : gemm_fn<TM>(sel);
   GemmArgs a = g;
   void* args[] = {&a};
-  const hipError_t e = hipLaunchKernel(fn, grid, dim3(256), args,
+  const hipError_t e = hipLaunchKernel(fn, grid, dim3(128 * wn), args,
                                        big ? gemm_lds_bytes(kBig, kBig) : gemm_lds_bytes(TM, TN),
                                        stream);
   return e != hipSuccess ? e : hipGetLastError();
@@ -1017,6 +1035,11 @@ int gemm(gcm_mlp* m, int M, int N, int K, const void* A, int64_t lda, int a_kc, 
              C, Cb, ldc, nullptr, 0, p.chunk_steps * TK, bias, Z, Z != nullptr, gz};
   const int sel = (a_kc ? 8 : 0) | (a_bf16 ? 4 : 0) | (b_kc ? 2 : 0) | (b_bf16 ? 1 : 0);
   const dim3 grid((N + p.T - 1) / p.T, (M + p.T - 1) / p.T, p.splits);
+  // the 128 x 128 tile runs in blocks of eight waves (2 x 4, each 64 x 32):
+  // half the registers per wave of a 2 x 2 layout, so four waves per SIMD
+  // instead of two hide each other's waits (measured 10% per training step
+  // at 4096 rows); the 64 x 64 tile keeps four
+  const int wn = p.big ? 4 : 2;
   if (head) {  // the fused loss head (the caller checked that the launch is unsplit)
     if (p.splits != 1 || !bias) return fail(GCM_ERR_ARG, "gemm: loss head needs an unsplit launch with a bias");
     g.head = 1;
@@ -1033,7 +1056,7 @@ int gemm(gcm_mlp* m, int M, int N, int K, const void* A, int64_t lda, int a_kc, 
     // (>= 256 blocks); a small tiling leaves it to one wide elementwise pass
     const bool gz_in_epilogue = gz && (int64_t)grid.x * grid.y >= 256;
     if (!gz_in_epilogue) g.gz = nullptr;
-    MLP_HIP(launch_gemm(p.big, sel, grid, stream, g));
+    MLP_HIP(launch_gemm(p.big, wn, sel, grid, stream, g));
     if (gz && !gz_in_epilogue) {
       gelu_backward_kernel<<<grid_for((int64_t)M * N), 256, 0, stream>>>(C, gz, (int64_t)M * N);
       MLP_HIP(hipGetLastError());
@@ -1050,7 +1073,7 @@ int gemm(gcm_mlp* m, int M, int N, int K, const void* A, int64_t lda, int a_kc, 
     if (rc) return rc;
     g.ws = m->ws;
   }
-  MLP_HIP(launch_gemm(p.big, sel, grid, stream, g));
+  MLP_HIP(launch_gemm(p.big, wn, sel, grid, stream, g));
   splitk_reduce_kernel<<<reduce_grid(M, N % 4 == 0 ? N / 4 : N), 256, 0, stream>>>(g, p.splits);
   MLP_HIP(hipGetLastError());
   return GCM_OK;
