@@ -57,6 +57,11 @@ typedef float f32x16 __attribute__((ext_vector_type(16)));
 constexpr int TM = 64, TN = 64, TK = 64;
 constexpr int kWaveMlp = 64;
 constexpr int kBig = 128;  // the large tile's side
+// The 128 x 128 tile runs in blocks of eight waves (2 x 4, each 64 x 32):
+// half the registers per wave of a 2 x 2 layout, so four waves per SIMD
+// instead of two hide each other's waits (measured 10% per training step at
+// 4096 rows); the 64 x 64 tile runs in four (2 x 2, each 32 x 32).
+constexpr int kBigWN = 4;
 constexpr int LDK = TK + 8;
 constexpr int kMaxSplit = 16;
 constexpr int kMinSplitSteps = 16;  // split only long reductions (K >= 1024)
@@ -926,9 +931,10 @@ bool vec_ok(const void* p, int64_t ld, int kc, int bf16, int R) {
   return a % (2 * es) == 0 && ld % 2 == 0 && R % 2 == 0;
 }
 
-// The kernel instance for a tile side (64 or 128) and operand layout
-// selector sel = A K-contiguous | A BF16 | B K-contiguous | B BF16 (bits 3..0).
-template <int T, int WN = 2>
+// The kernel instance for a tile side (64 or 128), its waves along N (2 or
+// 4) and operand layout selector sel = A K-contiguous | A BF16 | B
+// K-contiguous | B BF16 (bits 3..0).
+template <int T, int WN>
 const void* gemm_fn(int sel) {
   switch (sel) {
 #define GCM_GEMM_CASE(i) \
@@ -945,21 +951,20 @@ const void* gemm_fn(int sel) {
 
 // The large tile's LDS is past the default 64 KB: allow it (once per process).
 hipError_t gemm_set_lds_attrs() {
-  for (int sel = 0; sel < 16; ++sel)
-    for (const void* fn : {gemm_fn<kBig>(sel), gemm_fn<kBig, 4>(sel)}) {
-      const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                               gemm_lds_bytes(kBig, kBig));
-      if (e != hipSuccess) return e;
-    }
+  for (int sel = 0; sel < 16; ++sel) {
+    const hipError_t e = hipFuncSetAttribute(gemm_fn<kBig, kBigWN>(sel),
+                                             hipFuncAttributeMaxDynamicSharedMemorySize,
+                                             gemm_lds_bytes(kBig, kBig));
+    if (e != hipSuccess) return e;
+  }
   return hipSuccess;
 }
 
-hipError_t launch_gemm(bool big, int wn, int sel, dim3 grid, hipStream_t stream,
-                       const GemmArgs& g) {
-  const void* fn = big ? (wn == 4 ? gemm_fn<kBig, 4>(sel) : gemm_fn<kBig>(sel)) : gemm_fn<TM>(sel);
+hipError_t launch_gemm(bool big, int sel, dim3 grid, hipStream_t stream, const GemmArgs& g) {
+  const void* fn = big ? gemm_fn<kBig, kBigWN>(sel) : gemm_fn<TM, 2>(sel);
   GemmArgs a = g;
   void* args[] = {&a};
-  const hipError_t e = hipLaunchKernel(fn, grid, dim3(128 * wn), args,
+  const hipError_t e = hipLaunchKernel(fn, grid, dim3(128 * (big ? kBigWN : 2)), args,
                                        big ? gemm_lds_bytes(kBig, kBig) : gemm_lds_bytes(TM, TN),
                                        stream);
   return e != hipSuccess ? e : hipGetLastError();
@@ -1035,11 +1040,6 @@ int gemm(gcm_mlp* m, int M, int N, int K, const void* A, int64_t lda, int a_kc, 
              C, Cb, ldc, nullptr, 0, p.chunk_steps * TK, bias, Z, Z != nullptr, gz};
   const int sel = (a_kc ? 8 : 0) | (a_bf16 ? 4 : 0) | (b_kc ? 2 : 0) | (b_bf16 ? 1 : 0);
   const dim3 grid((N + p.T - 1) / p.T, (M + p.T - 1) / p.T, p.splits);
-  // the 128 x 128 tile runs in blocks of eight waves (2 x 4, each 64 x 32):
-  // half the registers per wave of a 2 x 2 layout, so four waves per SIMD
-  // instead of two hide each other's waits (measured 10% per training step
-  // at 4096 rows); the 64 x 64 tile keeps four
-  const int wn = p.big ? 4 : 2;
   if (head) {  // the fused loss head (the caller checked that the launch is unsplit)
     if (p.splits != 1 || !bias) return fail(GCM_ERR_ARG, "gemm: loss head needs an unsplit launch with a bias");
     g.head = 1;
@@ -1056,7 +1056,7 @@ int gemm(gcm_mlp* m, int M, int N, int K, const void* A, int64_t lda, int a_kc, 
     // (>= 256 blocks); a small tiling leaves it to one wide elementwise pass
     const bool gz_in_epilogue = gz && (int64_t)grid.x * grid.y >= 256;
     if (!gz_in_epilogue) g.gz = nullptr;
-    MLP_HIP(launch_gemm(p.big, wn, sel, grid, stream, g));
+    MLP_HIP(launch_gemm(p.big, sel, grid, stream, g));
     if (gz && !gz_in_epilogue) {
       gelu_backward_kernel<<<grid_for((int64_t)M * N), 256, 0, stream>>>(C, gz, (int64_t)M * N);
       MLP_HIP(hipGetLastError());
@@ -1073,7 +1073,7 @@ int gemm(gcm_mlp* m, int M, int N, int K, const void* A, int64_t lda, int a_kc, 
     if (rc) return rc;
     g.ws = m->ws;
   }
-  MLP_HIP(launch_gemm(p.big, wn, sel, grid, stream, g));
+  MLP_HIP(launch_gemm(p.big, sel, grid, stream, g));
   splitk_reduce_kernel<<<reduce_grid(M, N % 4 == 0 ? N / 4 : N), 256, 0, stream>>>(g, p.splits);
   MLP_HIP(hipGetLastError());
   return GCM_OK;

@@ -57,6 +57,45 @@ def test_mlp_forward_and_gradients_match_fp64(shape, rows):
         assert _rel(a, b) < 3e-2, _rel(a, b)
 
 
+@pytest.mark.parametrize("shape,rows", [((300, 512, 6000), 4096), ((300, 512, 600), 2048)])
+def test_split_k_sums_per_block_and_repeatable(shape, rows):
+    """The split-K sums of the backward pass: every 64 x 64 block of every
+    weight gradient within 10% of the FP64 restatement (a partial missing
+    from a tile's sum, or read before it was written, leaves ~50% in its
+    blocks; BF16 operands leave 0.3-0.4%, measured; a norm over the whole
+    matrix would not tell a few such blocks apart), and four calls on the
+    same rows equal bit for bit.  At 4096 rows dW3 (split 3) and the output
+    layer's input gradient (split 4, GELU derivative in the sum's epilogue)
+    take 128 x 128 tiles, dW2 and dW1 64 x 64 tiles split 16 ways."""
+    print("split-K sums", shape, rows, flush=True)
+    n_in, h, n_out = shape
+    rng = np.random.default_rng(8)
+    mlp = MLP(n_in, h, n_out, seed=6)
+    flat = mlp.params().astype(np.float64)
+    params = [a.copy() for a in mlp.split(flat)]
+    for k in (1, 3, 5):
+        params[k][:] = 0.1 * rng.standard_normal(params[k].size)
+    mlp.set_params(np.concatenate([a.ravel() for a in params]).astype(np.float32))
+    x = rng.standard_normal((rows, n_in)).astype(np.float32)
+    t = rng.standard_normal((rows, n_out)).astype(np.float32)
+    _, g0 = mlp.gradients(x, t)
+    _, g_ref = mlp_oracle.loss_and_grad(params, x.astype(np.float64), t.astype(np.float64))
+    worst = 0.0
+    for a, b in zip(mlp.split(g0.astype(np.float64)), g_ref):
+        if a.ndim != 2:
+            continue
+        for r0 in range(0, a.shape[0], 64):
+            for c0 in range(0, a.shape[1], 64):
+                blk, ref = a[r0:r0 + 64, c0:c0 + 64], b[r0:r0 + 64, c0:c0 + 64]
+                worst = max(worst, _rel(blk, ref))
+    print(f"  worst 64 x 64 block relative error {worst:.3e}", flush=True)
+    assert worst < 0.1
+    for _ in range(3):
+        _, g = mlp.gradients(x, t)
+        np.testing.assert_array_equal(g, g0)
+    mlp.close()
+
+
 def test_mlp_predict_row_chunks_and_row_counts():
     """predict over 1, 33 and 8193 rows against the FP64 restatement.  With
     1100 inputs the first layer's reduction is split along K for the small
