@@ -638,31 +638,38 @@ mse_colsum_kernel(const float* __restrict__ y, const float* __restrict__ t,
   }
 }
 
-// One row of n floats (16-byte vectors when n is a multiple of 4: the rows
-// then start 16-byte aligned), by the threads of the blocks that share the
-// row (gridDim.y of them, block blockIdx.y of those).
-__device__ __forceinline__ void copy_row(const float* __restrict__ src, float* __restrict__ dst,
-                                         int n) {
-  const int t0 = blockIdx.y * blockDim.x + threadIdx.x, ts = gridDim.y * blockDim.x;
+// dst row r = src row idx[r], n floats per row, over the rows flattened:
+// one 16-byte vector per thread and step (one float when n % 4 != 0), grid
+// stride, so that every thread copies (rows * n < 2^30, checked by the host,
+// keeps the 32-bit index and its last stride in range).
+__device__ __forceinline__ void gather_flat(const float* __restrict__ src,
+                                            float* __restrict__ dst,
+                                            const int32_t* __restrict__ idx, int rows, int n) {
+  const int t0 = blockIdx.x * blockDim.x + threadIdx.x, ts = gridDim.x * blockDim.x;
   if ((n & 3) == 0) {
+    const int n4 = n / 4, total = rows * n4;
     const float4* s4 = reinterpret_cast<const float4*>(src);
     float4* d4 = reinterpret_cast<float4*>(dst);
-    for (int i = t0; i < n / 4; i += ts) d4[i] = s4[i];
+    for (int i = t0; i < total; i += ts) {
+      const int r = i / n4;
+      d4[i] = s4[(int64_t)idx[r] * n4 + (i - r * n4)];
+    }
   } else {
-    for (int i = t0; i < n; i += ts) dst[i] = src[i];
+    const int total = rows * n;
+    for (int i = t0; i < total; i += ts) {
+      const int r = i / n;
+      dst[i] = src[(int64_t)idx[r] * n + (i - r * n)];
+    }
   }
 }
 
-// Rows idx[0 .. rows) of the resident inputs and targets into the minibatch
-// buffers, in one launch: blocks (row, part of the row), rows strided.
+// The minibatch's rows of the training set: inputs (cx columns) and targets
+// (ct columns; none with ct = 0).
 __global__ void gather_rows_kernel(const float* __restrict__ sx, const float* __restrict__ st,
                                    const int32_t* __restrict__ idx, int rows, int cx, int ct,
                                    float* __restrict__ dx, float* __restrict__ dt) {
-  for (int r = blockIdx.x; r < rows; r += gridDim.x) {
-    const int64_t s = idx[r];
-    copy_row(sx + s * cx, dx + (int64_t)r * cx, cx);
-    copy_row(st + s * ct, dt + (int64_t)r * ct, ct);
-  }
+  gather_flat(sx, dx, idx, rows, cx);
+  if (ct) gather_flat(st, dt, idx, rows, ct);
 }
 
 // Where flat parameter i lives in the BF16 weight copy: W1 with its rows
@@ -1461,6 +1468,8 @@ int gcm_mlp_train(gcm_mlp* m, int epochs, int batch, float lr, uint64_t seed,
                   double* loss_out) {
   if (!m || epochs < 0 || batch < 1) return fail(GCM_ERR_ARG, "invalid arguments");
   if (!m->dx) return fail(GCM_ERR_STATE, "gcm_mlp_set_data first");
+  if ((int64_t)batch * std::max(m->n_in, m->n_out) >= (int64_t(1) << 30))
+    return fail(GCM_ERR_ARG, "minibatch too large (rows x columns >= 2^30)");
   MLP_HIP(hipSetDevice(m->device));
   int rc = m->ensure_batch(batch);
   if (rc) return rc;
@@ -1481,8 +1490,9 @@ int gcm_mlp_train(gcm_mlp* m, int epochs, int batch, float lr, uint64_t seed,
       // the minibatch's input rows (and target rows, unless the loss head
       // reads them from the training set itself)
       const bool fused = head_fused(m, batch);
-      gather_rows_kernel<<<dim3(std::min(batch, 4096), 4), 256, 0, m->stream>>>(
-          m->dx, m->dt, idx, batch, m->n_in, fused ? 0 : m->n_out, m->x, m->t);
+      const int ct = fused ? 0 : m->n_out;
+      gather_rows_kernel<<<grid_for((int64_t)batch * ((std::max(m->n_in, ct) + 3) / 4)), 256, 0,
+                           m->stream>>>(m->dx, m->dt, idx, batch, m->n_in, ct, m->x, m->t);
       if ((rc = forward_train(m, batch, m->dt, idx))) return rc;
       // the loss is fetched (a synchronisation) on every 16th minibatch only
       const bool want = loss_out != nullptr && (nbatch % 16) == 0;

@@ -121,6 +121,40 @@ def test_mlp_predict_row_chunks_and_row_counts():
     mlp.close()
 
 
+@pytest.mark.parametrize("shape", [(20, 64, 40), (21, 1024, 42)])
+def test_mlp_train_step_matches_gradients(shape):
+    """One epoch whose minibatch is the whole training set is one Adam step on
+    the whole set's gradient, whatever the shuffle: the parameters after it
+    against gradients() on the rows in their own order and Adam's first step
+    on the host.  The first step moves each parameter by about lr times the
+    sign of its gradient, so rows gathered wrongly, or inputs and targets from
+    different rows, move most parameters the wrong way; the two row orders
+    sum in different orders, so a few parameters whose gradient is near zero
+    may differ.  (20, 64, 40) reads the targets through the row order in the
+    output layer's loss head; (21, 1024, 42), whose output layer is split
+    along K, gathers them, and its row lengths are not multiples of 4."""
+    print("mlp train step", shape, flush=True)
+    n_in, h, n_out = shape
+    rows, lr = 256, 1e-3
+    rng = np.random.default_rng(12)
+    x = rng.standard_normal((rows, n_in)).astype(np.float32)
+    t = rng.standard_normal((rows, n_out)).astype(np.float32)
+    mlp = MLP(n_in, h, n_out, seed=3)
+    p0 = mlp.params().astype(np.float64)
+    _, g = mlp.gradients(x, t)
+    g = g.astype(np.float64)
+    mlp.set_data(x, t)
+    mlp.train(1, batch=rows, lr=lr, seed=7)
+    p1 = mlp.params().astype(np.float64)
+    b1, b2, eps = 0.9, 0.999, 1e-8
+    m, v = (1 - b1) * g, (1 - b2) * g * g
+    want = p0 - lr * (m / (1 - b1)) / (np.sqrt(v / (1 - b2)) + eps)
+    off = np.abs(p1 - want) > 0.1 * lr
+    print(f"  parameters off by more than lr/10: {off.sum()} of {off.size}", flush=True)
+    assert off.mean() < 0.01
+    mlp.close()
+
+
 def test_mlp_training_reduces_loss():
     print("mlp training", flush=True)
     rng = np.random.default_rng(4)
