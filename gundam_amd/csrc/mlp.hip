@@ -492,45 +492,42 @@ __global__ void __launch_bounds__(128 * WN) gemm_bf16_kernel(GemmArgs g) {
 
 // Split-K epilogue: C = sum over s (in order) of ws[s][m][n], then the
 // GEMM's epilogue (bias, GELU keeping the pre-activation, or the GELU
-// derivative of the backward pass).
-// Grid (column blocks, row blocks), both strided; with N a multiple of 4
-// each thread takes four adjacent columns with 16-byte loads (the
-// workspace is dense and 16-byte aligned), the columns counted in fours.
+// derivative of the backward pass).  The workspace is dense (ld N), so the
+// elements are taken flat, grid-stride: four adjacent columns per thread with
+// 16-byte loads when N is a multiple of 4 (the workspace and, with gz, the
+// dense pre-activations then 16-byte aligned), else one.  Every split launch
+// has fewer than 384 tiles of 128 x 128 (M N < 2^23): 32-bit indices.
 __global__ void splitk_reduce_kernel(GemmArgs g, int splits) {
-  const int64_t n = (int64_t)g.M * g.N;
+  const int n = g.M * g.N;
+  const int t0 = blockIdx.x * blockDim.x + threadIdx.x, ts = gridDim.x * blockDim.x;
   if ((g.N & 3) == 0) {
-    const int n4 = g.N / 4;
-    for (int row = blockIdx.y; row < g.M; row += gridDim.y)
-      for (int c4 = blockIdx.x * blockDim.x + threadIdx.x; c4 < n4;
-           c4 += gridDim.x * blockDim.x) {
-        const int64_t i = (int64_t)row * g.N + 4 * c4;
-        float4 v = *reinterpret_cast<const float4*>(g.ws + i);
+    for (int i = 4 * t0; i < n; i += 4 * ts) {
+      float4 v = *reinterpret_cast<const float4*>(g.ws + i);
 #pragma unroll 8
-        for (int s = 1; s < splits; ++s) {
-          const float4 w = *reinterpret_cast<const float4*>(g.ws + s * n + i);
-          v.x += w.x;
-          v.y += w.y;
-          v.z += w.z;
-          v.w += w.w;
-        }
-        const int c = 4 * c4;
-        const float e[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-        for (int q = 0; q < 4; ++q)
-          finish_store(g, row, c + q, e[q], g.bias ? g.bias[c + q] : 0.0f,
-                       g.gz ? g.gz[(int64_t)row * g.ldc + c + q] : 0.0f);
+      for (int s = 1; s < splits; ++s) {
+        const float4 w = *reinterpret_cast<const float4*>(g.ws + s * n + i);
+        v.x += w.x;
+        v.y += w.y;
+        v.z += w.z;
+        v.w += w.w;
       }
+      const int row = i / g.N, c = i - row * g.N;
+      const float4 z = g.gz ? *reinterpret_cast<const float4*>(g.gz + i)  // ldc == N with gz
+                            : float4{0.0f, 0.0f, 0.0f, 0.0f};
+      const float e[4] = {v.x, v.y, v.z, v.w}, ze[4] = {z.x, z.y, z.z, z.w};
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        finish_store(g, row, c + q, e[q], g.bias ? g.bias[c + q] : 0.0f, ze[q]);
+    }
     return;
   }
-  for (int row = blockIdx.y; row < g.M; row += gridDim.y)
-    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < g.N; c += gridDim.x * blockDim.x) {
-      const int64_t i = (int64_t)row * g.N + c;
-      float v = g.ws[i];
+  for (int i = t0; i < n; i += ts) {
+    float v = g.ws[i];
 #pragma unroll 8
-      for (int s = 1; s < splits; ++s) v += g.ws[s * n + i];
-      finish_store(g, row, c, v, g.bias ? g.bias[c] : 0.0f,
-                   g.gz ? g.gz[(int64_t)row * g.ldc + c] : 0.0f);
-    }
+    for (int s = 1; s < splits; ++s) v += g.ws[s * n + i];
+    const int row = i / g.N, c = i - row * g.N;
+    finish_store(g, row, c, v, g.bias ? g.bias[c] : 0.0f, g.gz ? g.gz[i] : 0.0f);
+  }
 }
 
 // d *= gelu'(z) elementwise (the backward pass through a GELU layer after an
@@ -541,10 +538,6 @@ __global__ void gelu_backward_kernel(float* __restrict__ d, const float* __restr
   for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x)
     d[i] *= gelu_grad(z[i]);
-}
-
-dim3 reduce_grid(int M, int N) {
-  return dim3((unsigned)std::min(32, (N + 255) / 256), (unsigned)std::min(M, 2048));
 }
 
 // Column sums over M rows in a fixed order: the rows are cut into S slices
@@ -585,9 +578,17 @@ colsum_kernel(const float* __restrict__ d, int M, int N, float* __restrict__ out
 // db[n] = sum over slices s (in order) of part[s][n].
 __global__ void colsum_final_kernel(const float* __restrict__ part, int S, int N,
                                     float* __restrict__ db) {
+  constexpr int U = 16;  // slices loaded together (indices clamped), added in order
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x) {
-    float t = part[c];
-    for (int s = 1; s < S; ++s) t += part[(size_t)s * N + c];
+    float t = 0.0f;
+    for (int s0 = 0; s0 < S; s0 += U) {
+      float v[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) v[u] = part[(size_t)min(s0 + u, S - 1) * N + c];
+#pragma unroll
+      for (int u = 0; u < U; ++u)
+        if (s0 + u < S) t = s0 + u == 0 ? v[u] : t + v[u];
+    }
     db[c] = t;
   }
 }
@@ -1081,7 +1082,7 @@ int gemm(gcm_mlp* m, int M, int N, int K, const void* A, int64_t lda, int a_kc, 
     g.ws = m->ws;
   }
   MLP_HIP(launch_gemm(p.big, sel, grid, stream, g));
-  splitk_reduce_kernel<<<reduce_grid(M, N % 4 == 0 ? N / 4 : N), 256, 0, stream>>>(g, p.splits);
+  splitk_reduce_kernel<<<grid_for(N % 4 == 0 ? mn / 4 : mn), 256, 0, stream>>>(g, p.splits);
   MLP_HIP(hipGetLastError());
   return GCM_OK;
 }
