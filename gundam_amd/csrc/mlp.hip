@@ -544,7 +544,8 @@ __global__ void gelu_backward_kernel(float* __restrict__ d, const float* __restr
 // (S = col_slices(M)), one block per (64 columns, slice); a block's 16 row
 // sub-slices sum their rows in order and are added in order; with S > 1 the
 // slice sums go to part[s][n] and colsum_final_kernel adds them in slice
-// order.
+// order.  One launch sums the columns of up to two M x N matrices: d[z] into
+// out[z] for blockIdx.z = z.
 constexpr int kColsumSlices = 16;
 constexpr int kMaxColSlices = 32;
 int col_slices(int M) { return std::max(1, std::min(kMaxColSlices, M / 256)); }
@@ -555,8 +556,15 @@ __device__ __forceinline__ void slice_rows(int M, int S, int s, int* r0, int* r1
   *r1 = (int)((int64_t)M * (s + 1) / S);
 }
 
+struct ColsumPair {
+  const float* d[2];
+  float* out[2];
+};
+
 __global__ void __launch_bounds__(64 * kColsumSlices)
-colsum_kernel(const float* __restrict__ d, int M, int N, float* __restrict__ out) {
+colsum_kernel(ColsumPair pr, int M, int N) {
+  const float* __restrict__ d = pr.d[blockIdx.z];
+  float* __restrict__ out = pr.out[blockIdx.z];
   __shared__ float part[kColsumSlices][65];
   const int c = blockIdx.x * 64 + threadIdx.x;
   int r0, r1;
@@ -576,8 +584,22 @@ colsum_kernel(const float* __restrict__ d, int M, int N, float* __restrict__ out
 }
 
 // db[n] = sum over slices s (in order) of part[s][n].
-__global__ void colsum_final_kernel(const float* __restrict__ part, int S, int N,
-                                    float* __restrict__ db) {
+// The final passes of up to three column sums in one launch: segment
+// blockIdx.y adds its S slice sums part[s][n] in slice order into db[n].
+struct ColsumFinal {
+  const float* part;
+  int S, N;
+  float* db;
+};
+struct ColsumFinals {
+  ColsumFinal f[3];
+};
+
+__global__ void colsum_final_kernel(ColsumFinals fs) {
+  const ColsumFinal sg = fs.f[blockIdx.y];
+  const float* __restrict__ part = sg.part;
+  float* __restrict__ db = sg.db;
+  const int S = sg.S, N = sg.N;
   constexpr int U = 16;  // slices loaded together (indices clamped), added in order
   for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < N; c += gridDim.x * blockDim.x) {
     float t = 0.0f;
@@ -1149,13 +1171,16 @@ int backward(gcm_mlp* m, int rows, double* loss) {
   const __bf16* w = m->pb;
   float* g = m->g;
   const dim3 cs_block(64, kColsumSlices);
-  // column sums of an M x N matrix into db: one pass, or slices and a final pass
-  auto colsum = [&](const float* d, int N, float* db) {
-    colsum_kernel<<<dim3((N + 63) / 64, S), cs_block, 0, m->stream>>>(d, rows, N,
-                                                                       S > 1 ? m->colpart : db);
-    if (S > 1)
-      colsum_final_kernel<<<(N + 255) / 256, 256, 0, m->stream>>>(m->colpart, S, N, db);
-  };
+  // The bias gradients are column sums of dY, dH2 and dH1, read only by the
+  // optimizer: dY's partial sums come with dY (the loss head's per-tile sums,
+  // or mse_colsum_kernel's per-slice sums), dH2's and dH1's are taken in one
+  // launch after the last GEMM, and one launch adds all partial sums into g.
+  // colpart holds dY's slice sums (unfused), then dH2's, then dH1's.
+  float* const part_y = m->colpart;
+  float* const part_2 = part_y + (size_t)S * m->n_out;
+  float* const part_1 = part_2 + (size_t)S * m->H;
+  ColsumFinals fin{};
+  int nfin = 0;
   // the loss partials: per (row tile, column tile) of the output layer when
   // fused, else per (row slice, column block) of mse_colsum_kernel
   const int head_tile = gemm_plan(rows, m->n_out, m->H).T;
@@ -1163,15 +1188,12 @@ int backward(gcm_mlp* m, int rows, double* loss) {
   const size_t n_loss = fused ? (size_t)head_rows * ((m->n_out + head_tile - 1) / head_tile)
                               : (size_t)nb * S;
   if (fused) {
-    colsum_final_kernel<<<(m->n_out + 255) / 256, 256, 0, m->stream>>>(m->hcol, head_rows,
-                                                                       m->n_out, g + m->off_b3);
+    fin.f[nfin++] = ColsumFinal{m->hcol, head_rows, m->n_out, g + m->off_b3};
   } else {
     mse_colsum_kernel<<<dim3(nb, S), cs_block, 0, m->stream>>>(
-        m->y, m->t, m->dy, rows, m->n_out, 1.0f / (float)ny, S > 1 ? m->colpart : g + m->off_b3,
+        m->y, m->t, m->dy, rows, m->n_out, 1.0f / (float)ny, S > 1 ? part_y : g + m->off_b3,
         m->partial);
-    if (S > 1)
-      colsum_final_kernel<<<(m->n_out + 255) / 256, 256, 0, m->stream>>>(m->colpart, S, m->n_out,
-                                                                         g + m->off_b3);
+    if (S > 1) fin.f[nfin++] = ColsumFinal{part_y, S, m->n_out, g + m->off_b3};
   }
   MLP_HIP(hipGetLastError());
   // dW3 = dY^T h2 : (n_out x rows) . (rows x H)
@@ -1186,7 +1208,6 @@ int backward(gcm_mlp* m, int rows, double* loss) {
   rc = gemm(m, m->H, m->H, rows, m->d2, m->H, 0, 0, m->h1, m->H, 0, 1, g + m->off_w2, m->H,
             nullptr, nullptr);
   if (rc) return rc;
-  colsum(m->d2, m->H, g + m->off_b2);
   // dH1 = (dH2 W2) * gelu'(z1)
   rc = gemm(m, rows, m->H, m->H, m->d2, m->H, 1, 0, w + m->bmap.bw2, m->H, 0, 1, m->d1, m->H,
             nullptr, nullptr, m->z1);
@@ -1195,7 +1216,18 @@ int backward(gcm_mlp* m, int rows, double* loss) {
   rc = gemm(m, m->H, m->n_in, rows, m->d1, m->H, 0, 0, m->x, m->n_in, 0, 0, g + m->off_w1,
             m->n_in, nullptr, nullptr);
   if (rc) return rc;
-  colsum(m->d1, m->H, g + m->off_b1);
+  const ColsumPair pr{{m->d2, m->d1},
+                      {S > 1 ? part_2 : g + m->off_b2, S > 1 ? part_1 : g + m->off_b1}};
+  colsum_kernel<<<dim3((m->H + 63) / 64, S, 2), cs_block, 0, m->stream>>>(pr, rows, m->H);
+  if (S > 1) {
+    fin.f[nfin++] = ColsumFinal{part_2, S, m->H, g + m->off_b2};
+    fin.f[nfin++] = ColsumFinal{part_1, S, m->H, g + m->off_b1};
+  }
+  if (nfin) {
+    int n_max = 0;
+    for (int i = 0; i < nfin; ++i) n_max = std::max(n_max, fin.f[i].N);
+    colsum_final_kernel<<<dim3((n_max + 255) / 256, nfin), 256, 0, m->stream>>>(fin);
+  }
   MLP_HIP(hipGetLastError());
   if (loss) {
     std::vector<float> part(n_loss);
@@ -1245,7 +1277,7 @@ int gcm_mlp_create(int device, int n_in, int n_hidden, int n_out, uint64_t seed,
   step(m->alloc(&m->m2, m->n_params));
   step(m->alloc(&m->pb, m->n_bf16));
   step(m->alloc(&m->partial, (size_t)(n_out + 63) / 64 * kMaxColSlices));  // mse_colsum blocks
-  step(m->alloc(&m->colpart, (size_t)kMaxColSlices * std::max(n_out, n_hidden)));
+  step(m->alloc(&m->colpart, (size_t)kMaxColSlices * (n_out + 2 * (size_t)n_hidden)));
   if (rc == GCM_OK && hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking) != hipSuccess)
     rc = fail(GCM_ERR_HIP, "stream creation failed");
   step(noalloc_scratch(device, &m->scratch));
