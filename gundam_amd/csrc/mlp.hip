@@ -492,28 +492,35 @@ __global__ void __launch_bounds__(128 * WN) gemm_bf16_kernel(GemmArgs g) {
 
 // Split-K epilogue: C = sum over s (in order) of ws[s][m][n], then the
 // GEMM's epilogue (bias, GELU keeping the pre-activation, or the GELU
-// derivative of the backward pass).  The workspace is dense (ld N), so the
-// elements are taken flat, grid-stride: four adjacent columns per thread with
-// 16-byte loads when N is a multiple of 4 (the workspace and, with gz, the
-// dense pre-activations then 16-byte aligned), else one.  Every split launch
-// has fewer than 384 tiles of 128 x 128 (M N < 2^23): 32-bit indices.
-__global__ void splitk_reduce_kernel(GemmArgs g, int splits) {
+// derivative of the backward pass), for NS splits (2 .. kMaxSplit, one
+// instance each, so that every split's load is unconditional and all are in
+// flight together).  The workspace is dense (ld N), so the elements are taken
+// flat, grid-stride: four adjacent columns per thread with 16-byte loads when
+// N is a multiple of 4 (the workspace and, with gz, the dense pre-activations
+// then 16-byte aligned), else one.  Every split launch has fewer than 384
+// tiles of 128 x 128 (M N < 2^23): 32-bit indices.
+template <int NS>
+__global__ void splitk_reduce_kernel(GemmArgs g) {
   const int n = g.M * g.N;
   const int t0 = blockIdx.x * blockDim.x + threadIdx.x, ts = gridDim.x * blockDim.x;
   if ((g.N & 3) == 0) {
+    // the pre-activations are loaded with the partials (ldc == N with gz;
+    // without gz a workspace vector, unused)
+    const float* zsrc = g.gz ? g.gz : g.ws;
     for (int i = 4 * t0; i < n; i += 4 * ts) {
-      float4 v = *reinterpret_cast<const float4*>(g.ws + i);
-#pragma unroll 8
-      for (int s = 1; s < splits; ++s) {
-        const float4 w = *reinterpret_cast<const float4*>(g.ws + s * n + i);
-        v.x += w.x;
-        v.y += w.y;
-        v.z += w.z;
-        v.w += w.w;
+      const float4 z = *reinterpret_cast<const float4*>(zsrc + i);
+      float4 w[NS];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) w[s] = *reinterpret_cast<const float4*>(g.ws + s * n + i);
+      float4 v = w[0];
+#pragma unroll
+      for (int s = 1; s < NS; ++s) {
+        v.x += w[s].x;
+        v.y += w[s].y;
+        v.z += w[s].z;
+        v.w += w[s].w;
       }
       const int row = i / g.N, c = i - row * g.N;
-      const float4 z = g.gz ? *reinterpret_cast<const float4*>(g.gz + i)  // ldc == N with gz
-                            : float4{0.0f, 0.0f, 0.0f, 0.0f};
       const float e[4] = {v.x, v.y, v.z, v.w}, ze[4] = {z.x, z.y, z.z, z.w};
 #pragma unroll
       for (int q = 0; q < 4; ++q)
@@ -522,13 +529,31 @@ __global__ void splitk_reduce_kernel(GemmArgs g, int splits) {
     return;
   }
   for (int i = t0; i < n; i += ts) {
-    float v = g.ws[i];
-#pragma unroll 8
-    for (int s = 1; s < splits; ++s) v += g.ws[s * n + i];
+    float w[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) w[s] = g.ws[s * n + i];
+    float v = w[0];
+#pragma unroll
+    for (int s = 1; s < NS; ++s) v += w[s];
     const int row = i / g.N, c = i - row * g.N;
     finish_store(g, row, c, v, g.bias ? g.bias[c] : 0.0f, g.gz ? g.gz[i] : 0.0f);
   }
 }
+
+const void* splitk_reduce_fn(int splits) {
+  switch (splits) {
+#define GCM_REDUCE_CASE(i) \
+  case i:                  \
+    return (const void*)splitk_reduce_kernel<i>;
+    GCM_REDUCE_CASE(2) GCM_REDUCE_CASE(3) GCM_REDUCE_CASE(4) GCM_REDUCE_CASE(5)
+    GCM_REDUCE_CASE(6) GCM_REDUCE_CASE(7) GCM_REDUCE_CASE(8) GCM_REDUCE_CASE(9)
+    GCM_REDUCE_CASE(10) GCM_REDUCE_CASE(11) GCM_REDUCE_CASE(12) GCM_REDUCE_CASE(13)
+    GCM_REDUCE_CASE(14) GCM_REDUCE_CASE(15) GCM_REDUCE_CASE(16)
+#undef GCM_REDUCE_CASE
+  }
+  return nullptr;
+}
+static_assert(kMaxSplit == 16, "one splitk_reduce_kernel instance per split count");
 
 // d *= gelu'(z) elementwise (the backward pass through a GELU layer after an
 // unsplit GEMM with a small output tiling, whose epilogue would run this on
@@ -1104,7 +1129,11 @@ int gemm(gcm_mlp* m, int M, int N, int K, const void* A, int64_t lda, int a_kc, 
     g.ws = m->ws;
   }
   MLP_HIP(launch_gemm(p.big, sel, grid, stream, g));
-  splitk_reduce_kernel<<<grid_for(N % 4 == 0 ? mn / 4 : mn), 256, 0, stream>>>(g, p.splits);
+  const void* red = splitk_reduce_fn(p.splits);
+  if (!red) return fail(GCM_ERR_ARG, "gemm: split count out of range");
+  void* args[] = {&g};
+  MLP_HIP(hipLaunchKernel(red, dim3(grid_for(N % 4 == 0 ? mn / 4 : mn)), dim3(256), args, 0,
+                          stream));
   MLP_HIP(hipGetLastError());
   return GCM_OK;
 }
