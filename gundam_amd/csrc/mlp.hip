@@ -686,6 +686,23 @@ mse_colsum_kernel(const float* __restrict__ y, const float* __restrict__ t,
   }
 }
 
+// acc[0] += the sum of part[0 .. n) in FP64, in a fixed order (each thread
+// its elements i = t mod 256 in order, then a fixed tree over the threads):
+// one block.
+__global__ void __launch_bounds__(256) loss_sum_kernel(const float* __restrict__ part, int n,
+                                                       double* __restrict__ acc) {
+  __shared__ double sm[256];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += 256) v += part[i];
+  sm[threadIdx.x] = v;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) sm[threadIdx.x] += sm[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) acc[0] += sm[0];
+}
+
 // dst row r = src row idx[r], n floats per row, over the rows flattened:
 // one 16-byte vector per thread and step (one float when n % 4 != 0), grid
 // stride, so that every thread copies (rows * n < 2^30, checked by the host,
@@ -917,10 +934,17 @@ struct gcm_mlp {
   // split-K partial sums
   float* ws = nullptr;
   size_t ws_cap = 0;
-  // resident training set and the device copy of an epoch's shuffled order
+  // resident training set; the shuffled row order of two consecutive epochs
+  // (order: 2 n_data on the device; order_host: the same in pinned host
+  // memory, each half's copy marked done by order_ev[half]); the sampled
+  // minibatch losses summed on the device, one double per epoch
   float *dx = nullptr, *dt = nullptr;
   int32_t* order = nullptr;
+  int32_t* order_host = nullptr;
+  hipEvent_t order_ev[2] = {nullptr, nullptr};
   int64_t n_data = 0;
+  double* loss_dev = nullptr;
+  int loss_cap = 0;
   // likelihood head: mean, scale, data, ratio (n_out each) and per-row chi2
   double* head = nullptr;
   double* chi2 = nullptr;
@@ -1191,8 +1215,10 @@ int forward_train(gcm_mlp* m, int rows, const float* tgt, const int32_t* tidx) {
 
 // Loss and gradients into m->g for the rows in m->x / m->t, after
 // forward_train (with the loss head fused, dY and its partial sums are
-// already there).
-int backward(gcm_mlp* m, int rows, double* loss) {
+// already there).  The loss (the mean squared error) is returned in *loss
+// (a wait for the stream), or its sum of squared errors is added to
+// *loss_acc on the device.
+int backward(gcm_mlp* m, int rows, double* loss, double* loss_acc = nullptr) {
   const bool fused = head_fused(m, rows);
   const int64_t ny = (int64_t)rows * m->n_out;
   const int nb = (m->n_out + 63) / 64;  // mse_colsum column blocks
@@ -1258,6 +1284,10 @@ int backward(gcm_mlp* m, int rows, double* loss) {
     colsum_final_kernel<<<dim3((n_max + 255) / 256, nfin), 256, 0, m->stream>>>(fin);
   }
   MLP_HIP(hipGetLastError());
+  if (loss_acc) {  // the sum of squared errors added on the device, no wait
+    loss_sum_kernel<<<1, 256, 0, m->stream>>>(fused ? m->hsq : m->partial, (int)n_loss, loss_acc);
+    MLP_HIP(hipGetLastError());
+  }
   if (loss) {
     std::vector<float> part(n_loss);
     MLP_HIP(hipMemcpyAsync(part.data(), fused ? m->hsq : m->partial, part.size() * sizeof(float),
@@ -1343,6 +1373,9 @@ int gcm_mlp_destroy(gcm_mlp* m) {
   (void)hipSetDevice(m->device);
   if (m->stream) (void)hipStreamSynchronize(m->stream);
   for (void* q : m->allocs) (void)hipFree(q);
+  if (m->order_host) (void)hipHostFree(m->order_host);
+  for (hipEvent_t e : m->order_ev)
+    if (e) (void)hipEventDestroy(e);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
   return GCM_OK;
@@ -1514,9 +1547,22 @@ int gcm_mlp_set_data(gcm_mlp* m, const float* x, const float* t, int64_t rows) {
     m->dx = m->dt = nullptr;
     m->order = nullptr;
   }
+  if (m->order_host) {
+    (void)hipHostFree(m->order_host);
+    m->order_host = nullptr;
+  }
   int rc = m->alloc(&m->dx, (size_t)rows * m->n_in);
   if (rc == GCM_OK) rc = m->alloc(&m->dt, (size_t)rows * m->n_out);
-  if (rc == GCM_OK) rc = m->alloc(&m->order, (size_t)rows);
+  if (rc == GCM_OK) rc = m->alloc(&m->order, 2 * (size_t)rows);
+  if (rc == GCM_OK && hipHostMalloc((void**)&m->order_host, 2 * (size_t)rows * sizeof(int32_t),
+                                    hipHostMallocDefault) != hipSuccess) {
+    m->order_host = nullptr;
+    rc = fail(GCM_ERR_NOMEM, "hipHostMalloc failed (mlp)");
+  }
+  for (int h = 0; h < 2 && rc == GCM_OK; ++h)
+    if (!m->order_ev[h] &&
+        hipEventCreateWithFlags(&m->order_ev[h], hipEventDisableTiming) != hipSuccess)
+      rc = fail(GCM_ERR_HIP, "event creation failed (mlp)");
   if (rc) return rc;
   MLP_HIP(hipMemcpy(m->dx, x, (size_t)rows * m->n_in * 4, hipMemcpyHostToDevice));
   MLP_HIP(hipMemcpy(m->dt, t, (size_t)rows * m->n_out * 4, hipMemcpyHostToDevice));
@@ -1539,16 +1585,41 @@ int gcm_mlp_train(gcm_mlp* m, int epochs, int batch, float lr, uint64_t seed,
   for (int64_t i = 0; i < m->n_data; ++i) order[i] = (int32_t)i;
   std::mt19937_64 rng(seed);
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-  for (int ep = 0; ep < epochs; ++ep) {
-    // the previous epoch's copy of `order` has completed before it is reshuffled
+  const int64_t n = m->n_data;
+  const int nb = (int)(n / batch);     // minibatches per epoch
+  const int n_sampled = (nb + 15) / 16;  // of which the loss is taken: every 16th
+  if (loss_out && epochs > m->loss_cap) {
     MLP_HIP(hipStreamSynchronize(m->stream));
+    if (m->loss_dev) {
+      (void)hipFree(m->loss_dev);
+      m->allocs.erase(std::remove(m->allocs.begin(), m->allocs.end(), (void*)m->loss_dev),
+                      m->allocs.end());
+      m->loss_dev = nullptr;
+    }
+    m->loss_cap = 0;
+    if ((rc = m->alloc(&m->loss_dev, (size_t)epochs))) return rc;
+    m->loss_cap = epochs;
+  }
+  if (loss_out && epochs > 0)
+    MLP_HIP(hipMemsetAsync(m->loss_dev, 0, (size_t)epochs * sizeof(double), m->stream));
+  // Epoch ep's order: the next shuffle of the running permutation, in half
+  // ep % 2 of the device and pinned host buffers.  It is shuffled and queued
+  // while the device still runs the previous epoch; the half's host copy is
+  // rewritten only once its copy two epochs back has completed.
+  auto stage = [&](int ep) -> int {
+    const int h = ep & 1;
+    MLP_HIP(hipEventSynchronize(m->order_ev[h]));
     std::shuffle(order.begin(), order.end(), rng);
-    MLP_HIP(hipMemcpyAsync(m->order, order.data(), (size_t)m->n_data * 4, hipMemcpyHostToDevice,
-                           m->stream));
-    double acc = 0;
-    int nbatch = 0, nsampled = 0;
-    for (int64_t r0 = 0; r0 + batch <= m->n_data; r0 += batch, ++nbatch) {
-      const int32_t* idx = m->order + r0;
+    std::copy(order.begin(), order.end(), m->order_host + h * n);
+    MLP_HIP(hipMemcpyAsync(m->order + h * n, m->order_host + h * n, (size_t)n * 4,
+                           hipMemcpyHostToDevice, m->stream));
+    MLP_HIP(hipEventRecord(m->order_ev[h], m->stream));
+    return GCM_OK;
+  };
+  if (epochs > 0 && (rc = stage(0))) return rc;
+  for (int ep = 0; ep < epochs; ++ep) {
+    for (int nbatch = 0; nbatch < nb; ++nbatch) {
+      const int32_t* idx = m->order + (ep & 1) * n + (int64_t)nbatch * batch;
       // the minibatch's input rows (and target rows, unless the loss head
       // reads them from the training set itself)
       const bool fused = head_fused(m, batch);
@@ -1556,14 +1627,9 @@ int gcm_mlp_train(gcm_mlp* m, int epochs, int batch, float lr, uint64_t seed,
       gather_rows_kernel<<<grid_for((int64_t)batch * ((std::max(m->n_in, ct) + 3) / 4)), 256, 0,
                            m->stream>>>(m->dx, m->dt, idx, batch, m->n_in, ct, m->x, m->t);
       if ((rc = forward_train(m, batch, m->dt, idx))) return rc;
-      // the loss is fetched (a synchronisation) on every 16th minibatch only
+      // the loss of every 16th minibatch, summed on the device
       const bool want = loss_out != nullptr && (nbatch % 16) == 0;
-      double l = 0;
-      if ((rc = backward(m, batch, want ? &l : nullptr))) return rc;
-      if (want) {
-        acc += l;
-        ++nsampled;
-      }
+      if ((rc = backward(m, batch, nullptr, want ? m->loss_dev + ep : nullptr))) return rc;
       m->adam_t++;
       const float c1 = 1.0f - std::pow(b1, (float)m->adam_t);
       const float c2 = 1.0f - std::pow(b2, (float)m->adam_t);
@@ -1571,7 +1637,15 @@ int gcm_mlp_train(gcm_mlp* m, int epochs, int batch, float lr, uint64_t seed,
                                                                 m->n_params, lr, b1, b2, eps, c1, c2);
       MLP_HIP(hipGetLastError());
     }
-    if (loss_out) loss_out[ep] = acc / std::max(1, nsampled);
+    if (ep + 1 < epochs && (rc = stage(ep + 1))) return rc;
+  }
+  if (loss_out && epochs > 0) {
+    std::vector<double> acc(epochs);
+    MLP_HIP(hipMemcpyAsync(acc.data(), m->loss_dev, (size_t)epochs * sizeof(double),
+                           hipMemcpyDeviceToHost, m->stream));
+    MLP_HIP(hipStreamSynchronize(m->stream));
+    const double per = (double)batch * m->n_out * std::max(1, n_sampled);
+    for (int ep = 0; ep < epochs; ++ep) loss_out[ep] = nb > 0 ? acc[ep] / per : 0.0;
   }
   MLP_HIP(hipStreamSynchronize(m->stream));
   return GCM_OK;
