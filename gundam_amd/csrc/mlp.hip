@@ -106,6 +106,16 @@ struct GemmArgs {
   float scale;
   float* hcol;
   float* hsq;
+  // Adam fused into the epilogue (adam set; no bias, GELU or gz; C dense,
+  // ldc == N): output element o = row N + col is the gradient of parameter
+  // ap[o], whose moments am1[o], am2[o] and BF16 copy apb[o] are updated in
+  // place with it (adam_one), and nothing is stored to C.
+  int adam;
+  float* ap;
+  float* am1;
+  float* am2;
+  __bf16* apb;
+  float lr, b1, b2, eps, c1, c2;
 };
 
 // One ROWS x 64 (k) operand tile (ROWS = 64 or 128) held in registers
@@ -245,6 +255,17 @@ struct Stage {
     }
   }
 };
+
+// One Adam step (Kingma & Ba 2015) of parameter p with gradient gi and
+// moments m1, m2 (updated in place); c1, c2 are the bias corrections.
+__device__ __forceinline__ float adam_one(float p, float gi, float& m1, float& m2, float lr,
+                                          float b1, float b2, float eps, float c1, float c2) {
+  const float a = b1 * m1 + (1.0f - b1) * gi;
+  const float b = b2 * m2 + (1.0f - b2) * gi * gi;
+  m1 = a;
+  m2 = b;
+  return p - lr * (a / c1) / (sqrtf(b / c2) + eps);
+}
 
 __device__ __forceinline__ float gelu(float z) {
   return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f));
@@ -463,6 +484,50 @@ __global__ void __launch_bounds__(128 * WN) gemm_bf16_kernel(GemmArgs g) {
     }
     return;
   }
+  // Adam on the parameters whose gradients the tile holds, in the one
+  // operand layout that produces them (weight gradients dY^T h: both
+  // operands row-contiguous BF16): values and moments of eight rows loaded
+  // unconditionally (indices clamped), in flight together, ahead of the
+  // bounds-checked updates
+  if constexpr (!AKC && ABF && !BKC && BBF) {
+    if (g.adam && splits == 1) {
+#pragma unroll
+      for (int i = 0; i < MI; ++i)
+#pragma unroll
+        for (int j = 0; j < NJ; ++j) {
+          const int col = n0 + wn * (BN / WN) + j * 32 + lane % 32;
+          const int cc = min(col, g.N - 1);
+#pragma unroll
+          for (int h = 0; h < 16; h += 8) {
+            float pv[8], av[8], bv[8];
+#pragma unroll
+            for (int r = h; r < h + 8; ++r) {
+              const int row = min(m0 + wm * (BM / 2) + i * 32 + (r / 4) * 8 + (lane / 32) * 4 +
+                                      r % 4,
+                                  g.M - 1);
+              const int64_t o = (int64_t)row * g.N + cc;
+              pv[r - h] = g.ap[o];
+              av[r - h] = g.am1[o];
+              bv[r - h] = g.am2[o];
+            }
+#pragma unroll
+            for (int r = h; r < h + 8; ++r) {
+              const int row = m0 + wm * (BM / 2) + i * 32 + (r / 4) * 8 + (lane / 32) * 4 + r % 4;
+              if (row < g.M && col < g.N) {
+                const int64_t o = (int64_t)row * g.N + col;
+                const float w = adam_one(pv[r - h], acc[i][j][r], av[r - h], bv[r - h], g.lr,
+                                         g.b1, g.b2, g.eps, g.c1, g.c2);
+                g.ap[o] = w;
+                g.am1[o] = av[r - h];
+                g.am2[o] = bv[r - h];
+                g.apb[o] = (__bf16)w;
+              }
+            }
+          }
+        }
+      return;
+    }
+  }
   float* P = g.ws + (int64_t)bz * g.split_stride;
 #pragma unroll
   for (int i = 0; i < MI; ++i)
@@ -507,8 +572,15 @@ __global__ void splitk_reduce_kernel(GemmArgs g) {
     // the pre-activations are loaded with the partials (ldc == N with gz;
     // without gz a workspace vector, unused)
     const float* zsrc = g.gz ? g.gz : g.ws;
+    // with Adam, the parameters and moments likewise (ldc == N)
+    const float* psrc = g.adam ? g.ap : g.ws;
+    const float* asrc = g.adam ? g.am1 : g.ws;
+    const float* bsrc = g.adam ? g.am2 : g.ws;
     for (int i = 4 * t0; i < n; i += 4 * ts) {
       const float4 z = *reinterpret_cast<const float4*>(zsrc + i);
+      const float4 pq = *reinterpret_cast<const float4*>(psrc + i);
+      float4 aq = *reinterpret_cast<const float4*>(asrc + i);
+      float4 bq = *reinterpret_cast<const float4*>(bsrc + i);
       float4 w[NS];
 #pragma unroll
       for (int s = 0; s < NS; ++s) w[s] = *reinterpret_cast<const float4*>(g.ws + s * n + i);
@@ -519,6 +591,21 @@ __global__ void splitk_reduce_kernel(GemmArgs g) {
         v.y += w[s].y;
         v.z += w[s].z;
         v.w += w[s].w;
+      }
+      if (g.adam) {
+        float4 u;
+        u.x = adam_one(pq.x, v.x, aq.x, bq.x, g.lr, g.b1, g.b2, g.eps, g.c1, g.c2);
+        u.y = adam_one(pq.y, v.y, aq.y, bq.y, g.lr, g.b1, g.b2, g.eps, g.c1, g.c2);
+        u.z = adam_one(pq.z, v.z, aq.z, bq.z, g.lr, g.b1, g.b2, g.eps, g.c1, g.c2);
+        u.w = adam_one(pq.w, v.w, aq.w, bq.w, g.lr, g.b1, g.b2, g.eps, g.c1, g.c2);
+        *reinterpret_cast<float4*>(g.ap + i) = u;
+        *reinterpret_cast<float4*>(g.am1 + i) = aq;
+        *reinterpret_cast<float4*>(g.am2 + i) = bq;
+        g.apb[i] = (__bf16)u.x;
+        g.apb[i + 1] = (__bf16)u.y;
+        g.apb[i + 2] = (__bf16)u.z;
+        g.apb[i + 3] = (__bf16)u.w;
+        continue;
       }
       const int row = i / g.N, c = i - row * g.N;
       const float e[4] = {v.x, v.y, v.z, v.w}, ze[4] = {z.x, z.y, z.z, z.w};
@@ -535,6 +622,15 @@ __global__ void splitk_reduce_kernel(GemmArgs g) {
     float v = w[0];
 #pragma unroll
     for (int s = 1; s < NS; ++s) v += w[s];
+    if (g.adam) {
+      float a = g.am1[i], b = g.am2[i];
+      const float u = adam_one(g.ap[i], v, a, b, g.lr, g.b1, g.b2, g.eps, g.c1, g.c2);
+      g.ap[i] = u;
+      g.am1[i] = a;
+      g.am2[i] = b;
+      g.apb[i] = (__bf16)u;
+      continue;
+    }
     const int row = i / g.N, c = i - row * g.N;
     finish_store(g, row, c, v, g.bias ? g.bias[c] : 0.0f, g.gz ? g.gz[i] : 0.0f);
   }
@@ -752,28 +848,23 @@ __device__ __forceinline__ int64_t bf16_index(const Bf16Map& q, int64_t i) {
   return -1;
 }
 
-// Adam (Kingma & Ba 2015) on the FP32 master copy; refreshes the BF16 copy.
-// Four parameters per thread with 16-byte loads and stores (the arrays are
-// 16-byte aligned); their BF16 copies as one 8-byte store when they are
-// consecutive and aligned there, else one by one.
-__device__ __forceinline__ float adam_one(float p, float gi, float& m1, float& m2, float lr,
-                                          float b1, float b2, float eps, float c1, float c2) {
-  const float a = b1 * m1 + (1.0f - b1) * gi;
-  const float b = b2 * m2 + (1.0f - b2) * gi * gi;
-  m1 = a;
-  m2 = b;
-  return p - lr * (a / c1) / (sqrtf(b / c2) + eps);
-}
+// Adam on the FP32 master copy; refreshes the BF16 copy.  Four parameters
+// per thread with 16-byte loads and stores (the arrays are 16-byte aligned);
+// their BF16 copies as one 8-byte store when they are consecutive and aligned
+// there, else one by one.  Parameters [gap0, gap0 + gap) are skipped (updated
+// by the epilogue that produced their gradient); gap0 and gap are multiples
+// of 4.
 
 __global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g,
                             float* __restrict__ m1, float* __restrict__ m2,
-                            __bf16* __restrict__ pb, Bf16Map q, int64_t n, float lr, float b1,
-                            float b2, float eps, float c1, float c2) {
+                            __bf16* __restrict__ pb, Bf16Map q, int64_t n, int64_t gap0,
+                            int64_t gap, float lr, float b1, float b2, float eps, float c1,
+                            float c2) {
   typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
-  const int64_t n4 = n / 4;
+  const int64_t n4 = (n - gap) / 4;  // over the parameters outside the gap
   for (int64_t v = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; v < n4;
        v += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t i = 4 * v;
+    const int64_t i = 4 * v < gap0 ? 4 * v : 4 * v + gap;
     const float4 gv = *reinterpret_cast<const float4*>(g + i);
     float4 a = *reinterpret_cast<const float4*>(m1 + i);
     float4 b = *reinterpret_cast<const float4*>(m2 + i);
@@ -802,16 +893,17 @@ __global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ g,
       }
     }
   }
-  // the n % 4 last parameters
-  for (int64_t i = 4 * n4 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-       i += (int64_t)gridDim.x * blockDim.x) {
+  // the last (n - gap) % 4 parameters
+  for (int64_t j = 4 * n4 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < n - gap;
+       j += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t i = j < gap0 ? j : j + gap;
     float a = m1[i], b = m2[i];
     const float w = adam_one(p[i], g[i], a, b, lr, b1, b2, eps, c1, c2);
     m1[i] = a;
     m2[i] = b;
     p[i] = w;
-    const int64_t j = bf16_index(q, i);
-    if (j >= 0) pb[j] = (__bf16)w;
+    const int64_t k = bf16_index(q, i);
+    if (k >= 0) pb[k] = (__bf16)w;
   }
 }
 
@@ -1105,12 +1197,26 @@ struct Exec {
   size_t ws_cap;  // floats
 };
 
-// C is FP32 (C) or BF16 (Cb, with C null).
+// One Adam step applied to a dense M x N block of parameters by the GEMM
+// that produces their gradient (GemmArgs::adam): the block's FP32 values,
+// moments and BF16 copy, the step's scalars.
+struct AdamStep {
+  float *p, *m1, *m2;
+  __bf16* pb;
+  float lr, b1, b2, eps, c1, c2;
+};
+
+// C is FP32 (C) or BF16 (Cb, with C null); with adam the product is the
+// gradient of adam's parameters, which are updated instead of storing C.
 int gemm(gcm_mlp* m, int M, int N, int K, const void* A, int64_t lda, int a_kc, int a_bf16,
          const void* B, int64_t ldb, int b_kc, int b_bf16, float* C, int64_t ldc,
          const float* bias, float* Z, const float* gz = nullptr, const Exec* ex = nullptr,
-         __bf16* Cb = nullptr, const GemmArgs* head = nullptr) {
+         __bf16* Cb = nullptr, const GemmArgs* head = nullptr, const AdamStep* adam = nullptr) {
   if (M <= 0 || N <= 0) return GCM_OK;
+  if (adam && (bias || Z || gz || Cb || head || ex || ldc != N || a_kc || !a_bf16 || b_kc ||
+               !b_bf16))
+    return fail(GCM_ERR_ARG,
+                "gemm: the Adam epilogue needs a plain dense output of row-contiguous BF16 operands");
   if (gz && (ldc != N || Cb)) return fail(GCM_ERR_ARG, "gemm: GELU backward needs a dense FP32 output");
   const hipStream_t stream = ex ? ex->stream : m->stream;
   const GemmPlan p = gemm_plan(M, N, K);
@@ -1129,6 +1235,19 @@ int gemm(gcm_mlp* m, int M, int N, int K, const void* A, int64_t lda, int a_kc, 
     g.scale = head->scale;
     g.hcol = head->hcol;
     g.hsq = head->hsq;
+  }
+  if (adam) {
+    g.adam = 1;
+    g.ap = adam->p;
+    g.am1 = adam->m1;
+    g.am2 = adam->m2;
+    g.apb = adam->pb;
+    g.lr = adam->lr;
+    g.b1 = adam->b1;
+    g.b2 = adam->b2;
+    g.eps = adam->eps;
+    g.c1 = adam->c1;
+    g.c2 = adam->c2;
   }
   if (p.splits == 1) {
     // the GELU derivative in the epilogue when the output tiling is wide
@@ -1217,8 +1336,10 @@ int forward_train(gcm_mlp* m, int rows, const float* tgt, const int32_t* tidx) {
 // forward_train (with the loss head fused, dY and its partial sums are
 // already there).  The loss (the mean squared error) is returned in *loss
 // (a wait for the stream), or its sum of squared errors is added to
-// *loss_acc on the device.
-int backward(gcm_mlp* m, int rows, double* loss, double* loss_acc = nullptr) {
+// *loss_acc on the device.  With w3_adam the output layer's weights W3 take
+// their Adam step in the dW3 product (their gradient is not stored in g).
+int backward(gcm_mlp* m, int rows, double* loss, double* loss_acc = nullptr,
+             const AdamStep* w3_adam = nullptr) {
   const bool fused = head_fused(m, rows);
   const int64_t ny = (int64_t)rows * m->n_out;
   const int nb = (m->n_out + 63) / 64;  // mse_colsum column blocks
@@ -1251,13 +1372,22 @@ int backward(gcm_mlp* m, int rows, double* loss, double* loss_acc = nullptr) {
     if (S > 1) fin.f[nfin++] = ColsumFinal{part_y, S, m->n_out, g + m->off_b3};
   }
   MLP_HIP(hipGetLastError());
-  // dW3 = dY^T h2 : (n_out x rows) . (rows x H)
-  int rc = gemm(m, m->n_out, m->H, rows, m->dy, m->n_out, 0, 1, m->h2, m->H, 0, 1,
-                g + m->off_w3, m->H, nullptr, nullptr);
-  if (rc) return rc;
+  // dW3 = dY^T h2 : (n_out x rows) . (rows x H); with w3_adam W3 takes its
+  // Adam step in this product's epilogue instead of storing dW3, after dH2
+  // has read W3
+  auto dw3 = [&] {
+    return gemm(m, m->n_out, m->H, rows, m->dy, m->n_out, 0, 1, m->h2, m->H, 0, 1,
+                g + m->off_w3, m->H, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                w3_adam);
+  };
   // dH2 = (dY W3) * gelu'(z2) : (rows x n_out) . (n_out x H)
-  rc = gemm(m, rows, m->H, m->n_out, m->dy, m->n_out, 1, 1, w + m->bmap.bw3, m->H, 0, 1, m->d2,
-            m->H, nullptr, nullptr, m->z2);
+  auto dh2 = [&] {
+    return gemm(m, rows, m->H, m->n_out, m->dy, m->n_out, 1, 1, w + m->bmap.bw3, m->H, 0, 1,
+                m->d2, m->H, nullptr, nullptr, m->z2);
+  };
+  int rc = w3_adam ? dh2() : dw3();
+  if (rc) return rc;
+  rc = w3_adam ? dw3() : dh2();
   if (rc) return rc;
   // dW2 = dH2^T h1
   rc = gemm(m, m->H, m->H, rows, m->d2, m->H, 0, 0, m->h1, m->H, 0, 1, g + m->off_w2, m->H,
@@ -1587,6 +1717,9 @@ int gcm_mlp_train(gcm_mlp* m, int epochs, int batch, float lr, uint64_t seed,
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   const int64_t n = m->n_data;
   const int nb = (int)(n / batch);     // minibatches per epoch
+  // W3 takes its Adam step in the epilogue of its gradient's product when its
+  // range is aligned for the 16-byte paths (adam_kernel skips it)
+  const bool w3_fused = m->off_w3 % 4 == 0 && (m->off_b3 - m->off_w3) % 4 == 0;
   const int n_sampled = (nb + 15) / 16;  // of which the loss is taken: every 16th
   if (loss_out && epochs > m->loss_cap) {
     MLP_HIP(hipStreamSynchronize(m->stream));
@@ -1629,12 +1762,19 @@ int gcm_mlp_train(gcm_mlp* m, int epochs, int batch, float lr, uint64_t seed,
       if ((rc = forward_train(m, batch, m->dt, idx))) return rc;
       // the loss of every 16th minibatch, summed on the device
       const bool want = loss_out != nullptr && (nbatch % 16) == 0;
-      if ((rc = backward(m, batch, nullptr, want ? m->loss_dev + ep : nullptr))) return rc;
       m->adam_t++;
       const float c1 = 1.0f - std::pow(b1, (float)m->adam_t);
       const float c2 = 1.0f - std::pow(b2, (float)m->adam_t);
-      adam_kernel<<<grid_for((m->n_params + 3) / 4), 256, 0, m->stream>>>(m->p, m->g, m->m1, m->m2, m->pb, m->bmap,
-                                                                m->n_params, lr, b1, b2, eps, c1, c2);
+      const AdamStep st{m->p + m->off_w3, m->m1 + m->off_w3, m->m2 + m->off_w3,
+                        m->pb + m->bmap.bw3, lr, b1, b2, eps, c1, c2};
+      if ((rc = backward(m, batch, nullptr, want ? m->loss_dev + ep : nullptr,
+                         w3_fused ? &st : nullptr)))
+        return rc;
+      // Adam on the parameters outside W3 (all of them without the fused step)
+      const int64_t gap = w3_fused ? m->off_b3 - m->off_w3 : 0;
+      adam_kernel<<<grid_for((m->n_params - gap + 3) / 4), 256, 0, m->stream>>>(
+          m->p, m->g, m->m1, m->m2, m->pb, m->bmap, m->n_params, w3_fused ? m->off_w3 : 0, gap,
+          lr, b1, b2, eps, c1, c2);
       MLP_HIP(hipGetLastError());
     }
     if (ep + 1 < epochs && (rc = stage(ep + 1))) return rc;
