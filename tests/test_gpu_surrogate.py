@@ -121,8 +121,9 @@ def test_mlp_predict_row_chunks_and_row_counts():
     mlp.close()
 
 
-@pytest.mark.parametrize("shape", [(20, 64, 40), (21, 1024, 42)])
-def test_mlp_train_step_matches_gradients(shape):
+@pytest.mark.parametrize("shape,rows", [((20, 64, 40), 256), ((21, 1024, 42), 256),
+                                        ((19, 63, 37), 256), ((8, 512, 12000), 960)])
+def test_mlp_train_step_matches_gradients(shape, rows):
     """One epoch whose minibatch is the whole training set is one Adam step on
     the whole set's gradient, whatever the shuffle: the parameters after it
     against gradients() on the rows in their own order and Adam's first step
@@ -132,10 +133,15 @@ def test_mlp_train_step_matches_gradients(shape):
     sum in different orders, so a few parameters whose gradient is near zero
     may differ.  (20, 64, 40) reads the targets through the row order in the
     output layer's loss head; (21, 1024, 42), whose output layer is split
-    along K, gathers them, and its row lengths are not multiples of 4."""
-    print("mlp train step", shape, flush=True)
+    along K, gathers them, and its row lengths are not multiples of 4.  The
+    output layer's weights take their Adam step in the epilogue of their
+    gradient's product where their range is 16-byte aligned: in a 64 x 64
+    tile's epilogue here, in a 128 x 128 tile's at (8, 512, 12000) over 960
+    rows (376 tiles, 15 K steps: not split), and in the Adam launch at
+    (19, 63, 37), whose range is not aligned."""
+    print("mlp train step", shape, rows, flush=True)
     n_in, h, n_out = shape
-    rows, lr = 256, 1e-3
+    lr = 1e-3
     rng = np.random.default_rng(12)
     x = rng.standard_normal((rows, n_in)).astype(np.float32)
     t = rng.standard_normal((rows, n_out)).astype(np.float32)
