@@ -31,6 +31,12 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 MFMA_BF16_PEAK_TFLOPS = 2500.0  # MI355X dense BF16 MFMA, published
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64, published
+# FP64 operations of one 64-point launch on the 100k-event set's points
+# (SQ_INSTS_VALU_ADD_F64 and _MUL_F64 x 64 lanes, _FMA_F64 x 128, read per
+# dispatch by rocprofv3 on the same points; the count does not depend on the
+# launch shape)
+C3_FP64_FLOP_PER_LAUNCH = 1.179e9
 
 
 def log(msg):
@@ -222,13 +228,34 @@ def batched_workload3(args, log):
         cm.llh()
         lat.append(time.perf_counter() - t0)
     med = float(np.median(lat))
+    # the K-point kernel's own time (HIP events around it, the same launches
+    # without the graph) and its FP64 rate
+    cm.enable_graph(False)
+    cm.enable_timing(True)
+    for _ in range(reps):
+        cm.set_parameters(pts)
+        cm.fill()
+        cm.llh()
+    tm = cm.timing()
+    cm.enable_timing(False)
+    kern_ms = tm["fill_ms"] / max(1, tm["count"])
+    tflops = C3_FP64_FLOP_PER_LAUNCH / (kern_ms * 1e-3) / 1e12
     log(f"batched C3 (100k events, K=64, graph): {K/med:.0f} evals/s, "
-        f"{med*1e3:.3f} ms/batch, launch {cm.launch_config()}")
+        f"{med*1e3:.3f} ms/batch, launch {cm.launch_config()}; K-point kernel "
+        f"{kern_ms*1e3:.1f} us, {tflops:.2f} FP64 TFLOP/s "
+        f"({tflops / FP64_PEAK_TFLOPS:.1%} of peak)")
     return {"workload": "100k events in 4 samples (50k/30k/15k/5k) x 400 bins, 20 "
                         "parameters, 4 monotonic splines + 1 norm per event, BB OA2021",
             "points_per_launch": K, "evals_per_s": K / med,
             "latency_ms_per_batch": med * 1e3, "hipgraph": True,
-            "launch": cm.launch_config()}
+            "launch": cm.launch_config(),
+            "kernel_ms": kern_ms,
+            "fp64_flop_per_launch": C3_FP64_FLOP_PER_LAUNCH,
+            "fp64_flop_source": "SQ_INSTS_VALU_{ADD,MUL}_F64 x 64 + SQ_INSTS_VALU_FMA_F64 x "
+                                "128 per launch on these 64 points "
+                                "(profiles/fill_points_kernel_workload3_k64.md)",
+            "fp64_tflops": tflops, "fp64_peak_tflops": FP64_PEAK_TFLOPS,
+            "fp64_frac": tflops / FP64_PEAK_TFLOPS}
 
 
 def surrogate_da(args, cm, wl, data, err, log, device_tables=False):
