@@ -108,13 +108,14 @@ struct GemmArgs {
   float* hsq;
   // Adam fused into the epilogue (adam set; no bias, GELU or gz; C dense,
   // ldc == N): output element o = row N + col is the gradient of parameter
-  // ap[o], whose moments am1[o], am2[o] and BF16 copy apb[o] are updated in
-  // place with it (adam_one), and nothing is stored to C.
+  // ap[o], whose moments am1[o], am2[o] and BF16 copy apb[row ldpb + col]
+  // are updated in place with it (adam_one), and nothing is stored to C.
   int adam;
   float* ap;
   float* am1;
   float* am2;
   __bf16* apb;
+  int64_t ldpb;
   float lr, b1, b2, eps, c1, c2;
 };
 
@@ -484,12 +485,12 @@ __global__ void __launch_bounds__(128 * WN) gemm_bf16_kernel(GemmArgs g) {
     }
     return;
   }
-  // Adam on the parameters whose gradients the tile holds, in the one
-  // operand layout that produces them (weight gradients dY^T h: both
-  // operands row-contiguous BF16): values and moments of eight rows loaded
+  // Adam on the parameters whose gradients the tile holds, in the operand
+  // layouts that produce them (weight gradients dY^T h: both operands
+  // row-contiguous): values and moments of eight rows loaded
   // unconditionally (indices clamped), in flight together, ahead of the
   // bounds-checked updates
-  if constexpr (!AKC && ABF && !BKC && BBF) {
+  if constexpr (!AKC && !BKC) {
     if (g.adam && splits == 1) {
 #pragma unroll
       for (int i = 0; i < MI; ++i)
@@ -520,7 +521,7 @@ __global__ void __launch_bounds__(128 * WN) gemm_bf16_kernel(GemmArgs g) {
                 g.ap[o] = w;
                 g.am1[o] = av[r - h];
                 g.am2[o] = bv[r - h];
-                g.apb[o] = (__bf16)w;
+                g.apb[(int64_t)row * g.ldpb + col] = (__bf16)w;
               }
             }
           }
@@ -601,10 +602,12 @@ __global__ void splitk_reduce_kernel(GemmArgs g) {
         *reinterpret_cast<float4*>(g.ap + i) = u;
         *reinterpret_cast<float4*>(g.am1 + i) = aq;
         *reinterpret_cast<float4*>(g.am2 + i) = bq;
-        g.apb[i] = (__bf16)u.x;
-        g.apb[i + 1] = (__bf16)u.y;
-        g.apb[i + 2] = (__bf16)u.z;
-        g.apb[i + 3] = (__bf16)u.w;
+        const int row = i / g.N;
+        __bf16* pb = g.apb + (int64_t)row * g.ldpb + (i - row * g.N);
+        pb[0] = (__bf16)u.x;
+        pb[1] = (__bf16)u.y;
+        pb[2] = (__bf16)u.z;
+        pb[3] = (__bf16)u.w;
         continue;
       }
       const int row = i / g.N, c = i - row * g.N;
@@ -628,7 +631,8 @@ __global__ void splitk_reduce_kernel(GemmArgs g) {
       g.ap[i] = u;
       g.am1[i] = a;
       g.am2[i] = b;
-      g.apb[i] = (__bf16)u;
+      const int row = i / g.N;
+      g.apb[(int64_t)row * g.ldpb + (i - row * g.N)] = (__bf16)u;
       continue;
     }
     const int row = i / g.N, c = i - row * g.N;
@@ -677,10 +681,34 @@ __device__ __forceinline__ void slice_rows(int M, int S, int s, int* r0, int* r1
   *r1 = (int)((int64_t)M * (s + 1) / S);
 }
 
+// With ap[z] set (a single slice, so that the sum is the gradient), the
+// sums are the gradients of parameters ap[z][n] (biases), which take their
+// Adam step in place (moments am1, am2; the step's scalars st) instead of
+// being stored.
+struct AdamScalars {
+  float lr, b1, b2, eps, c1, c2;
+};
 struct ColsumPair {
   const float* d[2];
   float* out[2];
+  float* ap[2];
+  float* am1[2];
+  float* am2[2];
+  AdamScalars st;
 };
+
+// Parameter i's Adam step with gradient gi, or out[i] = gi without ap.
+__device__ __forceinline__ void store_or_step(float* out, float* ap, float* am1, float* am2,
+                                              const AdamScalars& st, int64_t i, float gi) {
+  if (ap) {
+    float a = am1[i], b = am2[i];
+    ap[i] = adam_one(ap[i], gi, a, b, st.lr, st.b1, st.b2, st.eps, st.c1, st.c2);
+    am1[i] = a;
+    am2[i] = b;
+  } else {
+    out[i] = gi;
+  }
+}
 
 __global__ void __launch_bounds__(64 * kColsumSlices)
 colsum_kernel(ColsumPair pr, int M, int N) {
@@ -700,20 +728,24 @@ colsum_kernel(ColsumPair pr, int M, int N) {
   if (threadIdx.y == 0 && c < N) {
     float t = 0.0f;
     for (int i = 0; i < kColsumSlices; ++i) t += part[i][threadIdx.x];
-    out[(size_t)blockIdx.y * N + c] = t;
+    const int z = blockIdx.z;
+    store_or_step(out, pr.ap[z], pr.am1[z], pr.am2[z], pr.st, (int64_t)blockIdx.y * N + c, t);
   }
 }
 
-// db[n] = sum over slices s (in order) of part[s][n].
 // The final passes of up to three column sums in one launch: segment
-// blockIdx.y adds its S slice sums part[s][n] in slice order into db[n].
+// blockIdx.y adds its S slice sums part[s][n] in slice order into db[n], or
+// with ap set takes the Adam step of parameters ap[n] with them (as
+// ColsumPair).
 struct ColsumFinal {
   const float* part;
   int S, N;
   float* db;
+  float *ap = nullptr, *am1 = nullptr, *am2 = nullptr;
 };
 struct ColsumFinals {
   ColsumFinal f[3];
+  AdamScalars st;
 };
 
 __global__ void colsum_final_kernel(ColsumFinals fs) {
@@ -732,7 +764,7 @@ __global__ void colsum_final_kernel(ColsumFinals fs) {
       for (int u = 0; u < U; ++u)
         if (s0 + u < S) t = s0 + u == 0 ? v[u] : t + v[u];
     }
-    db[c] = t;
+    store_or_step(db, sg.ap, sg.am1, sg.am2, fs.st, c, t);
   }
 }
 
@@ -1202,7 +1234,8 @@ struct Exec {
 // moments and BF16 copy, the step's scalars.
 struct AdamStep {
   float *p, *m1, *m2;
-  __bf16* pb;
+  __bf16* pb;  // BF16 copy, row pitch ldpb (null for biases: none)
+  int64_t ldpb;
   float lr, b1, b2, eps, c1, c2;
 };
 
@@ -1213,10 +1246,9 @@ int gemm(gcm_mlp* m, int M, int N, int K, const void* A, int64_t lda, int a_kc, 
          const float* bias, float* Z, const float* gz = nullptr, const Exec* ex = nullptr,
          __bf16* Cb = nullptr, const GemmArgs* head = nullptr, const AdamStep* adam = nullptr) {
   if (M <= 0 || N <= 0) return GCM_OK;
-  if (adam && (bias || Z || gz || Cb || head || ex || ldc != N || a_kc || !a_bf16 || b_kc ||
-               !b_bf16))
+  if (adam && (bias || Z || gz || Cb || head || ex || ldc != N || a_kc || b_kc))
     return fail(GCM_ERR_ARG,
-                "gemm: the Adam epilogue needs a plain dense output of row-contiguous BF16 operands");
+                "gemm: the Adam epilogue needs a plain dense output of row-contiguous operands");
   if (gz && (ldc != N || Cb)) return fail(GCM_ERR_ARG, "gemm: GELU backward needs a dense FP32 output");
   const hipStream_t stream = ex ? ex->stream : m->stream;
   const GemmPlan p = gemm_plan(M, N, K);
@@ -1242,6 +1274,7 @@ int gemm(gcm_mlp* m, int M, int N, int K, const void* A, int64_t lda, int a_kc, 
     g.am1 = adam->m1;
     g.am2 = adam->m2;
     g.apb = adam->pb;
+    g.ldpb = adam->ldpb;
     g.lr = adam->lr;
     g.b1 = adam->b1;
     g.b2 = adam->b2;
@@ -1338,9 +1371,31 @@ int forward_train(gcm_mlp* m, int rows, const float* tgt, const int32_t* tidx) {
 // (a wait for the stream), or its sum of squared errors is added to
 // *loss_acc on the device.  With w3_adam the output layer's weights W3 take
 // their Adam step in the dW3 product (their gradient is not stored in g).
+// One training step's Adam update fused into the backward pass: W3 always;
+// with all also W2, W1 and the biases, so that no parameter is left for the
+// Adam launch (the loss head must be fused).
+struct FusedAdam {
+  AdamScalars st;
+  bool all;
+};
+
 int backward(gcm_mlp* m, int rows, double* loss, double* loss_acc = nullptr,
-             const AdamStep* w3_adam = nullptr) {
+             const FusedAdam* fa = nullptr) {
   const bool fused = head_fused(m, rows);
+  if (fa && fa->all && !fused) return fail(GCM_ERR_ARG, "backward: fused Adam needs the loss head");
+  auto step_of = [&](int64_t off, int64_t bf_off, int64_t ldpb) {
+    return AdamStep{m->p + off, m->m1 + off, m->m2 + off, m->pb + bf_off, ldpb, fa->st.lr,
+                    fa->st.b1, fa->st.b2, fa->st.eps, fa->st.c1, fa->st.c2};
+  };
+  AdamStep s3{}, s2{}, s1{};
+  if (fa) s3 = step_of(m->off_w3, m->bmap.bw3, m->H);
+  if (fa && fa->all) {
+    s2 = step_of(m->off_w2, m->bmap.bw2, m->H);
+    s1 = step_of(m->off_w1, 0, m->bmap.n_in_pad);
+  }
+  const AdamStep* w3_adam = fa ? &s3 : nullptr;
+  const AdamStep* w2_adam = fa && fa->all ? &s2 : nullptr;
+  const AdamStep* w1_adam = fa && fa->all ? &s1 : nullptr;
   const int64_t ny = (int64_t)rows * m->n_out;
   const int nb = (m->n_out + 63) / 64;  // mse_colsum column blocks
   const int S = col_slices(rows);        // row slices: one partial per (slice, block)
@@ -1363,8 +1418,19 @@ int backward(gcm_mlp* m, int rows, double* loss, double* loss_acc = nullptr,
   const int head_rows = (rows + head_tile - 1) / head_tile;
   const size_t n_loss = fused ? (size_t)head_rows * ((m->n_out + head_tile - 1) / head_tile)
                               : (size_t)nb * S;
+  if (fa) fin.st = fa->st;
+  // with every step fused, each bias takes its step where its sum completes
+  auto bias_final = [&](const float* part, int S_, int N, int64_t off) {
+    ColsumFinal f{part, S_, N, g + off};
+    if (fa && fa->all) {
+      f.ap = m->p + off;
+      f.am1 = m->m1 + off;
+      f.am2 = m->m2 + off;
+    }
+    return f;
+  };
   if (fused) {
-    fin.f[nfin++] = ColsumFinal{m->hcol, head_rows, m->n_out, g + m->off_b3};
+    fin.f[nfin++] = bias_final(m->hcol, head_rows, m->n_out, m->off_b3);
   } else {
     mse_colsum_kernel<<<dim3(nb, S), cs_block, 0, m->stream>>>(
         m->y, m->t, m->dy, rows, m->n_out, 1.0f / (float)ny, S > 1 ? part_y : g + m->off_b3,
@@ -1389,24 +1455,36 @@ int backward(gcm_mlp* m, int rows, double* loss, double* loss_acc = nullptr,
   if (rc) return rc;
   rc = w3_adam ? dw3() : dh2();
   if (rc) return rc;
-  // dW2 = dH2^T h1
-  rc = gemm(m, m->H, m->H, rows, m->d2, m->H, 0, 0, m->h1, m->H, 0, 1, g + m->off_w2, m->H,
-            nullptr, nullptr);
-  if (rc) return rc;
+  // dW2 = dH2^T h1 (with w2_adam W2 takes its Adam step here, after dH1 has
+  // read W2)
+  auto dw2 = [&] {
+    return gemm(m, m->H, m->H, rows, m->d2, m->H, 0, 0, m->h1, m->H, 0, 1, g + m->off_w2, m->H,
+                nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, w2_adam);
+  };
   // dH1 = (dH2 W2) * gelu'(z1)
-  rc = gemm(m, rows, m->H, m->H, m->d2, m->H, 1, 0, w + m->bmap.bw2, m->H, 0, 1, m->d1, m->H,
-            nullptr, nullptr, m->z1);
+  auto dh1 = [&] {
+    return gemm(m, rows, m->H, m->H, m->d2, m->H, 1, 0, w + m->bmap.bw2, m->H, 0, 1, m->d1, m->H,
+                nullptr, nullptr, m->z1);
+  };
+  rc = w2_adam ? dh1() : dw2();
+  if (rc) return rc;
+  rc = w2_adam ? dw2() : dh1();
   if (rc) return rc;
   // dW1 = dH1^T x
   rc = gemm(m, m->H, m->n_in, rows, m->d1, m->H, 0, 0, m->x, m->n_in, 0, 0, g + m->off_w1,
-            m->n_in, nullptr, nullptr);
+            m->n_in, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, w1_adam);
   if (rc) return rc;
+  const bool bias_step = fa && fa->all && S == 1;  // else in the final pass
   const ColsumPair pr{{m->d2, m->d1},
-                      {S > 1 ? part_2 : g + m->off_b2, S > 1 ? part_1 : g + m->off_b1}};
+                      {S > 1 ? part_2 : g + m->off_b2, S > 1 ? part_1 : g + m->off_b1},
+                      {bias_step ? m->p + m->off_b2 : nullptr, bias_step ? m->p + m->off_b1 : nullptr},
+                      {bias_step ? m->m1 + m->off_b2 : nullptr, bias_step ? m->m1 + m->off_b1 : nullptr},
+                      {bias_step ? m->m2 + m->off_b2 : nullptr, bias_step ? m->m2 + m->off_b1 : nullptr},
+                      fa ? fa->st : AdamScalars{}};
   colsum_kernel<<<dim3((m->H + 63) / 64, S, 2), cs_block, 0, m->stream>>>(pr, rows, m->H);
   if (S > 1) {
-    fin.f[nfin++] = ColsumFinal{part_2, S, m->H, g + m->off_b2};
-    fin.f[nfin++] = ColsumFinal{part_1, S, m->H, g + m->off_b1};
+    fin.f[nfin++] = bias_final(part_2, S, m->H, m->off_b2);
+    fin.f[nfin++] = bias_final(part_1, S, m->H, m->off_b1);
   }
   if (nfin) {
     int n_max = 0;
@@ -1718,8 +1796,11 @@ int gcm_mlp_train(gcm_mlp* m, int epochs, int batch, float lr, uint64_t seed,
   const int64_t n = m->n_data;
   const int nb = (int)(n / batch);     // minibatches per epoch
   // W3 takes its Adam step in the epilogue of its gradient's product when its
-  // range is aligned for the 16-byte paths (adam_kernel skips it)
+  // range is aligned for the 16-byte paths (adam_kernel skips it),
   const bool w3_fused = m->off_w3 % 4 == 0 && (m->off_b3 - m->off_w3) % 4 == 0;
+  // and every parameter takes its step in the backward pass (no Adam launch)
+  // when W2's range is aligned too and the loss head is fused
+  const bool all_fused = w3_fused && m->off_w2 % 4 == 0 && head_fused(m, batch);
   const int n_sampled = (nb + 15) / 16;  // of which the loss is taken: every 16th
   if (loss_out && epochs > m->loss_cap) {
     MLP_HIP(hipStreamSynchronize(m->stream));
@@ -1765,17 +1846,18 @@ int gcm_mlp_train(gcm_mlp* m, int epochs, int batch, float lr, uint64_t seed,
       m->adam_t++;
       const float c1 = 1.0f - std::pow(b1, (float)m->adam_t);
       const float c2 = 1.0f - std::pow(b2, (float)m->adam_t);
-      const AdamStep st{m->p + m->off_w3, m->m1 + m->off_w3, m->m2 + m->off_w3,
-                        m->pb + m->bmap.bw3, lr, b1, b2, eps, c1, c2};
+      const FusedAdam fa{{lr, b1, b2, eps, c1, c2}, all_fused};
       if ((rc = backward(m, batch, nullptr, want ? m->loss_dev + ep : nullptr,
-                         w3_fused ? &st : nullptr)))
+                         w3_fused ? &fa : nullptr)))
         return rc;
-      // Adam on the parameters outside W3 (all of them without the fused step)
-      const int64_t gap = w3_fused ? m->off_b3 - m->off_w3 : 0;
-      adam_kernel<<<grid_for((m->n_params - gap + 3) / 4), 256, 0, m->stream>>>(
-          m->p, m->g, m->m1, m->m2, m->pb, m->bmap, m->n_params, w3_fused ? m->off_w3 : 0, gap,
-          lr, b1, b2, eps, c1, c2);
-      MLP_HIP(hipGetLastError());
+      if (!all_fused) {
+        // Adam on the parameters outside W3 (all of them without W3's step)
+        const int64_t gap = w3_fused ? m->off_b3 - m->off_w3 : 0;
+        adam_kernel<<<grid_for((m->n_params - gap + 3) / 4), 256, 0, m->stream>>>(
+            m->p, m->g, m->m1, m->m2, m->pb, m->bmap, m->n_params, w3_fused ? m->off_w3 : 0,
+            gap, lr, b1, b2, eps, c1, c2);
+        MLP_HIP(hipGetLastError());
+      }
     }
     if (ep + 1 < epochs && (rc = stage(ep + 1))) return rc;
   }
