@@ -537,10 +537,10 @@ def mlp_kernels(args, log, n_in=300, hidden=512, n_out=6000, rows=16384, batch=2
             "large_batch": {"batch": big_batch, "steps": steps_big,
                             "us_per_step": el_big / steps_big * 1e6, "tflops": tflops_big,
                             "frac": tflops_big / MFMA_BF16_PEAK_TFLOPS},
-            "bound": ("256 rows: launch and load latency (14 launches of 3-19 us per "
-                      "5.3-GFLOP step); 4096 rows: the three large GEMMs (369-523 TFLOP/s, "
-                      "177 us) and the memory-bound split-K reductions (with W3's Adam "
-                      "step), Adam, column sums and row gather (77 us)")}
+            "bound": ("256 rows: launch and load latency (13 launches of 3-19 us per "
+                      "5.3-GFLOP step); 4096 rows: the three large GEMMs (363-520 TFLOP/s, "
+                      "179 us) and the memory-bound split-K reductions (with the Adam "
+                      "steps), column sums and row gather (63 us)")}
 
 
 def posterior_agreement(args, log):
